@@ -1,0 +1,182 @@
+/*
+ * mgpcg.h -- C-ABI of the MI355X-native MGPCG Poisson solver (libmgpcg.so).
+ *
+ * The boundary replaces the reference's in-process Python interface
+ *   MGPCGPoissonSolver(dim, res, n_mg_levels, pre_and_post_smoothing,
+ *                      bottom_smoothing, real)              mgpcg.py:6
+ *   .reinitialize(cell_type, strategy)                      mgpcg.py:79-87
+ *   .full_reinitialize(strategy)                            mgpcg.py:89-94
+ *   .solve(max_iters, verbose, rel_tol, abs_tol, eps)       mgpcg.py:150-209
+ *   result in .x                                            mgpcg.py:39
+ * The strategies fill the solver's public fields b, grid_type[l], Adiag[l],
+ * Ax[l] (pressure_project.py:41-70); here they fill caller-owned device
+ * arrays in the same canonical layout (C-order over res >> l, Ax as
+ * [*res_l, dim] array-of-structs, cell types int8 FLUID=0 AIR=1 SOLID=2) and
+ * hand them to mgpcg_load_level / mgpcg_load_rhs.
+ *
+ * Conventions: every function returns 0 on success, -1 on an invalid
+ * argument, -2 on a HIP runtime error; mgpcg_last_error() returns a
+ * thread-local message.  All pointers to field data are device pointers.
+ * A handle is not re-entrant; distinct handles may be used from distinct
+ * threads.  Work is ordered after everything previously enqueued on the
+ * stream given to mgpcg_set_stream (NULL = the legacy default stream) and that stream is
+ * made to wait for the handle's work before the call returns.
+ */
+#ifndef MGPCG_H
+#define MGPCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGPCG_ABI_VERSION 1
+
+/* precision modes */
+#define MGPCG_PLAIN 0 /* FP32 everywhere, FP32 scalars: the reference's arithmetic (G11) */
+#define MGPCG_MIXED 1 /* FP32 V-cycle/Laplacian, FP64 x, dots, scalars, true-residual refinement */
+#define MGPCG_F64 2   /* FP64 everywhere */
+
+/* solve status */
+#define MGPCG_RUNNING 0
+#define MGPCG_CONVERGED 1
+#define MGPCG_MAX_ITERS 2
+#define MGPCG_BREAKDOWN 3 /* alpha or beta not finite */
+
+typedef struct mgpcg_solver mgpcg_solver_t;
+
+typedef struct {
+  int32_t iters;   /* completed non-final loop bodies (mgpcg.py:176-209) */
+  int32_t status;  /* MGPCG_CONVERGED / MGPCG_MAX_ITERS / MGPCG_BREAKDOWN */
+  double r0r0;     /* initial r.r over FLUID cells (mgpcg.py:158-159) */
+  double rr;       /* last r.r tested against tol (mgpcg.py:188-194) */
+  double true_rr;  /* last recomputed |b - A x|^2 (MIXED), -1 if none */
+  double tol;      /* max(abs_tol, rel_tol * r0r0) (mgpcg.py:164) */
+  double alpha, beta;
+} mgpcg_result_t;
+
+/* Replaces MGPCGPoissonSolver.__init__ (mgpcg.py:6-44): allocates the whole
+ * hierarchy (levels res >> l, l < n_levels) in device memory.
+ * n_mg_levels, pre_and_post_smoothing, bottom_smoothing as in mgpcg.py:6;
+ * every res[a] must be divisible by 2^(n_levels-1) (mgpcg.py:23). */
+int mgpcg_create(int dim, const int32_t* res, int n_levels, int pre_and_post_smoothing,
+                 int bottom_smoothing, int precision, mgpcg_solver_t** out);
+int mgpcg_destroy(mgpcg_solver_t* solver);
+
+/* Orders the handle's work after / before the caller's stream (hipStream_t). */
+int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
+
+/* Tuning knobs (no reference counterpart). */
+#define MGPCG_OPT_USE_GRAPH 1 /* replay iterations from a captured hipGraph (default 1) */
+#define MGPCG_OPT_BURST 2     /* iterations enqueued between host state reads (default 4) */
+int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
+
+/* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,
+ * reduction partials}; bytes = device bytes held by the handle. */
+int mgpcg_info(const mgpcg_solver_t* solver, int level, int32_t* out6, int64_t* bytes);
+
+/* Loads one level of the operator from the canonical fields a strategy's
+ * build_A wrote (pressure_project.py:45-64 and siblings) and zeroes that
+ * level's r and z.  Called for l = 0..n_levels-1 after the strategy ran;
+ * together with mgpcg_load_rhs this is the tail of reinitialize /
+ * full_reinitialize (mgpcg.py:79-94). */
+int mgpcg_load_level(mgpcg_solver_t* solver, int level, const int8_t* grid_type,
+                     const float* Adiag, const float* Ax);
+/* Loads b (the strategy's build_b output, mgpcg.py:20). */
+int mgpcg_load_rhs(mgpcg_solver_t* solver, const float* b);
+
+/* init_gridtype (mgpcg.py:46-60) on canonical int8 arrays, asynchronous on
+ * `stream`: coarse = AIR if any child is AIR, else FLUID if any child is
+ * FLUID, else SOLID.  fine_res has dim entries. */
+int mgpcg_coarsen(void* stream, int dim, const int32_t* fine_res, const int8_t* fine,
+                  int8_t* coarse);
+
+/* solve (mgpcg.py:150-209) with the loop, the reductions and every scalar
+ * resident on the device.  max_iters = -1 means no cap.  refine_every is the
+ * true-residual period of MGPCG_MIXED (ignored otherwise).  Blocks until the
+ * solve finished; the result stays in the handle (mgpcg_store_x). */
+int mgpcg_solve(mgpcg_solver_t* solver, int max_iters, double rel_tol, double abs_tol,
+                double eps, int refine_every, mgpcg_result_t* out);
+/* The same solve driven like the reference's host loop: one launch per
+ * kernel and a device->host read after every reduction (mgpcg.py:159, 172,
+ * 180, 189, 202).  The plain-GPU baseline. */
+int mgpcg_solve_hostsync(mgpcg_solver_t* solver, int max_iters, double rel_tol, double abs_tol,
+                         double eps, int refine_every, mgpcg_result_t* out);
+
+/* Copies x (mgpcg.py:39) into a canonical array; exactly one of the two
+ * pointers is non-NULL. */
+int mgpcg_store_x(mgpcg_solver_t* solver, float* x_f32, double* x_f64);
+
+/* Kernel-level access for parity tests (no reference counterpart): move a
+ * level vector between canonical layout (float, or double for MGPCG_F64 and
+ * for x outside MGPCG_PLAIN) and the handle, and run one stage. */
+#define MGPCG_FIELD_R 0
+#define MGPCG_FIELD_Z 1
+#define MGPCG_FIELD_P 2
+#define MGPCG_FIELD_AP 3
+#define MGPCG_FIELD_X 4
+#define MGPCG_OP_SMOOTH 0           /* smooth(level, arg) (mgpcg.py:104-109) */
+#define MGPCG_OP_SMOOTH_FROM_ZERO 1 /* z.fill(0); smooth(level, 0) (mgpcg.py:126-130) */
+#define MGPCG_OP_RESTRICT 2         /* restrict(level) into r[level+1] (mgpcg.py:111-118) */
+#define MGPCG_OP_PROLONGATE 3       /* prolongate(level) (mgpcg.py:120-123) */
+#define MGPCG_OP_VCYCLE 4           /* v_cycle() (mgpcg.py:125-148) */
+#define MGPCG_OP_COMPUTE_AP 5       /* compute_Ap() (mgpcg.py:218-224) */
+int mgpcg_debug_set(mgpcg_solver_t* solver, int field, int level, const void* canon);
+int mgpcg_debug_get(mgpcg_solver_t* solver, int field, int level, void* canon);
+int mgpcg_debug_op(mgpcg_solver_t* solver, int op, int level, int arg);
+
+/* ---------------------------------------------------------------------------
+ * Operator assembly on the device (the strategies' Taichi kernels).  All
+ * fields are canonical C-order float32 device arrays; MAC components
+ * velocity[k], H[k], H_ext[k] have shape res + e_k; n is [*res, dim].
+ * Every kernel writes every cell of its outputs (the reference relies on
+ * initialize() zeroing them, mgpcg.py:63-77).  Asynchronous on `stream`.
+ * ------------------------------------------------------------------------- */
+/* PressureProjectStrategy.build_b_kernel (pressure_project.py:15-39);
+ * c_max is the ghost-fluid clamp (1e3 in the reference, :37). */
+int mgpcg_pressure_build_b(void* stream, int dim, const int32_t* res, const int8_t* cell_type,
+                           const float* const* velocity, const float* phi, double scale_A,
+                           double scale_b, double p0, int ghost_fluid_method, double c_max,
+                           float* b);
+/* PressureProjectStrategy.build_A_kernel (pressure_project.py:45-64) on level
+ * `level` (grid_type, Adiag, Ax have shape res >> level; phi is level 0). */
+int mgpcg_pressure_build_A(void* stream, int dim, const int32_t* res, int level,
+                           const int8_t* grid_type, const float* phi, double scale_A,
+                           int ghost_fluid_method, double c_max, float* Adiag, float* Ax);
+/* MagneticForceStrategy.build_b_kernel (magnetic_force.py:17-39). */
+int mgpcg_magnetic_build_b(void* stream, int dim, const int32_t* res, const int8_t* cell_type,
+                           const float* const* velocity, const float* const* H, double scale_A,
+                           double scale_b, double p0, double mu0, double k, float* b);
+/* calc_volume (volume_control.py:22-31): sum over cells, FP64, written to
+ * *out_device; partials needs ceil(cells / 256) doubles of device memory. */
+int mgpcg_calc_volume(void* stream, int dim, const int32_t* res, const float* phi, double dx,
+                      double* partials, int partials_cap, double* out_device);
+/* add_c (volume_control.py:33-37): b[FLUID] += f32(c). */
+int mgpcg_add_c(void* stream, int dim, const int32_t* res, const int8_t* cell_type, double c,
+                float* b);
+/* PotentialFunctionStrategy (potential_function.py:19-34 / 40-55). */
+int mgpcg_potential_build_b(void* stream, int dim, const int32_t* res, const float* chi,
+                            const float* const* H_ext, float* b);
+int mgpcg_potential_build_A(void* stream, int dim, const int32_t* res, int level, double dx,
+                            const float* chi, float* Adiag, float* Ax);
+/* SurfaceTension.calc_n / calc_kappa (surface_tension.py:115-136) and
+ * SurfaceTensionStrategy (surface_tension.py:23-66 / 71-89) for axis d. */
+int mgpcg_calc_n(void* stream, int dim, const int32_t* res, int d, const float* phi, double dx,
+                 float* n);
+int mgpcg_calc_kappa(void* stream, int dim, const int32_t* res, const float* n, double dx,
+                     float* kappa);
+int mgpcg_surface_build_b(void* stream, int dim, const int32_t* res, int d,
+                          const float* velocity_d, const float* phi, const float* kappa,
+                          const float* n, double dt, double dx, double sigma, float* b);
+int mgpcg_surface_build_A(void* stream, int dim, const int32_t* res, int level, int d,
+                          const float* phi, double dt, double dx, double sigma, float* Adiag,
+                          float* Ax);
+
+const char* mgpcg_last_error(void);
+int mgpcg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGPCG_H */

@@ -1,0 +1,131 @@
+"""ctypes binding of libmgpcg.so (the HIP library; C-ABI in include/mgpcg.h).
+
+torch is imported first so that the library's libamdhip64.so.7 dependency
+resolves to the HIP runtime torch already mapped (one runtime per process).
+There is no fallback: a missing or unloadable library raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede loading the HIP library)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmgpcg.so")
+
+PLAIN, MIXED, F64 = 0, 1, 2
+RUNNING, CONVERGED, MAX_ITERS, BREAKDOWN = 0, 1, 2, 3
+FIELD_R, FIELD_Z, FIELD_P, FIELD_AP, FIELD_X = 0, 1, 2, 3, 4
+OP_SMOOTH, OP_SMOOTH_FROM_ZERO, OP_RESTRICT, OP_PROLONGATE, OP_VCYCLE, OP_COMPUTE_AP = range(6)
+OPT_USE_GRAPH, OPT_BURST = 1, 2
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+class MgpcgResult(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("r0r0", ctypes.c_double),
+        ("rr", ctypes.c_double),
+        ("true_rr", ctypes.c_double),
+        ("tol", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes); every symbol include/mgpcg.h declares
+SIGNATURES = {
+    "mgpcg_create": (_I, [_I, _P, _I, _I, _I, _I, ctypes.POINTER(_P)]),
+    "mgpcg_destroy": (_I, [_P]),
+    "mgpcg_set_stream": (_I, [_P, _P]),
+    "mgpcg_set_option": (_I, [_P, _I, _I]),
+    "mgpcg_info": (_I, [_P, _I, _P, _P]),
+    "mgpcg_load_level": (_I, [_P, _I, _P, _P, _P]),
+    "mgpcg_load_rhs": (_I, [_P, _P]),
+    "mgpcg_coarsen": (_I, [_P, _I, _P, _P, _P]),
+    "mgpcg_solve": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(MgpcgResult)]),
+    "mgpcg_solve_hostsync": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(MgpcgResult)]),
+    "mgpcg_store_x": (_I, [_P, _P, _P]),
+    "mgpcg_debug_set": (_I, [_P, _I, _I, _P]),
+    "mgpcg_debug_get": (_I, [_P, _I, _I, _P]),
+    "mgpcg_debug_op": (_I, [_P, _I, _I, _I]),
+    "mgpcg_pressure_build_b": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _D, _I, _D, _P]),
+    "mgpcg_pressure_build_A": (_I, [_P, _I, _P, _I, _P, _P, _D, _I, _D, _P, _P]),
+    "mgpcg_magnetic_build_b": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _D, _D, _D, _P]),
+    "mgpcg_calc_volume": (_I, [_P, _I, _P, _P, _D, _P, _I, _P]),
+    "mgpcg_add_c": (_I, [_P, _I, _P, _P, _D, _P]),
+    "mgpcg_potential_build_b": (_I, [_P, _I, _P, _P, _P, _P]),
+    "mgpcg_potential_build_A": (_I, [_P, _I, _P, _I, _D, _P, _P, _P]),
+    "mgpcg_calc_n": (_I, [_P, _I, _P, _I, _P, _D, _P]),
+    "mgpcg_calc_kappa": (_I, [_P, _I, _P, _P, _D, _P]),
+    "mgpcg_surface_build_b": (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _D, _D, _D, _P]),
+    "mgpcg_surface_build_A": (_I, [_P, _I, _P, _I, _I, _P, _D, _D, _D, _P, _P]),
+    "mgpcg_last_error": (ctypes.c_char_p, []),
+    "mgpcg_version": (_I, []),
+}
+
+_lib = None
+
+
+class MgpcgError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libmgpcg.so (raises if it is missing: there is no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MgpcgError(
+                f"{LIB_PATH} is missing; build it with `python taichi-ferrofluid_amd/build.py` "
+                "(or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise MgpcgError(load().mgpcg_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(t):
+    """Device (or host) address of a contiguous torch tensor."""
+    if t is None:
+        return None
+    assert t.is_contiguous(), "tensors crossing the C-ABI must be contiguous"
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def ptr_array(tensors):
+    return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+
+
+def int3(vals):
+    v = list(vals) + [0] * (3 - len(vals))
+    return (ctypes.c_int32 * 3)(*v)
+
+
+def stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def hip_runtimes_mapped():
+    """Paths of every libamdhip64 mapped into this process (should be one)."""
+    paths = set()
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64" in line:
+                paths.add(line.split()[-1])
+    return sorted(paths)

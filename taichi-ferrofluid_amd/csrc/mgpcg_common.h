@@ -1,0 +1,156 @@
+// Shared definitions of the MI355X MGPCG library: the colour-split padded
+// level layout, fast integer division, the device-resident CG state and the
+// error plumbing of the C-ABI.
+//
+// Level layout ("half-colour" storage)
+// ------------------------------------
+// A level of shape (n0, n1, n2) (2D grids use the view (1, n0, n1)) is stored
+// as two colour arrays, colour c = (i + j + k) & 1 of the original cell index
+// (mgpcg.py:107).  Each colour array is C-order over
+//     [P0][P1][W],  P0 = n0 + 2 (1 in 2D),  P1 = n1 + 2,  W = (n2 + 3) / 2
+// i.e. one ghost layer on every side (ghost cells are SOLID with zero
+// coefficients and zero vectors, which realises the "out-of-range neighbour
+// contributes 0" convention), and the contiguous axis split by colour: in
+// padded row (i', j') colour c holds padded k' = 2 m + q at slot m, where
+// q = (c + rp + 1) & 1 and rp = (i + j) & 1 is the original row parity.
+// A colour-c cell at flat slot t then finds its other-colour neighbours at
+//     t -+ P1*W  (axis 0, 3D),   t -+ W  (axis 1),   t + q - 1, t + q (axis 2)
+// so a half-sweep streams each colour array contiguously.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MG_FLUID 0
+#define MG_AIR 1
+#define MG_SOLID 2
+
+namespace mg {
+
+// Division by a runtime-invariant divisor (dividend < 2^31), multiply-high form.
+struct FastDiv {
+  uint32_t d, m, s;
+  FastDiv() : d(1), m(0), s(0) {}
+  explicit FastDiv(uint32_t div) : d(div), m(0), s(0) {
+    if (d > 1) {
+      uint32_t l = 0;
+      while ((1u << l) < d) ++l;  // l = ceil(log2 d)
+      uint32_t p = 31 + l;
+      m = (uint32_t)(((1ull << p) + d - 1) / d);
+      s = p - 32;
+    }
+  }
+  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    return d == 1 ? n : (__umulhi(n, m) >> s);
+#else
+    return d == 1 ? n : (uint32_t)(((uint64_t)n * m) >> 32) >> s;
+#endif
+  }
+};
+
+struct Geo {
+  int dim;         // 2 or 3
+  int n0, n1, n2;  // interior dims of the 3D view (n0 = 1 in 2D)
+  int P0, P1, W;   // padded dims of one colour array
+  int sx;          // axis-0 stride (P1 * W)
+  int size;        // P0 * P1 * W
+  int ncells;      // n0 * n1 * n2
+  FastDiv divW, divP1;
+};
+
+inline Geo make_geo(int dim, const int* n) {
+  Geo g;
+  g.dim = dim;
+  if (dim == 2) {
+    g.n0 = 1;
+    g.n1 = n[0];
+    g.n2 = n[1];
+    g.P0 = 1;
+  } else {
+    g.n0 = n[0];
+    g.n1 = n[1];
+    g.n2 = n[2];
+    g.P0 = g.n0 + 2;
+  }
+  g.P1 = g.n1 + 2;
+  g.W = (g.n2 + 3) / 2;
+  g.sx = g.P1 * g.W;
+  g.size = g.P0 * g.P1 * g.W;
+  g.ncells = g.n0 * g.n1 * g.n2;
+  g.divW = FastDiv((uint32_t)g.W);
+  g.divP1 = FastDiv((uint32_t)g.P1);
+  return g;
+}
+
+// Decoded slot: padded row (ip, jp), slot m, k' parity q, interior flag and
+// original coordinates (i, j, k) of the 3D view.
+struct Slot {
+  int ip, jp, m, q, kp;
+  int i, j, k;
+  bool interior;
+};
+
+__host__ __device__ __forceinline__ Slot decode(const Geo& g, int t, int colour) {
+  Slot s;
+  uint32_t row = g.divW.div((uint32_t)t);
+  s.m = t - (int)row * g.W;
+  uint32_t ipu = g.divP1.div(row);
+  s.ip = (int)ipu;
+  s.jp = (int)row - s.ip * g.P1;
+  // original indices (2D: the view's axis 0 has no ghost layer)
+  s.i = (g.dim == 3) ? s.ip - 1 : 0;
+  s.j = s.jp - 1;
+  int rp = (s.i + s.j) & 1;
+  s.q = (colour + rp + 1) & 1;
+  s.kp = 2 * s.m + s.q;
+  s.k = s.kp - 1;
+  s.interior = (s.k >= 0) && (s.k < g.n2) && (s.j >= 0) && (s.j < g.n1) && (s.i >= 0) &&
+               (s.i < g.n0);
+  return s;
+}
+
+// Flat slot of original cell (i, j, k) (3D view) and its colour.
+__host__ __device__ __forceinline__ int slot_of(const Geo& g, int i, int j, int k) {
+  int ip = (g.dim == 3) ? i + 1 : 0;
+  return (ip * g.P1 + (j + 1)) * g.W + ((k + 1) >> 1);
+}
+__host__ __device__ __forceinline__ int colour_of(int i, int j, int k) { return (i + j + k) & 1; }
+
+// Device-resident CG state of one solve (all scalars of mgpcg.py:150-209).
+struct CGState {
+  double r0r0, tol, zr_old, zr_new, pAp, alpha, beta, rr, true_rr;
+  double rel_tol, abs_tol, eps;
+  float alpha32, beta32;
+  int iter, done, status, since_refine;
+  int max_iters, refine_every, precision, pending_true;
+  uint32_t counter;  // last-block-done ticket of the current reduction
+  int pad[3];
+};
+
+enum Precision { PLAIN = 0, MIXED = 1, F64 = 2 };
+enum Status { RUNNING = 0, CONVERGED = 1, MAX_ITERS = 2, BREAKDOWN = 3 };
+
+}  // namespace mg
+
+// ---------------------------------------------------------------------------
+// error plumbing of the C-ABI
+// ---------------------------------------------------------------------------
+void mg_set_error(const char* fmt, ...);
+
+#define MG_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (call);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      mg_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__,    \
+                   __LINE__);                                                          \
+      return -2;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+#define MG_REQUIRE(cond, ...)  \
+  do {                         \
+    if (!(cond)) {             \
+      mg_set_error(__VA_ARGS__); \
+      return -1;               \
+    }                          \
+  } while (0)

@@ -1,0 +1,206 @@
+"""MGPCGPoissonSolver: the reference interface (mgpcg.py:4-238) over libmgpcg.
+
+Public fields keep the reference's names and canonical layout, as torch
+tensors on the GPU: ``b``, ``x`` (float32 [*res]), ``grid_type[l]`` (int8),
+``Adiag[l]`` (float32) and ``Ax[l]`` (float32 [*res_l, dim]) with
+``res_l = res // 2**l`` (mgpcg.py:20-40).  Strategies fill them exactly as the
+reference strategies fill the Taichi fields (``build_b(solver)``,
+``build_A(solver, level)``, pressure_project.py:41-70), so the strategy
+modules of this package call the solver unchanged.
+
+Everything numeric runs in the HIP library: ``solve`` loads the hierarchy
+into its colour-split layout and runs the whole PCG loop on the device.
+There is no CPU path.
+"""
+import ctypes
+import time
+
+import torch
+
+import _lib
+import utils
+
+PRECISIONS = {"plain": _lib.PLAIN, "mixed": _lib.MIXED, "f64": _lib.F64}
+
+
+def level_shape(res, l):
+    return [n // (2 ** l) for n in res]
+
+
+class MGPCGPoissonSolver:
+    """mgpcg.py:5-44.
+
+    ``real=float`` is the reference's FP32 (Taichi default_fp).  ``precision``
+    selects the arithmetic: "plain" (FP32 everywhere, as the reference),
+    "mixed" (FP32 V-cycle and Laplacian, FP64 x, dot products, CG scalars and
+    a true-residual recompute every ``refine_every`` iterations) or "f64".
+    """
+
+    def __init__(self, dim, res, n_mg_levels=4, pre_and_post_smoothing=2, bottom_smoothing=50,
+                 real=float, precision="plain", refine_every=10, device=None):
+        self.FLUID = utils.FLUID
+        self.SOLID = utils.SOLID
+        self.AIR = utils.AIR
+
+        self.dim = dim
+        self.res = list(res)
+        self.n_mg_levels = n_mg_levels
+        self.pre_and_post_smoothing = pre_and_post_smoothing
+        self.bottom_smoothing = bottom_smoothing
+        self.real = real
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.precision = precision
+        self.refine_every = refine_every
+        if len(self.res) != dim:
+            raise ValueError("len(res) must equal dim")
+        for n in self.res:
+            if n % (2 ** (n_mg_levels - 1)):
+                raise ValueError(f"res {self.res} not divisible by 2^(n_mg_levels-1)")
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+
+        f32 = dict(dtype=torch.float32, device=self.device)
+        self.b = torch.zeros(self.res, **f32)
+        self.x = torch.zeros(self.res, **f32)
+        self.grid_type = [torch.zeros(level_shape(self.res, l), dtype=torch.int8,
+                                      device=self.device) for l in range(n_mg_levels)]
+        self.Adiag = [torch.zeros(level_shape(self.res, l), **f32) for l in range(n_mg_levels)]
+        self.Ax = [torch.zeros(level_shape(self.res, l) + [dim], **f32)
+                   for l in range(n_mg_levels)]
+        self.x64 = None
+        self.last = None  # result dict of the last solve
+
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.mgpcg_create(dim, _lib.int3(self.res), n_mg_levels,
+                                          pre_and_post_smoothing, bottom_smoothing,
+                                          PRECISIONS[precision], ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.mgpcg_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- assembly --------------------------------------------------------------
+    def initialize(self):
+        """mgpcg.py:63-77: zero b, x and every level's grid_type/Adiag/Ax."""
+        self.b.zero_()
+        self.x.zero_()
+        for l in range(self.n_mg_levels):
+            self.grid_type[l].zero_()
+            self.Adiag[l].zero_()
+            self.Ax[l].zero_()
+
+    def _coarsen(self, l):
+        """init_gridtype (mgpcg.py:46-60) from level l-1 into level l."""
+        _lib.check(self._lib.mgpcg_coarsen(_lib.stream_handle(self.device), self.dim,
+                                           _lib.int3(level_shape(self.res, l - 1)),
+                                           _lib.ptr(self.grid_type[l - 1]),
+                                           _lib.ptr(self.grid_type[l])))
+
+    def reinitialize(self, cell_type, strategy):
+        """mgpcg.py:79-87.  ``cell_type`` is an int tensor of shape res."""
+        if not getattr(strategy, "writes_all_cells", False):
+            self.initialize()
+        self.grid_type[0].copy_(cell_type.to(self.grid_type[0].device))  # i32 -> i8
+        strategy.build_b(self)
+        strategy.build_A(self, 0)
+        for l in range(1, self.n_mg_levels):
+            self._coarsen(l)
+            strategy.build_A(self, l)
+
+    def full_reinitialize(self, strategy):
+        """mgpcg.py:89-94: every level all FLUID."""
+        if not getattr(strategy, "writes_all_cells", False):
+            self.initialize()
+        strategy.build_b(self)
+        for l in range(self.n_mg_levels):
+            self.grid_type[l].fill_(utils.FLUID)
+            strategy.build_A(self, l)
+
+    def load(self):
+        """Hand the canonical fields to the device hierarchy."""
+        h = self._h
+        _lib.check(self._lib.mgpcg_set_stream(h, _lib.stream_handle(self.device)))
+        for l in range(self.n_mg_levels):
+            _lib.check(self._lib.mgpcg_load_level(h, l, _lib.ptr(self.grid_type[l]),
+                                                  _lib.ptr(self.Adiag[l]), _lib.ptr(self.Ax[l])))
+        _lib.check(self._lib.mgpcg_load_rhs(h, _lib.ptr(self.b)))
+
+    # -- solve -------------------------------------------------------------------
+    def solve(self, max_iters=-1, verbose=False, rel_tol=1e-12, abs_tol=1e-14, eps=1e-12):
+        """mgpcg.py:150-209; the result is left in ``self.x`` (and ``self.x64``).
+
+        The loop runs on the device, so ``verbose`` prints a summary after the
+        solve instead of one line per iteration (mgpcg.py:161-162, 191-192).
+        """
+        t0 = time.perf_counter()
+        self.load()
+        out = _lib.MgpcgResult()
+        _lib.check(self._lib.mgpcg_solve(self._h, int(max_iters), float(rel_tol), float(abs_tol),
+                                         float(eps), int(self.refine_every), ctypes.byref(out)))
+        self._store_x()
+        self.last = out.as_dict()
+        self.last["seconds"] = time.perf_counter() - t0
+        if verbose:
+            print(f"init rtr = {self.last['r0r0']}")
+            print(f"iters {self.last['iters']}, |residual|_2={self.last['rr'] ** 0.5}, "
+                  f"status {self.last['status']}")
+
+    def solve_hostsync(self, max_iters=-1, rel_tol=1e-12, abs_tol=1e-14, eps=1e-12):
+        """The same solve driven like the reference's host loop (3 syncs per iteration)."""
+        self.load()
+        out = _lib.MgpcgResult()
+        _lib.check(self._lib.mgpcg_solve_hostsync(self._h, int(max_iters), float(rel_tol),
+                                                  float(abs_tol), float(eps),
+                                                  int(self.refine_every), ctypes.byref(out)))
+        self._store_x()
+        self.last = out.as_dict()
+        return self.last
+
+    def _store_x(self):
+        _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
+        _lib.check(self._lib.mgpcg_store_x(self._h, _lib.ptr(self.x), None))
+        if self.precision != "plain":
+            if self.x64 is None:
+                self.x64 = torch.zeros(self.res, dtype=torch.float64, device=self.device)
+            _lib.check(self._lib.mgpcg_store_x(self._h, None, _lib.ptr(self.x64)))
+
+    def set_option(self, option, value):
+        _lib.check(self._lib.mgpcg_set_option(self._h, option, value))
+
+    def info(self, level=0):
+        out = (ctypes.c_int32 * 6)()
+        nbytes = ctypes.c_int64()
+        _lib.check(self._lib.mgpcg_info(self._h, level, out, ctypes.byref(nbytes)))
+        return {"P0": out[0], "P1": out[1], "W": out[2], "slots": out[3], "cells": out[4],
+                "partials": out[5], "device_bytes": nbytes.value}
+
+    # -- kernel-level access (parity tests) --------------------------------------
+    def _vec_dtype(self, field):
+        if self.precision == "f64" or (field == _lib.FIELD_X and self.precision != "plain"):
+            return torch.float64
+        return torch.float32
+
+    def debug_set(self, field, level, t):
+        t = t.to(device=self.device, dtype=self._vec_dtype(field)).contiguous()
+        _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
+        _lib.check(self._lib.mgpcg_debug_set(self._h, field, level, _lib.ptr(t)))
+        torch.cuda.synchronize(self.device)
+
+    def debug_get(self, field, level):
+        out = torch.zeros(level_shape(self.res, level), dtype=self._vec_dtype(field),
+                          device=self.device)
+        _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
+        _lib.check(self._lib.mgpcg_debug_get(self._h, field, level, _lib.ptr(out)))
+        torch.cuda.synchronize(self.device)
+        return out
+
+    def debug_op(self, op, level=0, arg=0):
+        _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
+        _lib.check(self._lib.mgpcg_debug_op(self._h, op, level, arg))

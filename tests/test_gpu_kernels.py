@@ -1,0 +1,189 @@
+"""Kernel-level parity of the HIP library against the oracle (GPU).
+
+Each stage of the V-cycle and the CG operator runs once through the C-ABI
+(mgpcg_debug_set / _op / _get) on seeded inputs and is compared with the
+oracle's restatement of the same reference kernel.  FP32 results must be bit
+identical: both sides use the same operation order without contraction.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import random_hierarchy
+from oracle import core
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ((32, 24), 3),       # 2D
+    ((16, 16, 16), 3),   # 3D
+    ((48, 32, 16), 5),   # 3D, odd and length-1 coarse axes (3 x 2 x 1)
+    ((24, 40, 8), 4),    # 3D, odd coarse axes (3 x 5 x 1)
+]
+
+
+def _solver(h, precision="plain"):
+    import mgpcg
+
+    res = list(h["gt"][0].shape)
+    dim = len(res)
+    L = len(h["gt"])
+    s = mgpcg.MGPCGPoissonSolver(dim, res, L, 2, 10, precision=precision, device="cuda:0")
+    for l in range(L):
+        s.grid_type[l].copy_(torch.from_numpy(h["gt"][l]))
+        s.Adiag[l].copy_(torch.from_numpy(h["Adiag"][l].astype(np.float32)))
+        s.Ax[l].copy_(torch.from_numpy(h["Ax"][l].astype(np.float32)))
+    s.load()
+    return s
+
+
+def _get(s, field, level):
+    return s.debug_get(field, level).cpu().numpy()
+
+
+def _case_id(c):
+    return "x".join(map(str, c[0])) + f"_L{c[1]}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_coarsen_bit_exact(case):
+    import _lib
+
+    res, _ = case
+    rng = np.random.default_rng(1)
+    fine = rng.integers(0, 3, res).astype(np.int8)
+    t = torch.from_numpy(fine).cuda()
+    out = torch.empty([n // 2 for n in res], dtype=torch.int8, device="cuda")
+    lib = _lib.load()
+    _lib.check(lib.mgpcg_coarsen(_lib.stream_handle(), len(res), _lib.int3(res), _lib.ptr(t),
+                                 _lib.ptr(out)))
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), core.coarsen(fine))
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+@pytest.mark.parametrize("phase", [0, 1])
+def test_smooth_bit_exact(case, phase):
+    import _lib
+
+    res, L = case
+    h = random_hierarchy(res, L, seed=2)
+    s = _solver(h)
+    rng = np.random.default_rng(3)
+    for l in range(L):
+        shape = h["gt"][l].shape
+        r = rng.standard_normal(shape).astype(np.float32)
+        z = rng.standard_normal(shape).astype(np.float32)
+        s.debug_set(_lib.FIELD_R, l, torch.from_numpy(r))
+        s.debug_set(_lib.FIELD_Z, l, torch.from_numpy(z))
+        s.debug_op(_lib.OP_SMOOTH, l, phase)
+        got = _get(s, _lib.FIELD_Z, l)
+        core.smooth(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z, phase)
+        assert np.array_equal(got, z), f"level {l}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_smooth_from_zero(case):
+    import _lib
+
+    res, L = case
+    h = random_hierarchy(res, L, seed=4)
+    s = _solver(h)
+    rng = np.random.default_rng(5)
+    for l in range(L):
+        shape = h["gt"][l].shape
+        r = rng.standard_normal(shape).astype(np.float32)
+        s.debug_set(_lib.FIELD_R, l, torch.from_numpy(r))
+        s.debug_set(_lib.FIELD_Z, l, torch.from_numpy(rng.standard_normal(shape).astype(np.float32)))
+        s.debug_op(_lib.OP_SMOOTH_FROM_ZERO, l, 0)
+        got = _get(s, _lib.FIELD_Z, l)
+        z = np.zeros(shape, np.float32)
+        core.smooth(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z, 0)
+        assert np.array_equal(got, z), f"level {l}"  # == treats -0 and +0 alike
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_restrict_bit_exact(case):
+    import _lib
+
+    res, L = case
+    h = random_hierarchy(res, L, seed=6)
+    s = _solver(h)
+    rng = np.random.default_rng(7)
+    for l in range(L - 1):
+        shape = h["gt"][l].shape
+        r = rng.standard_normal(shape).astype(np.float32)
+        z = rng.standard_normal(shape).astype(np.float32)
+        s.debug_set(_lib.FIELD_R, l, torch.from_numpy(r))
+        s.debug_set(_lib.FIELD_Z, l, torch.from_numpy(z))
+        s.debug_op(_lib.OP_RESTRICT, l, 0)
+        got = _get(s, _lib.FIELD_R, l + 1)
+        ref = core.restrict(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z)
+        assert np.array_equal(got, ref), f"level {l}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_prolongate_bit_exact(case):
+    import _lib
+
+    res, L = case
+    h = random_hierarchy(res, L, seed=8)
+    s = _solver(h)
+    rng = np.random.default_rng(9)
+    for l in range(L - 1):
+        shape = h["gt"][l].shape
+        cshape = h["gt"][l + 1].shape
+        z = rng.standard_normal(shape).astype(np.float32)
+        zc = rng.standard_normal(cshape).astype(np.float32)
+        s.debug_set(_lib.FIELD_Z, l, torch.from_numpy(z))
+        s.debug_set(_lib.FIELD_Z, l + 1, torch.from_numpy(zc))
+        s.debug_op(_lib.OP_PROLONGATE, l, 0)
+        got = _get(s, _lib.FIELD_Z, l)
+        core.prolongate(z, zc)
+        assert np.array_equal(got, z), f"level {l}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_compute_Ap_bit_exact(case):
+    import _lib
+
+    res, L = case
+    h = random_hierarchy(res, L, seed=10)
+    s = _solver(h)
+    p = np.random.default_rng(11).standard_normal(res).astype(np.float32)
+    s.debug_set(_lib.FIELD_P, 0, torch.from_numpy(p))
+    s.debug_set(_lib.FIELD_AP, 0, torch.zeros(res))
+    s.debug_op(_lib.OP_COMPUTE_AP, 0, 0)
+    got = _get(s, _lib.FIELD_AP, 0)
+    ref = core.compute_Ap(h["gt"][0], h["Adiag"][0], h["Ax"][0], p)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_vcycle_bit_exact(case):
+    import _lib
+
+    res, L = case
+    h = random_hierarchy(res, L, seed=12)
+    s = _solver(h)
+    r0 = np.random.default_rng(13).standard_normal(res).astype(np.float32)
+    s.debug_set(_lib.FIELD_R, 0, torch.from_numpy(r0))
+    s.debug_op(_lib.OP_VCYCLE, 0, 0)
+    got = _get(s, _lib.FIELD_Z, 0)
+    ref, _, _ = core.v_cycle(h, r0, nu=2, bottom=10)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("case", CASES[:2], ids=_case_id)
+def test_vcycle_f64_matches_oracle(case):
+    import _lib
+
+    res, L = case
+    h = core.to_f64(random_hierarchy(res, L, seed=14))
+    s = _solver(h, precision="f64")
+    r0 = np.random.default_rng(15).standard_normal(res)
+    s.debug_set(_lib.FIELD_R, 0, torch.from_numpy(r0))
+    s.debug_op(_lib.OP_VCYCLE, 0, 0)
+    got = _get(s, _lib.FIELD_Z, 0)
+    ref, _, _ = core.v_cycle(h, r0, nu=2, bottom=10)
+    assert np.array_equal(got, ref)
