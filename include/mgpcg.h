@@ -67,13 +67,15 @@ int mgpcg_destroy(mgpcg_solver_t* solver);
 /* Orders the handle's work after / before the caller's stream (hipStream_t). */
 int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
 
-/* Tuning knobs (no reference counterpart). */
-#define MGPCG_OPT_USE_GRAPH 1 /* replay iterations from a captured hipGraph (default 1) */
-#define MGPCG_OPT_BURST 2     /* iterations enqueued between host state reads (default 4) */
+/* Process-wide tuning knobs (no reference counterpart; `solver` is unused). */
+#define MGPCG_OPT_COARSE_MAX_SLOTS 3 /* levels with at most this many slots per colour are
+                                        solved by one workgroup per system (default 4096);
+                                        applies to handles created afterwards */
+#define MGPCG_OPT_MAX_GRID 4         /* workgroups of a persistent stage launch (default 2048) */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,
- * reduction partials}; bytes = device bytes held by the handle. */
+ * first coarse level}; bytes = device bytes held by the handle. */
 int mgpcg_info(const mgpcg_solver_t* solver, int level, int32_t* out6, int64_t* bytes);
 
 /* Loads one level of the operator from the canonical fields a strategy's
@@ -108,23 +110,66 @@ int mgpcg_solve_hostsync(mgpcg_solver_t* solver, int max_iters, double rel_tol, 
  * pointers is non-NULL. */
 int mgpcg_store_x(mgpcg_solver_t* solver, float* x_f32, double* x_f64);
 
+/* ---------------------------------------------------------------------------
+ * Batched solves (no reference counterpart: the reference solves one system
+ * at a time).  Every stage of an iteration is one persistent kernel launch
+ * over all systems of the batch, which may have different resolutions but
+ * share dim, precision, n_levels and the smoothing counts.  results[i]
+ * belongs to systems[i]; each system's x is as after mgpcg_solve.
+ * ------------------------------------------------------------------------- */
+typedef struct mgpcg_batch mgpcg_batch_t;
+int mgpcg_batch_create(mgpcg_batch_t** out);
+int mgpcg_batch_destroy(mgpcg_batch_t* batch);
+int mgpcg_batch_set_stream(mgpcg_batch_t* batch, void* stream);
+int mgpcg_batch_solve(mgpcg_batch_t* batch, mgpcg_solver_t* const* systems, int n,
+                      int max_iters, double rel_tol, double abs_tol, double eps,
+                      int refine_every, mgpcg_result_t* results);
+
+/* ---------------------------------------------------------------------------
+ * Server: continuous batching.  mgpcg_server_submit queues an assembled
+ * system; each mgpcg_server_step admits the queued systems at the iteration
+ * boundary, enqueues one iteration for every member and retires the members
+ * whose state (read one iteration behind) says they are done; their results
+ * are returned by mgpcg_server_poll with the submit tag.  A system must not
+ * be modified between submit and its result.  Admission policy (HBM budget)
+ * is the caller's: it decides when to create and submit systems.
+ * ------------------------------------------------------------------------- */
+typedef struct mgpcg_server mgpcg_server_t;
+int mgpcg_server_create(int max_iters, double rel_tol, double abs_tol, double eps,
+                        int refine_every, mgpcg_server_t** out);
+int mgpcg_server_destroy(mgpcg_server_t* server);
+int mgpcg_server_set_stream(mgpcg_server_t* server, void* stream);
+int mgpcg_server_submit(mgpcg_server_t* server, mgpcg_solver_t* system, int64_t tag);
+int mgpcg_server_step(mgpcg_server_t* server, int* n_active);
+int mgpcg_server_poll(mgpcg_server_t* server, int64_t* tags, mgpcg_result_t* results, int cap,
+                      int* n_out);
+int mgpcg_server_pending(const mgpcg_server_t* server, int* n_pending, int* n_active,
+                         int* inflight);
+
 /* Kernel-level access for parity tests (no reference counterpart): move a
  * level vector between canonical layout (float, or double for MGPCG_F64 and
  * for x outside MGPCG_PLAIN) and the handle, and run one stage. */
 #define MGPCG_FIELD_R 0
 #define MGPCG_FIELD_Z 1
-#define MGPCG_FIELD_P 2
+#define MGPCG_FIELD_P 2      /* p buffer of even iterations */
 #define MGPCG_FIELD_AP 3
 #define MGPCG_FIELD_X 4
-#define MGPCG_OP_SMOOTH 0           /* smooth(level, arg) (mgpcg.py:104-109) */
-#define MGPCG_OP_SMOOTH_FROM_ZERO 1 /* z.fill(0); smooth(level, 0) (mgpcg.py:126-130) */
-#define MGPCG_OP_RESTRICT 2         /* restrict(level) into r[level+1] (mgpcg.py:111-118) */
-#define MGPCG_OP_PROLONGATE 3       /* prolongate(level) (mgpcg.py:120-123) */
-#define MGPCG_OP_VCYCLE 4           /* v_cycle() (mgpcg.py:125-148) */
-#define MGPCG_OP_COMPUTE_AP 5       /* compute_Ap() (mgpcg.py:218-224) */
+#define MGPCG_FIELD_P_NEXT 5 /* p buffer of odd iterations */
+#define MGPCG_OP_SMOOTH 0                /* smooth(level, arg) (mgpcg.py:104-109) */
+#define MGPCG_OP_SMOOTH_PAIR_FROM_ZERO 1 /* z.fill(0); smooth(level,0); smooth(level,1) */
+#define MGPCG_OP_RESTRICT 2              /* restrict(level) into r[level+1] (mgpcg.py:111-118) */
+#define MGPCG_OP_PROLONGATE 3            /* prolongate(level) (mgpcg.py:120-123) */
+#define MGPCG_OP_VCYCLE 4                /* v_cycle() (mgpcg.py:125-148) */
+#define MGPCG_OP_COMPUTE_AP 5 /* p_next = z + 0 * p; Ap = A p_next (mgpcg.py:234-238, 218-224) */
+#define MGPCG_OP_PROLONG_SMOOTH1 6 /* prolongate(level); smooth(level, 1), FLUID cells */
 int mgpcg_debug_set(mgpcg_solver_t* solver, int field, int level, const void* canon);
 int mgpcg_debug_get(mgpcg_solver_t* solver, int field, int level, void* canon);
 int mgpcg_debug_op(mgpcg_solver_t* solver, int op, int level, int arg);
+/* Average device time of one launch of a stage (SMOOTH, RESTRICT, PROLONGATE
+ * or VCYCLE), measured with HIP events around `reps` back-to-back launches
+ * on the handle's stream (no reference counterpart: measurement only). */
+int mgpcg_time_op(mgpcg_solver_t* solver, int op, int level, int arg, int reps,
+                  double* ms_per_launch);
 
 /* ---------------------------------------------------------------------------
  * Operator assembly on the device (the strategies' Taichi kernels).  All

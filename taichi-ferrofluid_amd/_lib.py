@@ -14,9 +14,10 @@ LIB_PATH = os.path.join(HERE, "libmgpcg.so")
 
 PLAIN, MIXED, F64 = 0, 1, 2
 RUNNING, CONVERGED, MAX_ITERS, BREAKDOWN = 0, 1, 2, 3
-FIELD_R, FIELD_Z, FIELD_P, FIELD_AP, FIELD_X = 0, 1, 2, 3, 4
-OP_SMOOTH, OP_SMOOTH_FROM_ZERO, OP_RESTRICT, OP_PROLONGATE, OP_VCYCLE, OP_COMPUTE_AP = range(6)
-OPT_USE_GRAPH, OPT_BURST = 1, 2
+FIELD_R, FIELD_Z, FIELD_P, FIELD_AP, FIELD_X, FIELD_P_NEXT = 0, 1, 2, 3, 4, 5
+(OP_SMOOTH, OP_SMOOTH_PAIR_FROM_ZERO, OP_RESTRICT, OP_PROLONGATE, OP_VCYCLE, OP_COMPUTE_AP,
+ OP_PROLONG_SMOOTH1) = range(7)
+OPT_COARSE_MAX_SLOTS, OPT_MAX_GRID = 3, 4
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -55,6 +56,7 @@ SIGNATURES = {
     "mgpcg_debug_set": (_I, [_P, _I, _I, _P]),
     "mgpcg_debug_get": (_I, [_P, _I, _I, _P]),
     "mgpcg_debug_op": (_I, [_P, _I, _I, _I]),
+    "mgpcg_time_op": (_I, [_P, _I, _I, _I, _I, ctypes.POINTER(_D)]),
     "mgpcg_pressure_build_b": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _D, _I, _D, _P]),
     "mgpcg_pressure_build_A": (_I, [_P, _I, _P, _I, _P, _P, _D, _I, _D, _P, _P]),
     "mgpcg_magnetic_build_b": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _D, _D, _D, _P]),
@@ -66,6 +68,18 @@ SIGNATURES = {
     "mgpcg_calc_kappa": (_I, [_P, _I, _P, _P, _D, _P]),
     "mgpcg_surface_build_b": (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _D, _D, _D, _P]),
     "mgpcg_surface_build_A": (_I, [_P, _I, _P, _I, _I, _P, _D, _D, _D, _P, _P]),
+    "mgpcg_batch_create": (_I, [ctypes.POINTER(_P)]),
+    "mgpcg_batch_destroy": (_I, [_P]),
+    "mgpcg_batch_set_stream": (_I, [_P, _P]),
+    "mgpcg_batch_solve": (_I, [_P, _P, _I, _I, _D, _D, _D, _I, _P]),
+    "mgpcg_server_create": (_I, [_I, _D, _D, _D, _I, ctypes.POINTER(_P)]),
+    "mgpcg_server_destroy": (_I, [_P]),
+    "mgpcg_server_set_stream": (_I, [_P, _P]),
+    "mgpcg_server_submit": (_I, [_P, _P, ctypes.c_int64]),
+    "mgpcg_server_step": (_I, [_P, ctypes.POINTER(_I)]),
+    "mgpcg_server_poll": (_I, [_P, _P, _P, _I, ctypes.POINTER(_I)]),
+    "mgpcg_server_pending": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                  ctypes.POINTER(_I)]),
     "mgpcg_last_error": (ctypes.c_char_p, []),
     "mgpcg_version": (_I, []),
 }

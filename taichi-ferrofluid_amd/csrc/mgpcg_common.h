@@ -123,7 +123,7 @@ struct CGState {
   float alpha32, beta32;
   int iter, done, status, since_refine;
   int max_iters, refine_every, precision, pending_true;
-  uint32_t counter;  // last-block-done ticket of the current reduction
+  int phase;  // 0: before the loop (r = b, first V-cycle, first z.r), 1: in the loop
   int pad[3];
 };
 

@@ -1,14 +1,32 @@
 // HIP kernels of the MGPCG solver for gfx950 (MI355X), colour-split padded
-// layout (see mgpcg_common.h).  Every kernel here is memory bound; none uses
-// MFMA.  Floating-point operation order follows mgpcg.py exactly and the
-// library is compiled with -ffp-contract=off so that FP32 results are
-// reproducible bit for bit against a CPU restatement with the same order.
+// layout (see mgpcg_common.h).  Every kernel is memory bound; none uses MFMA.
+//
+// Batching and persistence: every stage kernel serves a whole batch of
+// systems (of possibly different shapes) in one launch.  The host builds,
+// per stage, an item table {descriptor, first virtual block}; a launch has a
+// fixed number of workgroups that stride over the virtual blocks, each XCD
+// taking a contiguous range of them so stencil neighbours share its L2.
+// Systems that are done cost no dispatches.
+//
+// Exactness: floating-point operation order follows mgpcg.py and the library
+// is built with -ffp-contract=off, so FP32 values on FLUID cells reproduce a
+// CPU restatement bit for bit.  Fusions used here change no FLUID value:
+//  * the first half-sweep from z = 0 is recomputed on the fly inside the
+//    second one (z0 = r0 / Adiag0);
+//  * the prolongation (mgpcg.py:120-123) is folded into the first post-
+//    smoothing half-sweep: colour-1 cells read z0 + z_coarse[parent]; the
+//    colour-0 FLUID values are rewritten by the next half-sweep without
+//    reading their old value, so no prolonged value is ever stored.  Non-FLUID
+//    z stays unprolonged; it is inert (every coupling to it is 0).
+//  * update_p (mgpcg.py:234-238) is fused into compute_Ap with p double-
+//    buffered by iteration parity; reduce(z, r) into the last half-sweep.
 #pragma once
 #include "mgpcg_common.h"
 
 namespace mg {
 
 constexpr int BLOCK = 256;
+constexpr int COARSE_THREADS = 1024;
 
 template <typename T>
 struct LevelPtrs {
@@ -19,12 +37,75 @@ struct LevelPtrs {
   T* z[2];
 };
 
-// blocks are dispatched round-robin over the 8 XCDs; hand each XCD a
-// contiguous range of logical blocks so stencil neighbours share its L2.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-  int q = nb >> 3, rem = nb & 7;
-  int xcd = b & 7, local = b >> 3;
-  return (xcd < rem) ? xcd * (q + 1) + local : rem * (q + 1) + (xcd - rem) * q + local;
+// One level of one system.
+template <typename T>
+struct LevelDesc {
+  Geo g, gc;          // this level, next coarser level
+  LevelPtrs<T> P, C;  // this level, next coarser level
+  CGState* st;
+};
+
+// Level 0 of one system and its CG vectors.
+template <typename T, typename TX>
+struct SysDesc {
+  Geo g;
+  LevelPtrs<T> P;
+  T* b[2];
+  TX* x[2];
+  T* p[2][2];  // [iteration parity][colour]
+  T* Ap[2];
+  CGState* st;
+  double* partials;  // one per virtual block of the level-0 CG kernels
+  int nparts;
+  int coarse_from;  // first level handled by the single-workgroup coarse kernel
+  int L;
+  int desc0;  // index of this system's level-0 LevelDesc
+};
+
+struct Item {
+  int desc;    // LevelDesc / SysDesc index
+  int block0;  // first virtual block of this item
+};
+
+struct Stage {
+  const Item* items;
+  int nitems;
+  int nblocks;  // total virtual blocks
+};
+
+// ---------------------------------------------------------------------------
+// persistent launch helpers
+// ---------------------------------------------------------------------------
+// Virtual blocks handled by this workgroup.  Workgroups are dispatched
+// round-robin over the 8 XCDs, so group x = blockIdx % ng (ng = min(8, grid))
+// lives on one XCD; it takes the contiguous range [x*per, (x+1)*per) of
+// virtual blocks and its workgroups stride through it.
+struct VBIter {
+  int vb, end, step;
+};
+__device__ __forceinline__ VBIter vb_begin(int nblocks) {
+  const int G = gridDim.x;
+  const int ng = G < 8 ? G : 8;
+  const int grp = blockIdx.x % ng;
+  const int nx = (G - grp + ng - 1) / ng;  // workgroups in this group
+  const int j = blockIdx.x / ng;
+  const int per = (nblocks + ng - 1) / ng;
+  VBIter it;
+  const int start = grp * per;
+  it.end = min(nblocks, start + per);
+  it.vb = start + j;
+  it.step = nx;
+  return it;
+}
+
+__device__ __forceinline__ int find_item(const Item* items, int n, int vb) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= vb) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -36,87 +117,24 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Returns the block total in every thread.
+template <int NT>
 __device__ __forceinline__ double block_sum(double v) {
-  __shared__ double sm[BLOCK / 64];
+  __shared__ double sm[NT / 64];
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) sm[wid] = v;
   __syncthreads();
   double t = 0.0;
 #pragma unroll
-  for (int w = 0; w < BLOCK / 64; ++w) t += sm[w];
+  for (int w = 0; w < NT / 64; ++w) t += sm[w];
   __syncthreads();
   return t;
 }
 
-// Writes this block's partial, returns true in the (single) block that
-// arrives last; that block then sees every partial.
-__device__ __forceinline__ bool publish_partial(double* partials, uint32_t* counter, int slot,
-                                                int nparts, double v) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    partials[slot] = v;
-    __threadfence();
-    uint32_t prev = atomicAdd(counter, 1u);
-    last = (prev == (uint32_t)nparts - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-// Sum of all partials in a fixed order (run by the last block).
-__device__ __forceinline__ double gather_partials(const double* partials, int nparts) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += BLOCK) v += __builtin_nontemporal_load(&partials[i]);
-  return block_sum(v);
-}
-
 // ---------------------------------------------------------------------------
-// neighbour sum (mgpcg.py:96-102) for the colour-c cell at slot t:
-//   ret = ret + (Ax[I-e_a][a] * x[I-e_a] + Ax[I][a] * x[I+e_a]),  a = 0..dim-1
-// xo = other-colour vector, Axo/Axc = other/own colour coefficient arrays.
+// stencil helpers
 // ---------------------------------------------------------------------------
-template <typename T, int DIM, typename V>
-__device__ __forceinline__ T nsum(const Geo& g, const T* const Axc[3], const T* const Axo[3],
-                                  const V* xo, int t, int q) {
-  T ret = (T)0;
-  if (DIM == 3) {
-    T lo = Axo[0][t - g.sx] * (T)xo[t - g.sx];
-    T hi = Axc[0][t] * (T)xo[t + g.sx];
-    T s = lo + hi;
-    ret = ret + s;
-  }
-  {
-    const int a = (DIM == 3) ? 1 : 0;
-    T lo = Axo[a][t - g.W] * (T)xo[t - g.W];
-    T hi = Axc[a][t] * (T)xo[t + g.W];
-    T s = lo + hi;
-    ret = ret + s;
-  }
-  {
-    const int a = DIM - 1;
-    const int tl = t + q - 1, th = t + q;
-    T lo = Axo[a][tl] * (T)xo[tl];
-    T hi = Axc[a][t] * (T)xo[th];
-    T s = lo + hi;
-    ret = ret + s;
-  }
-  return ret;
-}
-
-template <typename T>
-__device__ __forceinline__ void own_other(const LevelPtrs<T>& L, int c, const T* Axc[3],
-                                          const T* Axo[3]) {
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    Axc[a] = L.Ax[a][c];
-    Axo[a] = L.Ax[a][c ^ 1];
-  }
-}
-
-__device__ __forceinline__ int row_parity_q(const Geo& g, int t, int colour) {
+__device__ __forceinline__ int row_q(const Geo& g, int t, int colour) {
   uint32_t row = g.divW.div((uint32_t)t);
   uint32_t ip = g.divP1.div(row);
   int jp = (int)row - (int)ip * g.P1;
@@ -125,78 +143,143 @@ __device__ __forceinline__ int row_parity_q(const Geo& g, int t, int colour) {
   return (colour + ((i + j) & 1) + 1) & 1;
 }
 
+// Neighbour slots of a colour-c cell at t (other colour), in the reference's
+// accumulation order: axis 0 lower/upper, axis 1, axis 2 (mgpcg.py:98-101).
+template <int DIM>
+struct Nbr {
+  int lo[DIM], hi[DIM];
+};
+template <int DIM>
+__device__ __forceinline__ Nbr<DIM> neighbours(const Geo& g, int t, int q) {
+  Nbr<DIM> n;
+  int a = 0;
+  if (DIM == 3) {
+    n.lo[0] = t - g.sx;
+    n.hi[0] = t + g.sx;
+    a = 1;
+  }
+  n.lo[a] = t - g.W;
+  n.hi[a] = t + g.W;
+  n.lo[DIM - 1] = t + q - 1;
+  n.hi[DIM - 1] = t + q;
+  return n;
+}
+
+// neighbour_sum (mgpcg.py:96-102) with the other-colour values supplied by a
+// functor zo(slot):  ret = ret + (Ax_lo * z_lo + Ax_own * z_hi)
+template <typename T, int DIM, typename ZO>
+__device__ __forceinline__ T nsum(const LevelPtrs<T>& P, int c, int t, const Nbr<DIM>& n,
+                                  ZO zo) {
+  T ret = (T)0;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    T lo = P.Ax[a][c ^ 1][n.lo[a]] * zo(n.lo[a]);
+    T hi = P.Ax[a][c][t] * zo(n.hi[a]);
+    T s = lo + hi;
+    ret = ret + s;
+  }
+  return ret;
+}
+
+// parent slot in the coarser level of fine-level slot `t` of colour `c`
+// (fine original cell (i, j, k) -> (i/2, j/2, k/2)); returns colour in pc.
+__device__ __forceinline__ int parent_slot(const Geo& gf, const Geo& gc, int t, int c, int& pc) {
+  Slot s = decode(gf, t, c);
+  const int pi = s.i >> 1, pj = s.j >> 1, pk = s.k >> 1;
+  pc = colour_of(pi, pj, pk);
+  return slot_of(gc, pi, pj, pk);
+}
+
 // ---------------------------------------------------------------------------
-// smooth(l, phase) (mgpcg.py:104-109): one red-black half-sweep on colour c.
-// FIRST: z was just zeroed (mgpcg.py:126 / :132) -> the half-sweep reduces to
-// z = (r - 0) / Adiag on colour c and the kernel also writes the zeros of the
-// whole level, replacing the separate fill.
+// per-slot stage bodies (shared by the batched kernels and the coarse kernel)
 // ---------------------------------------------------------------------------
-template <typename T, int DIM, bool FIRST>
-__global__ void __launch_bounds__(BLOCK) k_smooth(Geo g, LevelPtrs<T> L, int c,
-                                                  const int* __restrict__ done) {
-  if (done && *done) return;
-  const int nb = gridDim.x;
-  const int t = xcd_remap(blockIdx.x, nb) * BLOCK + threadIdx.x;
-  if (t >= g.size) return;
-  if (FIRST) {
-    // neighbour_sum over an all-zero z is +-0, so (r - ns) / Adiag equals
-    // r / Adiag (up to the sign of a zero result); the Ax reads are skipped.
-    T zc = (T)0;
-    if (L.gt[c][t] == MG_FLUID) zc = L.r[c][t] / L.Adiag[c][t];
-    L.z[c][t] = zc;
-    L.z[c ^ 1][t] = (T)0;
+// smooth(l, c) (mgpcg.py:104-109) at slot t of colour c.
+template <typename T, int DIM>
+__device__ __forceinline__ void smooth_slot(const Geo& g, const LevelPtrs<T>& P, int c, int t) {
+  if (P.gt[c][t] != MG_FLUID) return;
+  const int q = row_q(g, t, c);
+  Nbr<DIM> n = neighbours<DIM>(g, t, q);
+  const T* zo = P.z[c ^ 1];
+  T s = nsum<T, DIM>(P, c, t, n, [&](int u) { return zo[u]; });
+  T num = P.r[c][t] - s;
+  P.z[c][t] = num / P.Adiag[c][t];
+}
+
+// z.fill(0); smooth(l, 0); smooth(l, 1) (mgpcg.py:126-130 / 132-134) at slot t:
+// colour 0 from zero is r / Adiag, recomputed for every colour-1 stencil.
+template <typename T, int DIM>
+__device__ __forceinline__ void smooth_pair_from_zero_slot(const Geo& g, const LevelPtrs<T>& P,
+                                                           int t) {
+  const bool f0 = P.gt[0][t] == MG_FLUID;
+  P.z[0][t] = f0 ? P.r[0][t] / P.Adiag[0][t] : (T)0;
+  if (P.gt[1][t] != MG_FLUID) {
+    P.z[1][t] = (T)0;
     return;
   }
-  if (L.gt[c][t] != MG_FLUID) return;
-  const T* Axc[3];
-  const T* Axo[3];
-  own_other(L, c, Axc, Axo);
-  const int q = row_parity_q(g, t, c);
-  T s = nsum<T, DIM>(g, Axc, Axo, L.z[c ^ 1], t, q);
-  T num = L.r[c][t] - s;
-  L.z[c][t] = num / L.Adiag[c][t];
+  const int q = row_q(g, t, 1);
+  Nbr<DIM> n = neighbours<DIM>(g, t, q);
+  auto z0 = [&](int u) -> T {
+    return P.gt[0][u] == MG_FLUID ? P.r[0][u] / P.Adiag[0][u] : (T)0;
+  };
+  T s = nsum<T, DIM>(P, 1, t, n, z0);
+  T num = P.r[1][t] - s;
+  P.z[1][t] = num / P.Adiag[1][t];
 }
 
-// z <- 0 on both colours (used only when a level gets no smoothing).
+// prolongate(l); smooth(l, 1) (mgpcg.py:146-148) at colour-1 slot t: the
+// colour-0 neighbours read z0 + z_coarse[parent].
+template <typename T, int DIM>
+__device__ __forceinline__ void prolong_smooth1_slot(const Geo& g, const Geo& gc,
+                                                     const LevelPtrs<T>& P, const T* zc0,
+                                                     const T* zc1, int t) {
+  if (P.gt[1][t] != MG_FLUID) return;
+  const Slot c = decode(g, t, 1);  // FLUID cells are interior
+  Nbr<DIM> n = neighbours<DIM>(g, t, c.q);
+  const T* z0 = P.z[0];
+  // value of the colour-0 neighbour (a, dir) after prolongation; a ghost
+  // neighbour (outside the level) has no parent and stays 0
+  auto zp = [&](int a, int dir, int u) -> T {
+    int i = c.i, j = c.j, k = c.k;
+    const int va = a + 3 - DIM;  // view axis of original axis a
+    if (va == 0) i += dir;
+    else if (va == 1) j += dir;
+    else k += dir;
+    if (i < 0 || i >= g.n0 || j < 0 || j >= g.n1 || k < 0 || k >= g.n2) return z0[u];
+    const int pi = i >> 1, pj = j >> 1, pk = k >> 1;
+    const int ps = slot_of(gc, pi, pj, pk);
+    const T zcv = colour_of(pi, pj, pk) ? zc1[ps] : zc0[ps];
+    return z0[u] + zcv;
+  };
+  T ret = (T)0;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    T lo = P.Ax[a][0][n.lo[a]] * zp(a, -1, n.lo[a]);
+    T hi = P.Ax[a][1][t] * zp(a, +1, n.hi[a]);
+    T sm = lo + hi;
+    ret = ret + sm;
+  }
+  T num = P.r[1][t] - ret;
+  P.z[1][t] = num / P.Adiag[1][t];
+}
+
+// prolongate(l) alone (only when no post-smoothing follows): every interior cell.
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_zero2(int size, T* a0, T* a1, const int* done) {
-  if (done && *done) return;
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= size) return;
-  a0[t] = (T)0;
-  a1[t] = (T)0;
+__device__ __forceinline__ void prolongate_slot(const Geo& g, const Geo& gc, T* const z[2],
+                                                const T* zc0, const T* zc1, int c, int t) {
+  Slot s = decode(g, t, c);
+  if (!s.interior) return;
+  const int pi = s.i >> 1, pj = s.j >> 1, pk = s.k >> 1;
+  const int pc = colour_of(pi, pj, pk);
+  const int ps = slot_of(gc, pi, pj, pk);
+  z[c][t] = z[c][t] + (pc ? zc1[ps] : zc0[ps]);
 }
 
-// ---------------------------------------------------------------------------
-// restrict(l) (mgpcg.py:111-118), gather form: every interior coarse cell sums
-// the residuals r - (Adiag z + neighbour_sum) of its FLUID children in
-// lexicographic offset order, starting from 0.
-// ---------------------------------------------------------------------------
+// restrict(l) (mgpcg.py:111-118) for the interior coarse cell at coarse slot tc
+// of colour cc: the residuals of the FLUID children, lexicographic order.
 template <typename T, int DIM>
-__device__ __forceinline__ T child_residual(const Geo& gf, const LevelPtrs<T>& F, int i, int j,
-                                            int k) {
-  const int c = colour_of(i, j, k);
-  const int t = slot_of(gf, i, j, k);
-  const T* Axc[3];
-  const T* Axo[3];
-  own_other(F, c, Axc, Axo);
-  const int q = ((k + 1) & 1);  // padded k' = k + 1 = 2m + q
-  T Az = F.Adiag[c][t] * F.z[c][t];
-  T s = nsum<T, DIM>(gf, Axc, Axo, F.z[c ^ 1], t, q);
-  Az = Az + s;
-  return F.r[c][t] - Az;
-}
-
-template <typename T, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_restrict(Geo gf, Geo gc, LevelPtrs<T> F, T* rc0,
-                                                    T* rc1, const int* __restrict__ done) {
-  if (done && *done) return;
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int cc = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  if (t >= gc.size) return;
-  Slot s = decode(gc, t, cc);
+__device__ __forceinline__ void restrict_slot(const Geo& gf, const LevelPtrs<T>& F,
+                                              const Geo& gc, T* rc0, T* rc1, int cc, int tc) {
+  Slot s = decode(gc, tc, cc);
   if (!s.interior) return;
   T sum = (T)0;
   const int o0max = (DIM == 3) ? 2 : 1;
@@ -206,326 +289,495 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(Geo gf, Geo gc, LevelPtrs<T>
         const int i = (DIM == 3) ? 2 * s.i + o0 : 0;
         const int j = 2 * s.j + o1, k = 2 * s.k + o2;
         const int c = colour_of(i, j, k);
-        const int tf = slot_of(gf, i, j, k);
-        if (F.gt[c][tf] != MG_FLUID) continue;
-        T res = child_residual<T, DIM>(gf, F, i, j, k);
+        const int t = slot_of(gf, i, j, k);
+        if (F.gt[c][t] != MG_FLUID) continue;
+        const int q = (k + 1) & 1;  // padded k' = k + 1 = 2m + q
+        Nbr<DIM> n = neighbours<DIM>(gf, t, q);
+        const T* zo = F.z[c ^ 1];
+        T Az = F.Adiag[c][t] * F.z[c][t];
+        T ns = nsum<T, DIM>(F, c, t, n, [&](int u) { return zo[u]; });
+        Az = Az + ns;
+        T res = F.r[c][t] - Az;
         sum = sum + res;
       }
-  (cc ? rc1 : rc0)[t] = sum;
+  (cc ? rc1 : rc0)[tc] = sum;
+}
+
+// z <- 0 on both colours (a level that gets no smoothing).
+template <typename T>
+__device__ __forceinline__ void zero_slot(const LevelPtrs<T>& P, int t) {
+  P.z[0][t] = (T)0;
+  P.z[1][t] = (T)0;
 }
 
 // ---------------------------------------------------------------------------
-// prolongate(l) (mgpcg.py:120-123): z_l[I] += z_{l+1}[I // 2] on every cell.
+// batched stage kernels over level descriptors
+// ---------------------------------------------------------------------------
+enum StageOp {
+  ST_SMOOTH0 = 0,       // smooth(l, 0)
+  ST_SMOOTH1 = 1,       // smooth(l, 1)
+  ST_PAIR_FROM_ZERO = 2,  // fill(0); smooth(l,0); smooth(l,1)
+  ST_PROLONG_SMOOTH1 = 3,  // prolongate(l); smooth(l, 1)
+  ST_ZERO = 4,
+  ST_PROLONGATE = 5,
+};
+
+template <typename T, int DIM, int OP>
+__global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict__ descs,
+                                                 Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const LevelDesc<T>& D = descs[item.desc];
+    if (D.st->done) continue;
+    const int t = (it.vb - item.block0) * BLOCK + threadIdx.x;
+    if (t >= D.g.size) continue;
+    if (OP == ST_SMOOTH0) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    else if (OP == ST_SMOOTH1) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    else if (OP == ST_PAIR_FROM_ZERO) smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
+    else if (OP == ST_PROLONG_SMOOTH1)
+      prolong_smooth1_slot<T, DIM>(D.g, D.gc, D.P, D.C.z[0], D.C.z[1], t);
+    else if (OP == ST_ZERO) zero_slot<T>(D.P, t);
+    else if (OP == ST_PROLONGATE) {
+      prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 0, t);
+      prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 1, t);
+    }
+  }
+}
+
+// restrict(l): virtual blocks cover the coarse level's slots, both colours per thread.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_restrict(const LevelDesc<T>* __restrict__ descs,
+                                                    Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const LevelDesc<T>& D = descs[item.desc];
+    if (D.st->done) continue;
+    const int tc = (it.vb - item.block0) * BLOCK + threadIdx.x;
+    if (tc >= D.gc.size) continue;
+    restrict_slot<T, DIM>(D.g, D.P, D.gc, D.C.r[0], D.C.r[1], 0, tc);
+    restrict_slot<T, DIM>(D.g, D.P, D.gc, D.C.r[0], D.C.r[1], 1, tc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// coarse levels: one workgroup per system runs levels [from, L) of the V-cycle
+// (mgpcg.py:127-148 restricted to those levels), barriers between stages.
 // ---------------------------------------------------------------------------
 template <typename T, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_prolongate(Geo gf, Geo gc, T* zf0, T* zf1,
-                                                      const T* __restrict__ zc0,
-                                                      const T* __restrict__ zc1,
-                                                      const int* __restrict__ done) {
-  if (done && *done) return;
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  if (t >= gf.size) return;
-  Slot s = decode(gf, t, c);
-  if (!s.interior) return;
-  const int pi = s.i >> 1, pj = s.j >> 1, pk = s.k >> 1;
-  const int pc = colour_of(pi, pj, pk);
-  const int tc = slot_of(gc, pi, pj, pk);
-  T* zf = c ? zf1 : zf0;
-  const T* zc = pc ? zc1 : zc0;
-  zf[t] = zf[t] + zc[tc];
+__device__ void coarse_pre(const LevelDesc<T>& D, int nu) {
+  const int n = D.g.size;
+  if (nu <= 0) {
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(D.P, t);
+    __syncthreads();
+    return;
+  }
+  for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
+    smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
+  __syncthreads();
+  for (int i = 1; i < nu; ++i) {
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    __syncthreads();
+  }
+}
+
+template <typename T, int DIM>
+__device__ void coarse_post(const LevelDesc<T>& D, int nu) {
+  const int n = D.g.size;
+  if (nu <= 0) {
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) {
+      prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 0, t);
+      prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 1, t);
+    }
+    __syncthreads();
+    return;
+  }
+  for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
+    prolong_smooth1_slot<T, DIM>(D.g, D.gc, D.P, D.C.z[0], D.C.z[1], t);
+  __syncthreads();
+  for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+  __syncthreads();
+  for (int i = 1; i < nu; ++i) {
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    __syncthreads();
+  }
+}
+
+// bottom solve (mgpcg.py:136-142): half x (smooth 0, smooth 1) from z = 0,
+// then half x (smooth 1, smooth 0).
+template <typename T, int DIM>
+__device__ void coarse_bottom(const LevelDesc<T>& D, int bottom) {
+  const int n = D.g.size;
+  const int half = bottom / 2;
+  if (half <= 0) {
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(D.P, t);
+    __syncthreads();
+    return;
+  }
+  for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
+    smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
+  __syncthreads();
+  for (int i = 1; i < half; ++i) {
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    __syncthreads();
+  }
+  for (int i = 0; i < half; ++i) {
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    __syncthreads();
+  }
+}
+
+// One workgroup per item: item.desc = the system's LevelDesc of its first
+// coarse level, item.block0 = number of levels from there to the bottom
+// (a system's LevelDescs are contiguous).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* __restrict__ descs,
+                                                           const Item* __restrict__ items,
+                                                           int nitems, int nu, int bottom) {
+  if ((int)blockIdx.x >= nitems) return;
+  const int d0 = items[blockIdx.x].desc;
+  const int nlev = items[blockIdx.x].block0;
+  if (descs[d0].st->done) return;
+  for (int k = 0; k < nlev - 1; ++k) {
+    const LevelDesc<T>& D = descs[d0 + k];
+    coarse_pre<T, DIM>(D, nu);
+    for (int tc = threadIdx.x; tc < D.gc.size; tc += COARSE_THREADS) {
+      restrict_slot<T, DIM>(D.g, D.P, D.gc, D.C.r[0], D.C.r[1], 0, tc);
+      restrict_slot<T, DIM>(D.g, D.P, D.gc, D.C.r[0], D.C.r[1], 1, tc);
+    }
+    __syncthreads();
+  }
+  coarse_bottom<T, DIM>(descs[d0 + nlev - 1], bottom);
+  for (int k = nlev - 2; k >= 0; --k) coarse_post<T, DIM>(descs[d0 + k], nu);
 }
 
 // ---------------------------------------------------------------------------
-// CG kernels on level 0 (mgpcg.py:150-238) with fused FLUID-masked FP64 dots
-// and the scalar updates done on device by the last block.
+// CG kernels on level 0 (mgpcg.py:150-238).  Reductions: one FP64 partial per
+// virtual block; a per-system finalize kernel sums them in a fixed order.
 // ---------------------------------------------------------------------------
-template <typename T, typename TX>
-struct CGPtrs {
-  T* b[2];
-  TX* x[2];
-  T* p[2];
-  T* Ap[2];
-};
-
-struct RedPtrs {
-  CGState* st;
-  double* partials;
-};
-
 __device__ __forceinline__ double rnd_plain(const CGState* st, double v) {
   return st->precision == PLAIN ? (double)(float)v : v;
 }
 
-// r = b (copy_from, mgpcg.py:157), x = p = Ap = 0, reduce(r, r) -> r0r0, tol.
+// r = b (mgpcg.py:157), x = p = Ap = 0; partial r.r.  Systems in phase 0 only.
 template <typename T, typename TX>
-__global__ void __launch_bounds__(BLOCK) k_cg_init(Geo g, LevelPtrs<T> L0, CGPtrs<T, TX> V,
-                                                   RedPtrs R) {
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  double acc = 0.0;
-  if (t < g.size) {
-    T bv = V.b[c][t];
-    L0.r[c][t] = bv;
-    V.x[c][t] = (TX)0;
-    V.p[c][t] = (T)0;
-    V.Ap[c][t] = (T)0;
-    if (L0.gt[c][t] == MG_FLUID) acc = (double)bv * (double)bv;
-  }
-  double s = block_sum(acc);
-  if (publish_partial(R.partials, &R.st->counter, blockIdx.x, nb, s)) {
-    double tot = gather_partials(R.partials, nb);
-    if (threadIdx.x == 0) {
-      CGState* st = R.st;
-      st->counter = 0;
-      st->r0r0 = rnd_plain(st, tot);
-      st->rr = st->r0r0;
-      st->tol = fmax(st->abs_tol, st->r0r0 * st->rel_tol);
-      st->iter = 0;
-      st->done = 0;
-      st->status = RUNNING;
-      st->since_refine = 0;
-      st->pending_true = 0;
-      st->beta = 0.0;
-      st->beta32 = 0.0f;
-      st->true_rr = -1.0;
+__global__ void __launch_bounds__(BLOCK) k_cg_init(const SysDesc<T, TX>* __restrict__ sys,
+                                                   Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const SysDesc<T, TX>& D = sys[item.desc];
+    const int lb = it.vb - item.block0;
+    const int t = lb * BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (t < D.g.size) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        T bv = D.b[c][t];
+        D.P.r[c][t] = bv;
+        D.x[c][t] = (TX)0;
+        D.p[0][c][t] = (T)0;
+        D.p[1][c][t] = (T)0;
+        D.Ap[c][t] = (T)0;
+        if (D.P.gt[c][t] == MG_FLUID) acc += (double)bv * (double)bv;
+      }
     }
+    double s = block_sum<BLOCK>(acc);
+    if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
 
-// update_p (mgpcg.py:234-238): p = z + beta p on FLUID cells.
-template <typename T, typename TX>
-__global__ void __launch_bounds__(BLOCK) k_update_p(Geo g, LevelPtrs<T> L0, CGPtrs<T, TX> V,
-                                                    const CGState* __restrict__ st) {
-  if (st->done) return;
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  if (t >= g.size) return;
-  if (L0.gt[c][t] != MG_FLUID) return;
-  if (st->precision == PLAIN) {
+// p = z + beta p (update_p, mgpcg.py:234-238) into the other parity buffer,
+// Ap = Adiag p + neighbour_sum(p) (compute_Ap, :218-224), partial p.Ap.
+template <typename T, typename TX, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_cg_ap(const SysDesc<T, TX>* __restrict__ sys,
+                                                 Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const SysDesc<T, TX>& D = sys[item.desc];
+    const CGState* st = D.st;
+    if (st->done) continue;
+    const int lb = it.vb - item.block0;
+    const int t = lb * BLOCK + threadIdx.x;
+    const int par = st->iter & 1;
+    const T* pold[2] = {D.p[par][0], D.p[par][1]};
+    T* pnew[2] = {D.p[par ^ 1][0], D.p[par ^ 1][1]};
+    const bool plain = st->precision == PLAIN;
     const float b32 = st->beta32;
-    T tt = (T)b32 * V.p[c][t];
-    V.p[c][t] = L0.z[c][t] + tt;
-  } else {
     const double beta = st->beta;
-    V.p[c][t] = (T)((double)L0.z[c][t] + beta * (double)V.p[c][t]);
-  }
-}
-
-// reduce(z, r) (mgpcg.py:171, 201) -> beta (mgpcg.py:203), iteration count.
-// FIRSTDOT: the dot before the loop (mgpcg.py:171-172) only sets old_zTr.
-template <typename T, bool FIRSTDOT>
-__global__ void __launch_bounds__(BLOCK) k_dot_zr(Geo g, LevelPtrs<T> L0, RedPtrs R) {
-  if (R.st->done) return;
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  double acc = 0.0;
-  if (t < g.size && L0.gt[c][t] == MG_FLUID) acc = (double)L0.z[c][t] * (double)L0.r[c][t];
-  double s = block_sum(acc);
-  if (publish_partial(R.partials, &R.st->counter, blockIdx.x, nb, s)) {
-    double tot = gather_partials(R.partials, nb);
-    if (threadIdx.x == 0) {
-      CGState* st = R.st;
-      st->counter = 0;
-      double zr = rnd_plain(st, tot);
-      if (FIRSTDOT) {
-        st->zr_old = zr;
-      } else {
-        st->zr_new = zr;
-        double beta = zr / (st->zr_old + st->eps);
-        st->beta = beta;
-        st->beta32 = (float)beta;
-        st->zr_old = zr;
-        st->iter += 1;
-        if (!isfinite(beta)) {
-          st->done = 1;
-          st->status = BREAKDOWN;
-        } else if (st->max_iters >= 0 && st->iter >= st->max_iters) {
-          st->done = 1;
-          st->status = MAX_ITERS;
-        }
+    auto upd = [&](int c, int u) -> T {
+      // update_p on a FLUID cell; other cells keep p (inert)
+      if (D.P.gt[c][u] != MG_FLUID) return pold[c][u];
+      if (plain) {
+        T tt = (T)b32 * pold[c][u];
+        return D.P.z[c][u] + tt;
+      }
+      return (T)((double)D.P.z[c][u] + beta * (double)pold[c][u]);
+    };
+    double acc = 0.0;
+    if (t < D.g.size) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (D.P.gt[c][t] != MG_FLUID) continue;
+        const int q = row_q(D.g, t, c);
+        Nbr<DIM> n = neighbours<DIM>(D.g, t, q);
+        T pv = upd(c, t);
+        pnew[c][t] = pv;
+        T rr = D.P.Adiag[c][t] * pv;
+        T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return upd(c ^ 1, u); });
+        T ap = rr + ns;
+        D.Ap[c][t] = ap;
+        acc += (double)pv * (double)ap;
       }
     }
+    double s = block_sum<BLOCK>(acc);
+    if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
 
-// compute_Ap (mgpcg.py:218-224) + reduce(p, Ap) -> alpha (mgpcg.py:178-181).
-template <typename T, typename TX, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_compute_Ap(Geo g, LevelPtrs<T> L0, CGPtrs<T, TX> V,
-                                                      RedPtrs R) {
-  if (R.st->done) return;
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  double acc = 0.0;
-  if (t < g.size && L0.gt[c][t] == MG_FLUID) {
-    const T* Axc[3];
-    const T* Axo[3];
-    own_other(L0, c, Axc, Axo);
-    const int q = row_parity_q(g, t, c);
-    T pv = V.p[c][t];
-    T rr = L0.Adiag[c][t] * pv;
-    T s = nsum<T, DIM>(g, Axc, Axo, V.p[c ^ 1], t, q);
-    T ap = rr + s;
-    V.Ap[c][t] = ap;
-    acc = (double)pv * (double)ap;
-  }
-  double s = block_sum(acc);
-  if (publish_partial(R.partials, &R.st->counter, blockIdx.x, nb, s)) {
-    double tot = gather_partials(R.partials, nb);
-    if (threadIdx.x == 0) {
-      CGState* st = R.st;
-      st->counter = 0;
-      st->pAp = rnd_plain(st, tot);
-      double alpha = st->zr_old / (st->pAp + st->eps);
-      st->alpha = alpha;
-      st->alpha32 = (float)alpha;
-      if (!isfinite(alpha)) {
-        st->done = 1;
-        st->status = BREAKDOWN;
-      }
-    }
-  }
-}
-
-// update_xr (mgpcg.py:226-232) + reduce(r, r) + convergence test (:188-194).
-// MIXED: x is FP64; on a refinement iteration r is left for k_true_residual.
+// update_xr (mgpcg.py:226-232) + partial r.r.  MIXED: x is FP64 and on a
+// refinement iteration r is left for k_true_residual.
 template <typename T, typename TX>
-__global__ void __launch_bounds__(BLOCK) k_update_xr(Geo g, LevelPtrs<T> L0, CGPtrs<T, TX> V,
-                                                     RedPtrs R) {
-  const CGState* cst = R.st;
-  if (cst->done) return;
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  const bool mixed = cst->precision != PLAIN;
-  const bool refine = cst->precision == MIXED && cst->refine_every > 0 &&
-                      cst->since_refine + 1 >= cst->refine_every;
-  double acc = 0.0;
-  if (t < g.size && L0.gt[c][t] == MG_FLUID) {
-    if (!mixed) {
-      const float a32 = cst->alpha32;
-      T tp = (T)a32 * V.p[c][t];
-      V.x[c][t] = (TX)((T)V.x[c][t] + tp);
-      T ta = (T)a32 * V.Ap[c][t];
-      T rv = L0.r[c][t] - ta;
-      L0.r[c][t] = rv;
-      acc = (double)rv * (double)rv;
-    } else {
-      const double alpha = cst->alpha;
-      if (cst->precision == F64) {
-        // everything FP64: the same op order as the reference in double
-        T tp = (T)(alpha * (double)V.p[c][t]);
-        V.x[c][t] = (TX)((double)V.x[c][t] + (double)tp);
-        T ta = (T)(alpha * (double)V.Ap[c][t]);
-        T rv = L0.r[c][t] - ta;
-        L0.r[c][t] = rv;
-        acc = (double)rv * (double)rv;
-      } else {
-        V.x[c][t] = (TX)((double)V.x[c][t] + alpha * (double)V.p[c][t]);
-        if (!refine) {
-          T rv = (T)((double)L0.r[c][t] - alpha * (double)V.Ap[c][t]);
-          L0.r[c][t] = rv;
-          acc = (double)rv * (double)rv;
-        }
-      }
-    }
-  }
-  double s = block_sum(acc);
-  if (publish_partial(R.partials, &R.st->counter, blockIdx.x, nb, s)) {
-    double tot = gather_partials(R.partials, nb);
-    if (threadIdx.x == 0) {
-      CGState* st = R.st;
-      st->counter = 0;
-      if (st->precision == MIXED) {
-        st->since_refine += 1;
-        if (refine) {
-          st->pending_true = 1;
+__global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restrict__ sys,
+                                                 Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const SysDesc<T, TX>& D = sys[item.desc];
+    const CGState* st = D.st;
+    if (st->done) continue;
+    const int lb = it.vb - item.block0;
+    const int t = lb * BLOCK + threadIdx.x;
+    const int par = (st->iter & 1) ^ 1;  // the p written by k_cg_ap
+    const int prec = st->precision;
+    const bool refine = prec == MIXED && st->refine_every > 0 &&
+                        st->since_refine + 1 >= st->refine_every;
+    double acc = 0.0;
+    if (t < D.g.size) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (D.P.gt[c][t] != MG_FLUID) continue;
+        const T pv = D.p[par][c][t];
+        const T apv = D.Ap[c][t];
+        if (prec == PLAIN) {
+          const float a32 = st->alpha32;
+          T tp = (T)a32 * pv;
+          D.x[c][t] = (TX)((T)D.x[c][t] + tp);
+          T ta = (T)a32 * apv;
+          T rv = D.P.r[c][t] - ta;
+          D.P.r[c][t] = rv;
+          acc += (double)rv * (double)rv;
+        } else if (prec == F64) {
+          const double alpha = st->alpha;
+          T tp = (T)(alpha * (double)pv);
+          D.x[c][t] = (TX)((double)D.x[c][t] + (double)tp);
+          T ta = (T)(alpha * (double)apv);
+          T rv = D.P.r[c][t] - ta;
+          D.P.r[c][t] = rv;
+          acc += (double)rv * (double)rv;
         } else {
-          st->rr = tot;
-          if (tot < st->tol) st->pending_true = 1;  // accept only on the true residual
-        }
-      } else {
-        st->rr = rnd_plain(st, tot);
-        if (st->rr < st->tol) {
-          st->done = 1;
-          st->status = CONVERGED;
+          const double alpha = st->alpha;
+          D.x[c][t] = (TX)((double)D.x[c][t] + alpha * (double)pv);
+          if (!refine) {
+            T rv = (T)((double)D.P.r[c][t] - alpha * (double)apv);
+            D.P.r[c][t] = rv;
+            acc += (double)rv * (double)rv;
+          }
         }
       }
     }
+    double s = block_sum<BLOCK>(acc);
+    if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
 
-// MIXED: r = b - A x with x in FP64, accumulated in FP64, stored as T; then
-// reduce(r, r) and the convergence test on this true residual.
+// MIXED: r = b - A x with x in FP64, accumulated in FP64, stored as T; partial r.r.
 template <typename T, typename TX, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_true_residual(Geo g, LevelPtrs<T> L0,
-                                                         CGPtrs<T, TX> V, RedPtrs R) {
-  const CGState* cst = R.st;
-  if (cst->done || !cst->pending_true) return;
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  double acc = 0.0;
-  if (t < g.size && L0.gt[c][t] == MG_FLUID) {
-    const T* Axc[3];
-    const T* Axo[3];
-    own_other(L0, c, Axc, Axo);
-    const int q = row_parity_q(g, t, c);
-    const TX* xo = V.x[c ^ 1];
-    double ret = 0.0;
-    if (DIM == 3) {
-      double lo = (double)Axo[0][t - g.sx] * (double)xo[t - g.sx];
-      double hi = (double)Axc[0][t] * (double)xo[t + g.sx];
-      double s = lo + hi;
-      ret = ret + s;
+__global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* __restrict__ sys,
+                                                         Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const SysDesc<T, TX>& D = sys[item.desc];
+    const CGState* st = D.st;
+    if (st->done || !st->pending_true) continue;
+    const int lb = it.vb - item.block0;
+    const int t = lb * BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (t < D.g.size) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (D.P.gt[c][t] != MG_FLUID) continue;
+        const int q = row_q(D.g, t, c);
+        Nbr<DIM> n = neighbours<DIM>(D.g, t, q);
+        const TX* xo = D.x[c ^ 1];
+        double ret = 0.0;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          double lo = (double)D.P.Ax[a][c ^ 1][n.lo[a]] * (double)xo[n.lo[a]];
+          double hi = (double)D.P.Ax[a][c][t] * (double)xo[n.hi[a]];
+          double s2 = lo + hi;
+          ret = ret + s2;
+        }
+        double Axv = (double)D.P.Adiag[c][t] * (double)D.x[c][t];
+        Axv = Axv + ret;
+        double rv = (double)D.b[c][t] - Axv;
+        T r32 = (T)rv;
+        D.P.r[c][t] = r32;
+        acc += (double)r32 * (double)r32;
+      }
     }
-    {
-      const int a = (DIM == 3) ? 1 : 0;
-      double lo = (double)Axo[a][t - g.W] * (double)xo[t - g.W];
-      double hi = (double)Axc[a][t] * (double)xo[t + g.W];
-      double s = lo + hi;
-      ret = ret + s;
-    }
-    {
-      const int a = DIM - 1;
-      const int tl = t + q - 1, th = t + q;
-      double lo = (double)Axo[a][tl] * (double)xo[tl];
-      double hi = (double)Axc[a][t] * (double)xo[th];
-      double s = lo + hi;
-      ret = ret + s;
-    }
-    double Axv = (double)L0.Adiag[c][t] * (double)V.x[c][t];
-    Axv = Axv + ret;
-    double rv = (double)V.b[c][t] - Axv;
-    T r32 = (T)rv;
-    L0.r[c][t] = r32;
-    acc = (double)r32 * (double)r32;
+    double s = block_sum<BLOCK>(acc);
+    if (threadIdx.x == 0) D.partials[lb] = s;
   }
-  double s = block_sum(acc);
-  if (publish_partial(R.partials, &R.st->counter, blockIdx.x, nb, s)) {
-    double tot = gather_partials(R.partials, nb);
-    if (threadIdx.x == 0) {
-      CGState* st = R.st;
-      st->counter = 0;
-      st->pending_true = 0;
-      st->since_refine = 0;
-      st->rr = tot;
-      st->true_rr = tot;
-      if (tot < st->tol) {
+}
+
+// Last post-smoothing half-sweep of level 0, smooth(0, 0) (mgpcg.py:148),
+// fused with reduce(z, r) (mgpcg.py:171 / 201) over both colours.
+template <typename T, typename TX, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __restrict__ sys,
+                                                       Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const SysDesc<T, TX>& D = sys[item.desc];
+    if (D.st->done) continue;
+    const int lb = it.vb - item.block0;
+    const int t = lb * BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (t < D.g.size) {
+      smooth_slot<T, DIM>(D.g, D.P, 0, t);
+      if (D.P.gt[0][t] == MG_FLUID) acc += (double)D.P.z[0][t] * (double)D.P.r[0][t];
+      if (D.P.gt[1][t] == MG_FLUID) acc += (double)D.P.z[1][t] * (double)D.P.r[1][t];
+    }
+    double s = block_sum<BLOCK>(acc);
+    if (threadIdx.x == 0) D.partials[lb] = s;
+  }
+}
+
+// reduce(z, r) alone (used when level 0 is inside the coarse kernel or gets
+// no post-smoothing).
+template <typename T, typename TX>
+__global__ void __launch_bounds__(BLOCK) k_dot_zr(const SysDesc<T, TX>* __restrict__ sys,
+                                                  Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const SysDesc<T, TX>& D = sys[item.desc];
+    if (D.st->done) continue;
+    const int lb = it.vb - item.block0;
+    const int t = lb * BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (t < D.g.size) {
+      if (D.P.gt[0][t] == MG_FLUID) acc += (double)D.P.z[0][t] * (double)D.P.r[0][t];
+      if (D.P.gt[1][t] == MG_FLUID) acc += (double)D.P.z[1][t] * (double)D.P.r[1][t];
+    }
+    double s = block_sum<BLOCK>(acc);
+    if (threadIdx.x == 0) D.partials[lb] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-system finalize: one workgroup per system sums its partials in a fixed
+// order and performs the scalar step of mgpcg.py:150-209.
+// ---------------------------------------------------------------------------
+enum FinOp { FIN_INIT = 0, FIN_ALPHA = 1, FIN_RR = 2, FIN_TRUE = 3, FIN_BETA = 4 };
+
+template <typename T, typename TX, int OP>
+__global__ void __launch_bounds__(BLOCK) k_finalize(const SysDesc<T, TX>* __restrict__ sys,
+                                                    const Item* __restrict__ items, int nitems) {
+  if ((int)blockIdx.x >= nitems) return;
+  const SysDesc<T, TX>& D = sys[items[blockIdx.x].desc];
+  CGState* st = D.st;
+  if (st->done) return;
+  if (OP == FIN_TRUE && !st->pending_true) return;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < D.nparts; i += BLOCK) v += D.partials[i];
+  const double tot = block_sum<BLOCK>(v);
+  if (threadIdx.x != 0) return;
+  if (OP == FIN_INIT) {
+    // mgpcg.py:157-164
+    st->r0r0 = rnd_plain(st, tot);
+    st->rr = st->r0r0;
+    st->tol = fmax(st->abs_tol, st->r0r0 * st->rel_tol);
+    st->iter = 0;
+    st->status = RUNNING;
+    st->since_refine = 0;
+    st->pending_true = 0;
+    st->beta = 0.0;
+    st->beta32 = 0.0f;
+    st->true_rr = -1.0;
+  } else if (OP == FIN_ALPHA) {
+    // mgpcg.py:178-181
+    st->pAp = rnd_plain(st, tot);
+    const double alpha = st->zr_old / (st->pAp + st->eps);
+    st->alpha = alpha;
+    st->alpha32 = (float)alpha;
+    if (!isfinite(alpha)) {
+      st->done = 1;
+      st->status = BREAKDOWN;
+    }
+  } else if (OP == FIN_RR) {
+    // mgpcg.py:188-194
+    if (st->precision == MIXED) {
+      const bool refine = st->refine_every > 0 && st->since_refine + 1 >= st->refine_every;
+      st->since_refine += 1;
+      if (refine) {
+        st->pending_true = 1;
+      } else {
+        st->rr = tot;
+        if (tot < st->tol) st->pending_true = 1;  // accept only on the true residual
+      }
+    } else {
+      st->rr = rnd_plain(st, tot);
+      if (st->rr < st->tol) {
         st->done = 1;
         st->status = CONVERGED;
+      }
+    }
+  } else if (OP == FIN_TRUE) {
+    st->pending_true = 0;
+    st->since_refine = 0;
+    st->rr = tot;
+    st->true_rr = tot;
+    if (tot < st->tol) {
+      st->done = 1;
+      st->status = CONVERGED;
+    }
+  } else if (OP == FIN_BETA) {
+    const double zr = rnd_plain(st, tot);
+    if (st->phase == 0) {
+      // the dot before the loop (mgpcg.py:171-172)
+      st->zr_old = zr;
+      st->phase = 1;
+      if (st->max_iters == 0) {
+        st->done = 1;
+        st->status = MAX_ITERS;
+      }
+    } else {
+      // mgpcg.py:201-209
+      st->zr_new = zr;
+      const double beta = zr / (st->zr_old + st->eps);
+      st->beta = beta;
+      st->beta32 = (float)beta;
+      st->zr_old = zr;
+      st->iter += 1;
+      if (!isfinite(beta)) {
+        st->done = 1;
+        st->status = BREAKDOWN;
+      } else if (st->max_iters >= 0 && st->iter >= st->max_iters) {
+        st->done = 1;
+        st->status = MAX_ITERS;
       }
     }
   }
@@ -538,68 +790,57 @@ __device__ __forceinline__ long canon_index(const Geo& g, int i, int j, int k) {
   return ((long)i * g.n1 + j) * g.n2 + k;
 }
 
-// one level: gt (int8), Adiag (f32), Ax (f32, [cells, dim] AoS) -> colour
-// arrays of T; ghost slots get SOLID and zeros; r and z are zeroed.
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __restrict__ gt,
                                                       const float* __restrict__ Adiag,
                                                       const float* __restrict__ Ax,
-                                                      int8_t* gt0, int8_t* gt1, T* Ad0, T* Ad1,
-                                                      T* Ax00, T* Ax01, T* Ax10, T* Ax11,
-                                                      T* Ax20, T* Ax21, T* r0, T* r1, T* z0,
-                                                      T* z1) {
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
+                                                      LevelPtrs<T> P) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
   if (t >= g.size) return;
-  Slot s = decode(g, t, c);
-  int8_t gv = MG_SOLID;
-  T ad = (T)0, a0 = (T)0, a1 = (T)0, a2 = (T)0;
-  if (s.interior) {
-    long ci = canon_index(g, s.i, s.j, s.k);
-    gv = gt[ci];
-    ad = (T)Adiag[ci];
-    a0 = (T)Ax[ci * g.dim + 0];
-    a1 = (T)Ax[ci * g.dim + 1];
-    if (g.dim == 3) a2 = (T)Ax[ci * g.dim + 2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    Slot s = decode(g, t, c);
+    int8_t gv = MG_SOLID;
+    T ad = (T)0, a[3] = {(T)0, (T)0, (T)0};
+    if (s.interior) {
+      const long ci = canon_index(g, s.i, s.j, s.k);
+      gv = gt[ci];
+      ad = (T)Adiag[ci];
+      for (int k = 0; k < g.dim; ++k) a[k] = (T)Ax[ci * g.dim + k];
+    }
+    ((int8_t*)P.gt[c])[t] = gv;
+    ((T*)P.Adiag[c])[t] = ad;
+    for (int k = 0; k < g.dim; ++k) ((T*)P.Ax[k][c])[t] = a[k];
+    P.r[c][t] = (T)0;
+    P.z[c][t] = (T)0;
   }
-  (c ? gt1 : gt0)[t] = gv;
-  (c ? Ad1 : Ad0)[t] = ad;
-  (c ? Ax01 : Ax00)[t] = a0;
-  (c ? Ax11 : Ax10)[t] = a1;
-  if (g.dim == 3) (c ? Ax21 : Ax20)[t] = a2;
-  (c ? r1 : r0)[t] = (T)0;
-  (c ? z1 : z0)[t] = (T)0;
 }
 
-// canonical f32 vector -> colour arrays of T (ghosts 0)
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_pack_vec(Geo g, const float* __restrict__ src, T* d0,
-                                                    T* d1) {
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
+template <typename S, typename D>
+__global__ void __launch_bounds__(BLOCK) k_pack_vec(Geo g, const S* __restrict__ src, D* d0,
+                                                    D* d1) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
   if (t >= g.size) return;
-  Slot s = decode(g, t, c);
-  T v = (T)0;
-  if (s.interior) v = (T)src[canon_index(g, s.i, s.j, s.k)];
-  (c ? d1 : d0)[t] = v;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    Slot s = decode(g, t, c);
+    D v = (D)0;
+    if (s.interior) v = (D)src[canon_index(g, s.i, s.j, s.k)];
+    (c ? d1 : d0)[t] = v;
+  }
 }
 
-// colour arrays of S -> canonical vector of D (interior cells only)
 template <typename S, typename D>
 __global__ void __launch_bounds__(BLOCK) k_unpack_vec(Geo g, const S* __restrict__ s0,
                                                       const S* __restrict__ s1, D* dst) {
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
   if (t >= g.size) return;
-  Slot s = decode(g, t, c);
-  if (!s.interior) return;
-  dst[canon_index(g, s.i, s.j, s.k)] = (D)(c ? s1 : s0)[t];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    Slot s = decode(g, t, c);
+    if (!s.interior) continue;
+    dst[canon_index(g, s.i, s.j, s.k)] = (D)(c ? s1 : s0)[t];
+  }
 }
 
 // init_gridtype (mgpcg.py:46-60) on canonical int8 arrays; bit-exact.

@@ -1,20 +1,24 @@
 // Host side of the MI355X MGPCG solver and its C-ABI (include/mgpcg.h).
 //
-// One handle = one multigrid hierarchy resident in HBM in the colour-split
-// padded layout, the level-0 CG vectors, a device-resident CG state and the
-// reduction partials.  The solve loop of mgpcg.py:150-209 runs entirely on
-// the device: every scalar (alpha, beta, the residual test) is computed by
-// the last block of the reduction kernel that produces it, and every kernel
-// of an iteration returns at once when the state says the solve is done.
-// The host enqueues iterations as replays of a captured hipGraph and reads
-// the state back asynchronously, one burst behind, so the GPU never idles on
-// a host round trip.
+// mgpcg_solver  one system: its multigrid hierarchy resident in HBM in the
+//               colour-split padded layout, the level-0 CG vectors, a
+//               device-resident CG state and the reduction partials.
+// Engine        runs the PCG loop of mgpcg.py:150-209 for a set of systems at
+//               once (possibly of different shapes): every stage of an
+//               iteration is ONE persistent kernel launch over all member
+//               systems; every scalar (alpha, beta, the residual test) is
+//               computed on the device by a per-system finalize kernel.
+//               The host only enqueues iterations, one ahead of the state it
+//               reads back, and retires / admits systems at iteration
+//               boundaries -- which is also how the server does continuous
+//               batching.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <new>
 #include <vector>
 
@@ -45,37 +49,38 @@ struct LevelBuf {
   void* z[2];
 };
 
+int g_coarse_max_slots = 4096;  // levels at most this large go to k_coarse
+int g_max_grid = 2048;          // workgroups of a persistent stage launch
+
 }  // namespace
+
+struct Engine;
 
 struct mgpcg_solver {
   int dim = 3, L = 1, nu = 2, bottom = 10, precision = PLAIN;
   int res[3] = {0, 0, 0};
-  size_t esz = 4, xsz = 4;  // hierarchy / x element sizes
+  size_t esz = 4, xsz = 4;
   std::vector<Geo> geo;
   std::vector<LevelBuf> lv;
   void* b[2] = {nullptr, nullptr};
   void* x[2] = {nullptr, nullptr};
-  void* p[2] = {nullptr, nullptr};
+  void* p[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   void* Ap[2] = {nullptr, nullptr};
   CGState* st = nullptr;
   double* partials = nullptr;
   int nparts = 0;
+  int coarse_from = 0;
   char* pool = nullptr;
   size_t pool_bytes = 0;
-  CGState* host_st = nullptr;  // pinned, 2 slots
-  hipStream_t stream = nullptr;  // own stream (capturable)
   hipStream_t user_stream = nullptr;
-  hipEvent_t ev_user = nullptr, ev_done = nullptr, ev_slot[2] = {nullptr, nullptr};
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraph_t graph = nullptr;
-  int use_graph = 1;
-  int burst = 4;
+  Engine* engine = nullptr;  // private engine of the single-system API
   int device = 0;
 };
 
 namespace {
 
 inline int nblk(int n) { return (n + BLOCK - 1) / BLOCK; }
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 template <typename T>
 LevelPtrs<T> level_ptrs(const mgpcg_solver* s, int l) {
@@ -91,290 +96,575 @@ LevelPtrs<T> level_ptrs(const mgpcg_solver* s, int l) {
   return P;
 }
 
-template <typename T, typename TX>
-CGPtrs<T, TX> cg_ptrs(const mgpcg_solver* s) {
-  CGPtrs<T, TX> V;
-  for (int c = 0; c < 2; ++c) {
-    V.b[c] = (T*)s->b[c];
-    V.x[c] = (TX*)s->x[c];
-    V.p[c] = (T*)s->p[c];
-    V.Ap[c] = (T*)s->Ap[c];
+template <typename T>
+LevelDesc<T> level_desc(const mgpcg_solver* s, int l) {
+  LevelDesc<T> d;
+  d.g = s->geo[l];
+  d.P = level_ptrs<T>(s, l);
+  if (l + 1 < s->L) {
+    d.gc = s->geo[l + 1];
+    d.C = level_ptrs<T>(s, l + 1);
+  } else {
+    d.gc = s->geo[l];
+    d.C = d.P;
   }
-  return V;
+  d.st = s->st;
+  return d;
 }
 
+template <typename T, typename TX>
+SysDesc<T, TX> sys_desc(const mgpcg_solver* s, int desc0) {
+  SysDesc<T, TX> d;
+  d.g = s->geo[0];
+  d.P = level_ptrs<T>(s, 0);
+  for (int c = 0; c < 2; ++c) {
+    d.b[c] = (T*)s->b[c];
+    d.x[c] = (TX*)s->x[c];
+    d.p[0][c] = (T*)s->p[0][c];
+    d.p[1][c] = (T*)s->p[1][c];
+    d.Ap[c] = (T*)s->Ap[c];
+  }
+  d.st = s->st;
+  d.partials = s->partials;
+  d.nparts = s->nparts;
+  d.coarse_from = s->coarse_from;
+  d.L = s->L;
+  d.desc0 = desc0;
+  return d;
+}
+
+// gather the CG states of the listed systems into one contiguous array
+template <typename T, typename TX>
+__global__ void k_gather_states(const SysDesc<T, TX>* __restrict__ sys,
+                                const Item* __restrict__ items, int n, CGState* out) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  out[i] = *sys[items[i].desc].st;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Engine
+// ===========================================================================
+struct Member {
+  mgpcg_solver* s = nullptr;
+  int phase = 0;     // host view: 0 = its first iteration (pre-loop) is next
+  int active = 0;
+  int64_t tag = 0;
+  long started = 0;  // iteration index at admission
+  uint32_t gen = 0;  // bumped whenever the slot gets a new system
+};
+
+struct Engine {
+  int precision = -1, dim = 0, L = 0, nu = 0, bottom = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t user_stream = nullptr;
+  hipEvent_t ev_user = nullptr, ev_done = nullptr;
+  int device = 0;
+
+  std::vector<Member> mem;  // slot -> member
+  int cap = 0;              // table capacity (members)
+  char* d_levels = nullptr;  // LevelDesc<T>[cap * L]
+  char* d_sys = nullptr;     // SysDesc<T,TX>[cap]
+  size_t level_sz = 0, sys_sz = 0;
+
+  // item lists of the current configuration
+  std::vector<Item> items;
+  struct StageRef {
+    int off = 0, n = 0, nblocks = 0;
+  };
+  StageRef st_init, st_cg, fin_init, fin_cg, fin_all, st_dot_fused, st_dot_plain, st_coarse,
+      st_gather;
+  std::vector<StageRef> st_slot, st_restrict, st_bottom;  // per level
+  int n_items_cap = 0;
+  Item* d_items = nullptr;
+  static const int RING = 4;
+  Item* h_items[RING] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_items[RING] = {nullptr, nullptr, nullptr, nullptr};
+  int items_slot = 0;
+  bool dirty = true;
+
+  // state readback
+  CGState* d_states = nullptr;
+  CGState* h_states[2] = {nullptr, nullptr};
+  hipEvent_t ev_states[2] = {nullptr, nullptr};
+  std::vector<std::pair<int, uint32_t>> gathered[2];  // (member slot, generation) per readback
+  long iteration = 0;
+
+  // pinned staging of admissions (per member slot): descriptors, initial state
+  char* h_desc = nullptr;
+  CGState* h_stinit = nullptr;
+
+  ~Engine() { release(); }
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (h_desc) (void)hipHostFree(h_desc);
+    if (h_stinit) (void)hipHostFree(h_stinit);
+    h_desc = nullptr;
+    h_stinit = nullptr;
+    if (d_levels) (void)hipFree(d_levels);
+    if (d_sys) (void)hipFree(d_sys);
+    if (d_items) (void)hipFree(d_items);
+    if (d_states) (void)hipFree(d_states);
+    for (int i = 0; i < RING; ++i) {
+      if (h_items[i]) (void)hipHostFree(h_items[i]);
+      if (ev_items[i]) (void)hipEventDestroy(ev_items[i]);
+    }
+    for (int i = 0; i < 2; ++i) {
+      if (h_states[i]) (void)hipHostFree(h_states[i]);
+      if (ev_states[i]) (void)hipEventDestroy(ev_states[i]);
+    }
+    if (ev_user) (void)hipEventDestroy(ev_user);
+    if (ev_done) (void)hipEventDestroy(ev_done);
+    if (stream) (void)hipStreamDestroy(stream);
+    d_levels = d_sys = nullptr;
+    d_items = nullptr;
+    d_states = nullptr;
+    stream = nullptr;
+  }
+};
+
+namespace {
+
+int engine_init(Engine* e) {
+  MG_HIP(hipGetDevice(&e->device));
+  MG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  MG_HIP(hipEventCreateWithFlags(&e->ev_user, hipEventDisableTiming));
+  MG_HIP(hipEventCreateWithFlags(&e->ev_done, hipEventDisableTiming));
+  for (int i = 0; i < Engine::RING; ++i)
+    MG_HIP(hipEventCreateWithFlags(&e->ev_items[i], hipEventDisableTiming));
+  for (int i = 0; i < 2; ++i)
+    MG_HIP(hipEventCreateWithFlags(&e->ev_states[i], hipEventDisableTiming));
+  return 0;
+}
+
+// ordered after the caller's stream, and the caller's stream after us
+int engine_begin(Engine* e) {
+  MG_HIP(hipSetDevice(e->device));
+  MG_HIP(hipEventRecord(e->ev_user, e->user_stream));
+  MG_HIP(hipStreamWaitEvent(e->stream, e->ev_user, 0));
+  return 0;
+}
+int engine_end(Engine* e) {
+  MG_HIP(hipEventRecord(e->ev_done, e->stream));
+  MG_HIP(hipStreamWaitEvent(e->user_stream, e->ev_done, 0));
+  return 0;
+}
+
+size_t level_desc_size(int precision) {
+  return precision == F64 ? sizeof(LevelDesc<double>) : sizeof(LevelDesc<float>);
+}
+size_t sys_desc_size(int precision) {
+  if (precision == F64) return sizeof(SysDesc<double, double>);
+  if (precision == MIXED) return sizeof(SysDesc<float, double>);
+  return sizeof(SysDesc<float, float>);
+}
+
+int engine_reserve(Engine* e, int cap) {
+  if (cap <= e->cap) return 0;
+  int ncap = std::max(cap, std::max(8, 2 * e->cap));
+  MG_HIP(hipStreamSynchronize(e->stream));
+  char *nl = nullptr, *ns = nullptr;
+  MG_HIP(hipMalloc(&nl, e->level_sz * (size_t)ncap * e->L));
+  MG_HIP(hipMalloc(&ns, e->sys_sz * (size_t)ncap));
+  if (e->d_levels) {
+    MG_HIP(hipMemcpy(nl, e->d_levels, e->level_sz * (size_t)e->cap * e->L,
+                     hipMemcpyDeviceToDevice));
+    MG_HIP(hipMemcpy(ns, e->d_sys, e->sys_sz * (size_t)e->cap, hipMemcpyDeviceToDevice));
+    MG_HIP(hipFree(e->d_levels));
+    MG_HIP(hipFree(e->d_sys));
+  }
+  e->d_levels = nl;
+  e->d_sys = ns;
+  if (e->d_states) MG_HIP(hipFree(e->d_states));
+  MG_HIP(hipMalloc(&e->d_states, sizeof(CGState) * (size_t)ncap));
+  for (int i = 0; i < 2; ++i) {
+    if (e->h_states[i]) MG_HIP(hipHostFree(e->h_states[i]));
+    MG_HIP(hipHostMalloc((void**)&e->h_states[i], sizeof(CGState) * (size_t)ncap, 0));
+  }
+  const size_t dbytes = e->level_sz * e->L + e->sys_sz;
+  char* nd = nullptr;
+  CGState* nst = nullptr;
+  MG_HIP(hipHostMalloc((void**)&nd, dbytes * (size_t)ncap, 0));
+  MG_HIP(hipHostMalloc((void**)&nst, sizeof(CGState) * (size_t)ncap, 0));
+  if (e->h_desc) MG_HIP(hipHostFree(e->h_desc));
+  if (e->h_stinit) MG_HIP(hipHostFree(e->h_stinit));
+  e->h_desc = nd;
+  e->h_stinit = nst;
+  e->cap = ncap;
+  return 0;
+}
+
+// Write member slot `m`'s descriptors into the device tables, asynchronously
+// and in stream order (kernels still in flight for the slot's previous member
+// run first).  The pinned staging slot of `m` is not touched again before
+// this copy ran: a slot is reused only after its member retired, which the
+// host learns from a readback recorded after this copy.
+int engine_write_desc(Engine* e, int m) {
+  const mgpcg_solver* s = e->mem[m].s;
+  const size_t dbytes = e->level_sz * e->L + e->sys_sz;
+  char* buf = e->h_desc + dbytes * (size_t)m;
+  for (int l = 0; l < e->L; ++l) {
+    if (e->precision == F64) {
+      LevelDesc<double> d = level_desc<double>(s, l);
+      memcpy(buf + e->level_sz * l, &d, sizeof(d));
+    } else {
+      LevelDesc<float> d = level_desc<float>(s, l);
+      memcpy(buf + e->level_sz * l, &d, sizeof(d));
+    }
+  }
+  char* sd = buf + e->level_sz * e->L;
+  const int desc0 = m * e->L;
+  if (e->precision == F64) {
+    SysDesc<double, double> d = sys_desc<double, double>(s, desc0);
+    memcpy(sd, &d, sizeof(d));
+  } else if (e->precision == MIXED) {
+    SysDesc<float, double> d = sys_desc<float, double>(s, desc0);
+    memcpy(sd, &d, sizeof(d));
+  } else {
+    SysDesc<float, float> d = sys_desc<float, float>(s, desc0);
+    memcpy(sd, &d, sizeof(d));
+  }
+  MG_HIP(hipMemcpyAsync(e->d_levels + e->level_sz * (size_t)m * e->L, buf, e->level_sz * e->L,
+                        hipMemcpyHostToDevice, e->stream));
+  MG_HIP(hipMemcpyAsync(e->d_sys + e->sys_sz * (size_t)m, sd, e->sys_sz, hipMemcpyHostToDevice,
+                        e->stream));
+  return 0;
+}
+
+int engine_configure(Engine* e, const mgpcg_solver* s) {
+  if (e->precision >= 0) {
+    MG_REQUIRE(s->precision == e->precision && s->dim == e->dim && s->L == e->L &&
+                   s->nu == e->nu && s->bottom == e->bottom,
+               "all systems of a batch must share precision, dim, n_levels and smoothing counts");
+    return 0;
+  }
+  e->precision = s->precision;
+  e->dim = s->dim;
+  e->L = s->L;
+  e->nu = s->nu;
+  e->bottom = s->bottom;
+  e->level_sz = align_up(level_desc_size(e->precision), 16);
+  e->sys_sz = align_up(sys_desc_size(e->precision), 16);
+  e->st_slot.assign(e->L, {});
+  e->st_restrict.assign(e->L, {});
+  e->st_bottom.assign(1, {});
+  return 0;
+}
+
+struct SolveParams {
+  int max_iters;
+  double rel_tol, abs_tol, eps;
+  int refine_every;
+};
+
+// Initial CG state of member slot m (parameters, phase 0), written
+// asynchronously in stream order through the slot's pinned staging entry.
+int engine_reset_state(Engine* e, int m, const SolveParams& prm) {
+  mgpcg_solver* s = e->mem[m].s;
+  CGState& init = e->h_stinit[m];
+  memset(&init, 0, sizeof(init));
+  init.rel_tol = prm.rel_tol;
+  init.abs_tol = prm.abs_tol;
+  init.eps = prm.eps;
+  init.max_iters = prm.max_iters;
+  init.refine_every = prm.refine_every;
+  init.precision = s->precision;
+  init.phase = 0;
+  init.true_rr = -1.0;
+  MG_HIP(hipMemcpyAsync(s->st, &init, sizeof(CGState), hipMemcpyHostToDevice, e->stream));
+  return 0;
+}
+
+int engine_add(Engine* e, mgpcg_solver* s, int64_t tag, const SolveParams& prm) {
+  if (engine_configure(e, s)) return -1;
+  int slot = -1;
+  for (int i = 0; i < (int)e->mem.size(); ++i)
+    if (!e->mem[i].active && e->mem[i].s == nullptr) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) {
+    slot = (int)e->mem.size();
+    e->mem.push_back(Member());
+  }
+  if (engine_reserve(e, (int)e->mem.size())) return -2;
+  Member& m = e->mem[slot];
+  m.s = s;
+  m.phase = 0;
+  m.active = 1;
+  m.tag = tag;
+  m.started = e->iteration;
+  m.gen += 1;
+  if (engine_write_desc(e, slot)) return -2;
+  if (engine_reset_state(e, slot, prm)) return -2;
+  e->dirty = true;
+  return slot;
+}
+
+// Rebuild the item lists of every stage from the member set.
+int engine_build_items(Engine* e) {
+  std::vector<Item>& it = e->items;
+  it.clear();
+  auto add_stage = [&](Engine::StageRef& ref, auto pred, auto blocks, auto descof) {
+    ref.off = (int)it.size();
+    int b0 = 0;
+    for (int m = 0; m < (int)e->mem.size(); ++m) {
+      const Member& M = e->mem[m];
+      if (!M.active || !pred(M)) continue;
+      Item x;
+      x.desc = descof(m);
+      x.block0 = b0;
+      it.push_back(x);
+      b0 += blocks(M);
+    }
+    ref.n = (int)it.size() - ref.off;
+    ref.nblocks = b0;
+  };
+  const int L = e->L;
+  auto sysidx = [](int m) { return m; };
+  auto lvl0 = [](const Member& M) { return nblk(M.s->geo[0].size); };
+  auto one = [](const Member&) { return 1; };
+  add_stage(e->st_init, [](const Member& M) { return M.phase == 0; }, lvl0, sysidx);
+  add_stage(e->fin_init, [](const Member& M) { return M.phase == 0; }, one, sysidx);
+  add_stage(e->st_cg, [](const Member& M) { return M.phase == 1; }, lvl0, sysidx);
+  add_stage(e->fin_cg, [](const Member& M) { return M.phase == 1; }, one, sysidx);
+  add_stage(e->fin_all, [](const Member&) { return true; }, one, sysidx);
+  add_stage(e->st_gather, [](const Member&) { return true; }, one, sysidx);
+  for (int l = 0; l < L; ++l) {
+    add_stage(e->st_slot[l], [l](const Member& M) { return l < M.s->coarse_from; },
+              [l](const Member& M) { return nblk(M.s->geo[l].size); },
+              [l, L](int m) { return m * L + l; });
+    add_stage(e->st_restrict[l],
+              [l, L](const Member& M) { return l < M.s->coarse_from && l + 1 < L; },
+              [l, L](const Member& M) { return l + 1 < L ? nblk(M.s->geo[l + 1].size) : 0; },
+              [l, L](int m) { return m * L + l; });
+  }
+  // members whose bottom level is too large for the coarse kernel
+  add_stage(e->st_bottom[0], [L](const Member& M) { return M.s->coarse_from >= L; },
+            [L](const Member& M) { return nblk(M.s->geo[L - 1].size); },
+            [L](int m) { return m * L + L - 1; });
+  // coarse kernel: desc = first coarse level, block0 = number of coarse levels
+  e->st_coarse.off = (int)it.size();
+  for (int m = 0; m < (int)e->mem.size(); ++m) {
+    const Member& M = e->mem[m];
+    if (!M.active || M.s->coarse_from >= L) continue;
+    Item x;
+    x.desc = m * L + M.s->coarse_from;
+    x.block0 = L - M.s->coarse_from;
+    it.push_back(x);
+  }
+  e->st_coarse.n = (int)it.size() - e->st_coarse.off;
+  e->st_coarse.nblocks = e->st_coarse.n;
+  // z.r after the V-cycle: fused into level 0's last half-sweep when level 0
+  // is smoothed by the batched kernels, a plain dot otherwise
+  const bool fused_ok = e->nu > 0 && e->L > 1;
+  add_stage(e->st_dot_fused,
+            [fused_ok](const Member& M) { return fused_ok && M.s->coarse_from > 0; }, lvl0,
+            sysidx);
+  add_stage(e->st_dot_plain,
+            [fused_ok](const Member& M) { return !(fused_ok && M.s->coarse_from > 0); }, lvl0,
+            sysidx);
+
+  // upload through a pinned ring slot
+  const int n = (int)it.size();
+  if (n > e->n_items_cap) {
+    MG_HIP(hipStreamSynchronize(e->stream));
+    int ncap = std::max(n, 2 * e->n_items_cap + 64);
+    if (e->d_items) MG_HIP(hipFree(e->d_items));
+    MG_HIP(hipMalloc(&e->d_items, sizeof(Item) * ncap));
+    for (int i = 0; i < Engine::RING; ++i) {
+      if (e->h_items[i]) MG_HIP(hipHostFree(e->h_items[i]));
+      MG_HIP(hipHostMalloc((void**)&e->h_items[i], sizeof(Item) * ncap, 0));
+    }
+    e->n_items_cap = ncap;
+  }
+  const int sl = e->items_slot;
+  e->items_slot = (sl + 1) % Engine::RING;
+  MG_HIP(hipEventSynchronize(e->ev_items[sl]));
+  if (n > 0) {
+    memcpy(e->h_items[sl], it.data(), sizeof(Item) * n);
+    MG_HIP(hipMemcpyAsync(e->d_items, e->h_items[sl], sizeof(Item) * n, hipMemcpyHostToDevice,
+                          e->stream));
+  }
+  MG_HIP(hipEventRecord(e->ev_items[sl], e->stream));
+  e->dirty = false;
+  return 0;
+}
+
+inline Stage stage_of(const Engine* e, const Engine::StageRef& r) {
+  Stage s;
+  s.items = e->d_items + r.off;
+  s.nitems = r.n;
+  s.nblocks = r.nblocks;
+  return s;
+}
+inline int grid_of(const Engine::StageRef& r) { return std::min(r.nblocks, g_max_grid); }
+
 // ---------------------------------------------------------------------------
-// The solve schedule, templated on hierarchy type T, x type TX and dimension.
+// iteration schedule, templated on hierarchy type T, x type TX and dimension
 // ---------------------------------------------------------------------------
 template <typename T, typename TX, int DIM>
-struct Impl {
-  static void smooth(mgpcg_solver* s, int l, int phase, bool first, const int* done) {
-    const Geo& g = s->geo[l];
-    LevelPtrs<T> P = level_ptrs<T>(s, l);
-    if (first)
-      hipLaunchKernelGGL((k_smooth<T, DIM, true>), dim3(nblk(g.size)), dim3(BLOCK), 0, s->stream,
-                         g, P, phase, done);
-    else
-      hipLaunchKernelGGL((k_smooth<T, DIM, false>), dim3(nblk(g.size)), dim3(BLOCK), 0,
-                         s->stream, g, P, phase, done);
-  }
-  static void zero_z(mgpcg_solver* s, int l, const int* done) {
-    const Geo& g = s->geo[l];
-    hipLaunchKernelGGL((k_zero2<T>), dim3(nblk(g.size)), dim3(BLOCK), 0, s->stream, g.size,
-                       (T*)s->lv[l].z[0], (T*)s->lv[l].z[1], done);
-  }
-  static void restrict_(mgpcg_solver* s, int l, const int* done) {
-    const Geo& gf = s->geo[l];
-    const Geo& gc = s->geo[l + 1];
-    LevelPtrs<T> P = level_ptrs<T>(s, l);
-    hipLaunchKernelGGL((k_restrict<T, DIM>), dim3(2 * nblk(gc.size)), dim3(BLOCK), 0, s->stream,
-                       gf, gc, P, (T*)s->lv[l + 1].r[0], (T*)s->lv[l + 1].r[1], done);
-  }
-  static void prolongate(mgpcg_solver* s, int l, const int* done) {
-    const Geo& gf = s->geo[l];
-    const Geo& gc = s->geo[l + 1];
-    hipLaunchKernelGGL((k_prolongate<T, DIM>), dim3(2 * nblk(gf.size)), dim3(BLOCK), 0,
-                       s->stream, gf, gc, (T*)s->lv[l].z[0], (T*)s->lv[l].z[1],
-                       (const T*)s->lv[l + 1].z[0], (const T*)s->lv[l + 1].z[1], done);
-  }
-  // The smoothing block of one level: `pairs` x (smooth(a), smooth(b)).
-  // `from_zero`: z of the level is known to be zero before the block.
-  static void smooth_block(mgpcg_solver* s, int l, int pairs, int a, int b, bool from_zero,
-                           const int* done) {
-    if (pairs <= 0) {
-      if (from_zero) zero_z(s, l, done);
-      return;
-    }
-    for (int i = 0; i < pairs; ++i) {
-      smooth(s, l, a, from_zero && i == 0 && a == 0, done);
-      smooth(s, l, b, false, done);
-    }
-  }
-  // v_cycle (mgpcg.py:125-148)
-  static void vcycle(mgpcg_solver* s, const int* done) {
-    const int L = s->L;
-    for (int l = 0; l < L - 1; ++l) {
-      smooth_block(s, l, s->nu, 0, 1, true, done);
-      restrict_(s, l, done);
-    }
-    const int half = s->bottom / 2;
-    if (half > 0) {
-      smooth_block(s, L - 1, half, 0, 1, true, done);
-      smooth_block(s, L - 1, half, 1, 0, false, done);
-    } else {
-      zero_z(s, L - 1, done);
-    }
-    for (int l = L - 2; l >= 0; --l) {
-      prolongate(s, l, done);
-      smooth_block(s, l, s->nu, 1, 0, false, done);
-    }
-  }
-  static RedPtrs red(mgpcg_solver* s) { return RedPtrs{s->st, s->partials}; }
+struct Sched {
+  static const LevelDesc<T>* levels(const Engine* e) { return (const LevelDesc<T>*)e->d_levels; }
+  static const SysDesc<T, TX>* sys(const Engine* e) { return (const SysDesc<T, TX>*)e->d_sys; }
 
-  static void pre_loop(mgpcg_solver* s) {
-    const Geo& g = s->geo[0];
-    const int nb = 2 * nblk(g.size);
-    LevelPtrs<T> P = level_ptrs<T>(s, 0);
-    CGPtrs<T, TX> V = cg_ptrs<T, TX>(s);
-    hipLaunchKernelGGL((k_cg_init<T, TX>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V, red(s));
-    vcycle(s, nullptr);
-    hipLaunchKernelGGL((k_update_p<T, TX>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V,
-                       (const CGState*)s->st);
-    hipLaunchKernelGGL((k_dot_zr<T, true>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, red(s));
+  template <int OP>
+  static void stage(Engine* e, const Engine::StageRef& r) {
+    if (r.nblocks == 0) return;
+    hipLaunchKernelGGL((k_stage<T, DIM, OP>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
+                       levels(e), stage_of(e, r));
   }
-  // One loop body of mgpcg.py:176-209.
-  static void iteration(mgpcg_solver* s) {
-    const Geo& g = s->geo[0];
-    const int nb = 2 * nblk(g.size);
-    LevelPtrs<T> P = level_ptrs<T>(s, 0);
-    CGPtrs<T, TX> V = cg_ptrs<T, TX>(s);
-    const int* done = &s->st->done;
-    hipLaunchKernelGGL((k_compute_Ap<T, TX, DIM>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V,
-                       red(s));
-    hipLaunchKernelGGL((k_update_xr<T, TX>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V,
-                       red(s));
-    if (s->precision == MIXED)
-      hipLaunchKernelGGL((k_true_residual<T, TX, DIM>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
-                         P, V, red(s));
-    vcycle(s, done);
-    hipLaunchKernelGGL((k_dot_zr<T, false>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, red(s));
-    hipLaunchKernelGGL((k_update_p<T, TX>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V,
-                       (const CGState*)s->st);
+  static void restrict_(Engine* e, const Engine::StageRef& r) {
+    if (r.nblocks == 0) return;
+    hipLaunchKernelGGL((k_restrict<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
+                       levels(e), stage_of(e, r));
   }
-  // The reference's host loop (3 device->host syncs per iteration); the
-  // baseline the device-resident loop is measured against.
-  static int solve_hostsync(mgpcg_solver* s, int max_iters) {
-    const Geo& g = s->geo[0];
-    const int nb = 2 * nblk(g.size);
-    LevelPtrs<T> P = level_ptrs<T>(s, 0);
-    CGPtrs<T, TX> V = cg_ptrs<T, TX>(s);
-    CGState* h = s->host_st;
-    auto sync_state = [&]() -> int {
-      MG_HIP(hipMemcpyAsync(h, s->st, sizeof(CGState), hipMemcpyDeviceToHost, s->stream));
-      MG_HIP(hipStreamSynchronize(s->stream));
-      return 0;
-    };
-    hipLaunchKernelGGL((k_cg_init<T, TX>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V, red(s));
-    if (sync_state()) return -2;
-    vcycle(s, nullptr);
-    hipLaunchKernelGGL((k_update_p<T, TX>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V,
-                       (const CGState*)s->st);
-    hipLaunchKernelGGL((k_dot_zr<T, true>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, red(s));
-    if (sync_state()) return -2;
-    for (int it = 0; max_iters < 0 || it < max_iters + 1; ++it) {
-      hipLaunchKernelGGL((k_compute_Ap<T, TX, DIM>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P,
-                         V, red(s));
-      if (sync_state()) return -2;
-      hipLaunchKernelGGL((k_update_xr<T, TX>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V,
-                         red(s));
-      if (s->precision == MIXED)
-        hipLaunchKernelGGL((k_true_residual<T, TX, DIM>), dim3(nb), dim3(BLOCK), 0, s->stream,
-                           g, P, V, red(s));
-      if (sync_state()) return -2;
-      if (h->done) break;
-      vcycle(s, nullptr);
-      hipLaunchKernelGGL((k_dot_zr<T, false>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P,
-                         red(s));
-      if (sync_state()) return -2;
-      if (h->done) break;
-      hipLaunchKernelGGL((k_update_p<T, TX>), dim3(nb), dim3(BLOCK), 0, s->stream, g, P, V,
-                         (const CGState*)s->st);
+  template <int OP>
+  static void fin(Engine* e, const Engine::StageRef& r) {
+    if (r.n == 0) return;
+    hipLaunchKernelGGL((k_finalize<T, TX, OP>), dim3(r.n), dim3(BLOCK), 0, e->stream, sys(e),
+                       (const Item*)(e->d_items + r.off), r.n);
+  }
+
+  // v_cycle (mgpcg.py:125-148) for every member
+  static void vcycle(Engine* e) {
+    const int L = e->L, nu = e->nu;
+    for (int l = 0; l + 1 < L; ++l) {
+      const Engine::StageRef& S = e->st_slot[l];
+      if (nu > 0) {
+        stage<ST_PAIR_FROM_ZERO>(e, S);
+        for (int i = 1; i < nu; ++i) {
+          stage<ST_SMOOTH0>(e, S);
+          stage<ST_SMOOTH1>(e, S);
+        }
+      } else {
+        stage<ST_ZERO>(e, S);
+      }
+      restrict_(e, e->st_restrict[l]);
     }
-    return 0;
+    // bottom level on the batched kernels (only when it is too big for k_coarse)
+    const Engine::StageRef& B = e->st_bottom[0];
+    if (B.nblocks) {
+      const int half = e->bottom / 2;
+      if (half > 0) {
+        stage<ST_PAIR_FROM_ZERO>(e, B);
+        for (int i = 1; i < half; ++i) {
+          stage<ST_SMOOTH0>(e, B);
+          stage<ST_SMOOTH1>(e, B);
+        }
+        for (int i = 0; i < half; ++i) {
+          stage<ST_SMOOTH1>(e, B);
+          stage<ST_SMOOTH0>(e, B);
+        }
+      } else {
+        stage<ST_ZERO>(e, B);
+      }
+    }
+    if (e->st_coarse.n)
+      hipLaunchKernelGGL((k_coarse<T, DIM>), dim3(e->st_coarse.n), dim3(COARSE_THREADS), 0,
+                         e->stream, levels(e), (const Item*)(e->d_items + e->st_coarse.off),
+                         e->st_coarse.n, nu, e->bottom);
+    for (int l = L - 2; l >= 0; --l) {
+      const Engine::StageRef& S = e->st_slot[l];
+      if (nu > 0) {
+        stage<ST_PROLONG_SMOOTH1>(e, S);
+        for (int i = 0; i < nu; ++i) {
+          if (i > 0) stage<ST_SMOOTH1>(e, S);
+          const bool last = (i == nu - 1);
+          if (!(last && l == 0)) stage<ST_SMOOTH0>(e, S);
+        }
+      } else {
+        stage<ST_PROLONGATE>(e, S);
+      }
+    }
+    // level 0's last half-sweep fused with reduce(z, r), or a plain dot
+    if (e->st_dot_fused.nblocks)
+      hipLaunchKernelGGL((k_smooth0_dot<T, TX, DIM>), dim3(grid_of(e->st_dot_fused)),
+                         dim3(BLOCK), 0, e->stream, sys(e), stage_of(e, e->st_dot_fused));
+    if (e->st_dot_plain.nblocks)
+      hipLaunchKernelGGL((k_dot_zr<T, TX>), dim3(grid_of(e->st_dot_plain)), dim3(BLOCK), 0,
+                         e->stream, sys(e), stage_of(e, e->st_dot_plain));
+  }
+
+  // One iteration for every member: the pre-loop of newly admitted systems
+  // (mgpcg.py:157-172) and one loop body (:176-209) of the others.
+  static void iteration(Engine* e, bool sync_each) {
+    auto sync = [&]() {
+      if (sync_each) (void)hipStreamSynchronize(e->stream);
+    };
+    if (e->st_init.nblocks) {
+      hipLaunchKernelGGL((k_cg_init<T, TX>), dim3(grid_of(e->st_init)), dim3(BLOCK), 0,
+                         e->stream, sys(e), stage_of(e, e->st_init));
+      fin<FIN_INIT>(e, e->fin_init);
+      sync();
+    }
+    if (e->st_cg.nblocks) {
+      hipLaunchKernelGGL((k_cg_ap<T, TX, DIM>), dim3(grid_of(e->st_cg)), dim3(BLOCK), 0,
+                         e->stream, sys(e), stage_of(e, e->st_cg));
+      fin<FIN_ALPHA>(e, e->fin_cg);
+      sync();
+      hipLaunchKernelGGL((k_cg_xr<T, TX>), dim3(grid_of(e->st_cg)), dim3(BLOCK), 0, e->stream,
+                         sys(e), stage_of(e, e->st_cg));
+      fin<FIN_RR>(e, e->fin_cg);
+      if (e->precision == MIXED) {
+        hipLaunchKernelGGL((k_true_residual<T, TX, DIM>), dim3(grid_of(e->st_cg)), dim3(BLOCK),
+                           0, e->stream, sys(e), stage_of(e, e->st_cg));
+        fin<FIN_TRUE>(e, e->fin_cg);
+      }
+      sync();
+    }
+    vcycle(e);
+    fin<FIN_BETA>(e, e->fin_all);
+    sync();
+  }
+
+  static void gather(Engine* e, int slot) {
+    const Engine::StageRef& g = e->st_gather;
+    if (g.n == 0) return;
+    hipLaunchKernelGGL((k_gather_states<T, TX>), dim3((g.n + 63) / 64), dim3(64), 0, e->stream,
+                       sys(e), (const Item*)(e->d_items + g.off), g.n, e->d_states);
+    (void)hipMemcpyAsync(e->h_states[slot], e->d_states, sizeof(CGState) * g.n,
+                         hipMemcpyDeviceToHost, e->stream);
   }
 };
 
 template <template <typename, typename, int> class F, typename... Args>
-auto dispatch(mgpcg_solver* s, Args&&... args) {
-  if (s->precision == F64) {
-    if (s->dim == 3) return F<double, double, 3>::run(s, args...);
-    return F<double, double, 2>::run(s, args...);
+auto dispatch(int precision, int dim, Args&&... args) {
+  if (precision == F64) {
+    if (dim == 3) return F<double, double, 3>::run(args...);
+    return F<double, double, 2>::run(args...);
   }
-  if (s->precision == MIXED) {
-    if (s->dim == 3) return F<float, double, 3>::run(s, args...);
-    return F<float, double, 2>::run(s, args...);
+  if (precision == MIXED) {
+    if (dim == 3) return F<float, double, 3>::run(args...);
+    return F<float, double, 2>::run(args...);
   }
-  if (s->dim == 3) return F<float, float, 3>::run(s, args...);
-  return F<float, float, 2>::run(s, args...);
+  if (dim == 3) return F<float, float, 3>::run(args...);
+  return F<float, float, 2>::run(args...);
 }
 
 template <typename T, typename TX, int DIM>
-struct RunPre {
-  static int run(mgpcg_solver* s) {
-    Impl<T, TX, DIM>::pre_loop(s);
+struct RunIteration {
+  static int run(Engine* e, bool sync_each) {
+    Sched<T, TX, DIM>::iteration(e, sync_each);
     return 0;
   }
 };
 template <typename T, typename TX, int DIM>
-struct RunIter {
-  static int run(mgpcg_solver* s) {
-    Impl<T, TX, DIM>::iteration(s);
+struct RunGather {
+  static int run(Engine* e, int slot) {
+    Sched<T, TX, DIM>::gather(e, slot);
     return 0;
-  }
-};
-template <typename T, typename TX, int DIM>
-struct RunHostSync {
-  static int run(mgpcg_solver* s, int max_iters) {
-    return Impl<T, TX, DIM>::solve_hostsync(s, max_iters);
   }
 };
 template <typename T, typename TX, int DIM>
 struct RunVcycle {
-  static int run(mgpcg_solver* s) {
-    Impl<T, TX, DIM>::vcycle(s, nullptr);
+  static int run(Engine* e) {
+    Sched<T, TX, DIM>::vcycle(e);
     return 0;
   }
 };
-template <typename T, typename TX, int DIM>
-struct RunOp {
-  static int run(mgpcg_solver* s, int op, int level, int arg) {
-    using I = Impl<T, TX, DIM>;
-    switch (op) {
-      case MGPCG_OP_SMOOTH:
-        I::smooth(s, level, arg & 1, false, nullptr);
-        return 0;
-      case MGPCG_OP_SMOOTH_FROM_ZERO:
-        I::smooth(s, level, 0, true, nullptr);
-        return 0;
-      case MGPCG_OP_RESTRICT:
-        I::restrict_(s, level, nullptr);
-        return 0;
-      case MGPCG_OP_PROLONGATE:
-        I::prolongate(s, level, nullptr);
-        return 0;
-      case MGPCG_OP_VCYCLE:
-        I::vcycle(s, nullptr);
-        return 0;
-      case MGPCG_OP_COMPUTE_AP: {
-        // Ap = A p on level 0, via the CG kernel; the reduction result lands in the state.
-        const Geo& g = s->geo[0];
-        const int nb = 2 * nblk(g.size);
-        hipLaunchKernelGGL((k_compute_Ap<T, TX, DIM>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
-                           level_ptrs<T>(s, 0), cg_ptrs<T, TX>(s), I::red(s));
-        return 0;
-      }
-      default:
-        mg_set_error("unknown op %d", op);
-        return -1;
-    }
-  }
-};
-
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-int capture_iteration(mgpcg_solver* s) {
-  if (s->graph_exec) return 0;
-  MG_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  dispatch<RunIter>(s);
-  hipGraph_t graph = nullptr;
-  MG_HIP(hipStreamEndCapture(s->stream, &graph));
-  MG_HIP(hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0));
-  s->graph = graph;
-  return 0;
-}
-
-int enqueue_iteration(mgpcg_solver* s) {
-  if (s->use_graph) {
-    MG_HIP(hipGraphLaunch(s->graph_exec, s->stream));
-  } else {
-    dispatch<RunIter>(s);
-  }
-  return 0;
-}
-
-// The handle's own stream is non-blocking, so it is always ordered against
-// the caller's stream explicitly -- also when that is the legacy NULL stream.
-int begin(mgpcg_solver* s) {
-  MG_HIP(hipSetDevice(s->device));
-  MG_HIP(hipEventRecord(s->ev_user, s->user_stream));
-  MG_HIP(hipStreamWaitEvent(s->stream, s->ev_user, 0));
-  return 0;
-}
-
-int end(mgpcg_solver* s) {
-  MG_HIP(hipEventRecord(s->ev_done, s->stream));
-  MG_HIP(hipStreamWaitEvent(s->user_stream, s->ev_done, 0));
-  return 0;
-}
-
-int set_params(mgpcg_solver* s, int max_iters, double rel_tol, double abs_tol, double eps,
-               int refine_every) {
-  CGState init;
-  memset(&init, 0, sizeof(init));
-  init.rel_tol = rel_tol;
-  init.abs_tol = abs_tol;
-  init.eps = eps;
-  init.max_iters = max_iters;
-  init.refine_every = refine_every;
-  init.precision = s->precision;
-  // stage through pinned slot 1 (slot 0 may be read by the caller)
-  CGState* h = &s->host_st[1];
-  MG_HIP(hipStreamSynchronize(s->stream));
-  *h = init;
-  MG_HIP(hipMemcpyAsync(s->st, h, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
-  MG_HIP(hipStreamSynchronize(s->stream));
-  return 0;
-}
 
 void fill_result(const CGState& h, mgpcg_result_t* out) {
   if (!out) return;
@@ -388,10 +678,149 @@ void fill_result(const CGState& h, mgpcg_result_t* out) {
   out->beta = h.beta;
 }
 
+// Enqueue one iteration plus the readback of the states of its members.
+int engine_enqueue(Engine* e, int slot, bool sync_each) {
+  if (e->dirty && engine_build_items(e)) return -2;
+  dispatch<RunIteration>(e->precision, e->dim, e, sync_each);
+  dispatch<RunGather>(e->precision, e->dim, e, slot);
+  e->gathered[slot].clear();
+  for (int m = 0; m < (int)e->mem.size(); ++m)
+    if (e->mem[m].active) e->gathered[slot].push_back({m, e->mem[m].gen});
+  MG_HIP(hipEventRecord(e->ev_states[slot], e->stream));
+  MG_HIP(hipGetLastError());
+  // phase transitions are deterministic: a system's pre-loop takes one iteration
+  bool changed = false;
+  for (Member& M : e->mem)
+    if (M.active && M.phase == 0) {
+      M.phase = 1;
+      changed = true;
+    }
+  if (changed) e->dirty = true;
+  e->iteration++;
+  return 0;
+}
+
+struct Finished {
+  int member;
+  CGState st;
+};
+
+// Wait for readback `slot`, retire the members that are done.
+int engine_collect(Engine* e, int slot, std::vector<Finished>* done) {
+  MG_HIP(hipEventSynchronize(e->ev_states[slot]));
+  const auto& g = e->gathered[slot];
+  for (int i = 0; i < (int)g.size(); ++i) {
+    const CGState& st = e->h_states[slot][i];
+    Member& M = e->mem[g[i].first];
+    // a slot re-admitted since this readback holds another system now
+    if (M.active && M.gen == g[i].second && st.done) {
+      M.active = 0;
+      e->dirty = true;
+      if (done) done->push_back({g[i].first, st});
+    }
+  }
+  return 0;
+}
+
+int engine_active(const Engine* e) {
+  int n = 0;
+  for (const Member& M : e->mem) n += M.active;
+  return n;
+}
+
+// Run every member to completion (iterations pipelined one deep).
+int engine_run(Engine* e, long host_cap, std::vector<Finished>* done) {
+  int slot = 0;
+  bool inflight = false;
+  long launched = 0;
+  while (true) {
+    const int nact = engine_active(e);
+    if (nact > 0 && launched < host_cap) {
+      if (engine_enqueue(e, slot, false)) return -2;
+      launched++;
+      if (inflight && engine_collect(e, slot ^ 1, done)) return -2;
+      inflight = true;
+      slot ^= 1;
+    } else if (inflight) {
+      if (engine_collect(e, slot ^ 1, done)) return -2;
+      inflight = false;
+    } else {
+      break;
+    }
+  }
+  MG_HIP(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+// remove retired members from the table (keeps slots of active ones)
+void engine_clear_retired(Engine* e) {
+  for (Member& M : e->mem)
+    if (!M.active) M.s = nullptr;
+  e->dirty = true;
+}
+
+Engine* new_engine() {
+  Engine* e = new (std::nothrow) Engine();
+  if (!e) return nullptr;
+  if (engine_init(e)) {
+    delete e;
+    return nullptr;
+  }
+  return e;
+}
+
+// The private engine of a single system (holds only that system).
+int single_engine(mgpcg_solver* s, Engine** out) {
+  if (!s->engine) {
+    s->engine = new_engine();
+    MG_REQUIRE(s->engine != nullptr, "engine creation failed: %s", g_err);
+  }
+  Engine* e = s->engine;
+  e->user_stream = s->user_stream;
+  *out = e;
+  return 0;
+}
+
+int solve_single(mgpcg_solver* s, int max_iters, double rel_tol, double abs_tol, double eps,
+                 int refine_every, mgpcg_result_t* out, bool hostsync) {
+  Engine* e = nullptr;
+  if (single_engine(s, &e)) return -1;
+  if (engine_begin(e)) return -2;
+  for (Member& M : e->mem) {
+    M.active = 0;
+    M.s = nullptr;
+  }
+  e->mem.clear();
+  const SolveParams prm{max_iters, rel_tol, abs_tol, eps, refine_every};
+  if (engine_add(e, s, 0, prm) < 0) return -2;
+  std::vector<Finished> done;
+  const long cap = max_iters >= 0 ? (long)max_iters + 4 : 10000000L;
+  if (!hostsync) {
+    if (engine_run(e, cap, &done)) return -2;
+  } else {
+    // the reference's host loop: a round trip after every reduction
+    for (long it = 0; it < cap && engine_active(e); ++it) {
+      if (engine_enqueue(e, 0, true)) return -2;
+      if (engine_collect(e, 0, &done)) return -2;
+    }
+  }
+  if (done.empty()) {
+    // host cap reached: report the last state
+    MG_HIP(hipStreamSynchronize(e->stream));
+    CGState st;
+    MG_HIP(hipMemcpy(&st, s->st, sizeof(CGState), hipMemcpyDeviceToHost));
+    fill_result(st, out);
+  } else {
+    fill_result(done.back().st, out);
+  }
+  MG_HIP(hipGetLastError());
+  return engine_end(e);
+}
+
 }  // namespace
 
 // ===========================================================================
-// C-ABI
+// C-ABI: systems
 // ===========================================================================
 extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, int bottom,
                             int precision, mgpcg_solver_t** out) {
@@ -429,7 +858,6 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     return -2;
   }
 
-  // geometry and pool layout
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -440,11 +868,13 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     size_t gt[2], Ad[2], Ax[3][2], r[2], z[2];
   };
   std::vector<Offs> lo;
+  s->coarse_from = n_levels;
   for (int l = 0; l < n_levels; ++l) {
     int n[3] = {0, 0, 0};
     for (int a = 0; a < dim; ++a) n[a] = res[a] >> l;
     s->geo.push_back(make_geo(dim, n));
     const Geo& g = s->geo.back();
+    if (s->coarse_from == n_levels && g.size <= g_coarse_max_slots) s->coarse_from = l;
     Offs o;
     for (int c = 0; c < 2; ++c) {
       o.gt[c] = take(g.size);
@@ -456,15 +886,16 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     lo.push_back(o);
   }
   const Geo& g0 = s->geo[0];
-  size_t ob[2], ox[2], op[2], oap[2];
+  size_t ob[2], ox[2], op[2][2], oap[2];
   for (int c = 0; c < 2; ++c) {
     ob[c] = take(g0.size * s->esz);
     ox[c] = take(g0.size * s->xsz);
-    op[c] = take(g0.size * s->esz);
+    op[0][c] = take(g0.size * s->esz);
+    op[1][c] = take(g0.size * s->esz);
     oap[c] = take(g0.size * s->esz);
   }
   size_t ost = take(sizeof(CGState));
-  s->nparts = 2 * nblk(g0.size);
+  s->nparts = nblk(g0.size);
   size_t opart = take(sizeof(double) * s->nparts);
   s->pool_bytes = off;
 
@@ -486,33 +917,20 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       B.z[c] = s->pool + lo[l].z[c];
     }
     s->lv.push_back(B);
+    const Geo& g = s->geo[l];
+    // ghost cells are SOLID (the zero fill made them FLUID)
+    if ((e = hipMemset(B.gt[0], MG_SOLID, g.size)) != hipSuccess) return fail(e, "memset");
+    if ((e = hipMemset(B.gt[1], MG_SOLID, g.size)) != hipSuccess) return fail(e, "memset");
   }
   for (int c = 0; c < 2; ++c) {
     s->b[c] = s->pool + ob[c];
     s->x[c] = s->pool + ox[c];
-    s->p[c] = s->pool + op[c];
+    s->p[0][c] = s->pool + op[0][c];
+    s->p[1][c] = s->pool + op[1][c];
     s->Ap[c] = s->pool + oap[c];
   }
   s->st = (CGState*)(s->pool + ost);
   s->partials = (double*)(s->pool + opart);
-  // ghost-layer invariants: gt of ghosts must be SOLID (memset gave FLUID=0)
-  for (int l = 0; l < n_levels; ++l) {
-    const Geo& g = s->geo[l];
-    if ((e = hipMemset(s->lv[l].gt[0], MG_SOLID, g.size)) != hipSuccess) return fail(e, "memset");
-    if ((e = hipMemset(s->lv[l].gt[1], MG_SOLID, g.size)) != hipSuccess) return fail(e, "memset");
-  }
-  if ((e = hipHostMalloc((void**)&s->host_st, 2 * sizeof(CGState), 0)) != hipSuccess)
-    return fail(e, "hipHostMalloc");
-  memset(s->host_st, 0, 2 * sizeof(CGState));
-  if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess)
-    return fail(e, "hipStreamCreate");
-  if ((e = hipEventCreateWithFlags(&s->ev_user, hipEventDisableTiming)) != hipSuccess)
-    return fail(e, "hipEventCreate");
-  if ((e = hipEventCreateWithFlags(&s->ev_done, hipEventDisableTiming)) != hipSuccess)
-    return fail(e, "hipEventCreate");
-  for (int i = 0; i < 2; ++i)
-    if ((e = hipEventCreateWithFlags(&s->ev_slot[i], hipEventDisableTiming)) != hipSuccess)
-      return fail(e, "hipEventCreate");
   if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "hipDeviceSynchronize");
   *out = s;
   return 0;
@@ -520,16 +938,14 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
 
 extern "C" int mgpcg_destroy(mgpcg_solver_t* s) {
   if (!s) return 0;
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
-  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
-  if (s->ev_user) (void)hipEventDestroy(s->ev_user);
-  if (s->ev_done) (void)hipEventDestroy(s->ev_done);
-  for (int i = 0; i < 2; ++i)
-    if (s->ev_slot[i]) (void)hipEventDestroy(s->ev_slot[i]);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
-  if (s->host_st) (void)hipHostFree(s->host_st);
-  if (s->pool) (void)hipFree(s->pool);
+  if (s->engine) {
+    delete s->engine;
+    s->engine = nullptr;
+  }
+  if (s->pool) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(s->pool);
+  }
   delete s;
   return 0;
 }
@@ -537,20 +953,22 @@ extern "C" int mgpcg_destroy(mgpcg_solver_t* s) {
 extern "C" int mgpcg_set_stream(mgpcg_solver_t* s, void* stream) {
   MG_REQUIRE(s != nullptr, "solver is NULL");
   s->user_stream = (hipStream_t)stream;
+  if (s->engine) s->engine->user_stream = s->user_stream;
   return 0;
 }
 
 extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
-  MG_REQUIRE(s != nullptr, "solver is NULL");
   switch (option) {
-    case MGPCG_OPT_USE_GRAPH:
-      s->use_graph = value != 0;
+    case MGPCG_OPT_COARSE_MAX_SLOTS:
+      MG_REQUIRE(value >= 0, "coarse slot limit must be >= 0");
+      g_coarse_max_slots = value;
       return 0;
-    case MGPCG_OPT_BURST:
-      MG_REQUIRE(value >= 1 && value <= 64, "burst must be in [1, 64]");
-      s->burst = value;
+    case MGPCG_OPT_MAX_GRID:
+      MG_REQUIRE(value >= 8, "grid limit must be >= 8");
+      g_max_grid = value;
       return 0;
     default:
+      (void)s;
       mg_set_error("unknown option %d", option);
       return -1;
   }
@@ -566,137 +984,94 @@ extern "C" int mgpcg_info(const mgpcg_solver_t* s, int level, int32_t* out6, int
     out6[2] = g.W;
     out6[3] = g.size;
     out6[4] = g.ncells;
-    out6[5] = s->nparts;
+    out6[5] = s->coarse_from;
   }
   if (bytes) *bytes = (int64_t)s->pool_bytes;
   return 0;
 }
+
+namespace {
+// short stream-ordered section on the system's caller stream
+struct UserOrder {
+  hipStream_t st;
+  explicit UserOrder(mgpcg_solver* s) : st(s->user_stream) {}
+};
+}  // namespace
 
 extern "C" int mgpcg_load_level(mgpcg_solver_t* s, int level, const int8_t* grid_type,
                                 const float* Adiag, const float* Ax) {
   MG_REQUIRE(s != nullptr, "solver is NULL");
   MG_REQUIRE(level >= 0 && level < s->L, "level %d out of range [0, %d)", level, s->L);
   MG_REQUIRE(grid_type && Adiag && Ax, "NULL field pointer");
-  if (begin(s)) return -2;
+  MG_HIP(hipSetDevice(s->device));
   const Geo& g = s->geo[level];
-  const LevelBuf& B = s->lv[level];
-  const int nb = 2 * nblk(g.size);
-  if (s->precision == F64) {
-    typedef double T;
-    hipLaunchKernelGGL((k_pack_level<T>), dim3(nb), dim3(BLOCK), 0, s->stream, g, grid_type,
-                       Adiag, Ax, B.gt[0], B.gt[1], (T*)B.Adiag[0], (T*)B.Adiag[1],
-                       (T*)B.Ax[0][0], (T*)B.Ax[0][1], (T*)B.Ax[1][0], (T*)B.Ax[1][1],
-                       (T*)B.Ax[2][0], (T*)B.Ax[2][1], (T*)B.r[0], (T*)B.r[1], (T*)B.z[0],
-                       (T*)B.z[1]);
-  } else {
-    typedef float T;
-    hipLaunchKernelGGL((k_pack_level<T>), dim3(nb), dim3(BLOCK), 0, s->stream, g, grid_type,
-                       Adiag, Ax, B.gt[0], B.gt[1], (T*)B.Adiag[0], (T*)B.Adiag[1],
-                       (T*)B.Ax[0][0], (T*)B.Ax[0][1], (T*)B.Ax[1][0], (T*)B.Ax[1][1],
-                       (T*)B.Ax[2][0], (T*)B.Ax[2][1], (T*)B.r[0], (T*)B.r[1], (T*)B.z[0],
-                       (T*)B.z[1]);
-  }
+  if (s->precision == F64)
+    hipLaunchKernelGGL((k_pack_level<double>), dim3(nblk(g.size)), dim3(BLOCK), 0,
+                       s->user_stream, g, grid_type, Adiag, Ax, level_ptrs<double>(s, level));
+  else
+    hipLaunchKernelGGL((k_pack_level<float>), dim3(nblk(g.size)), dim3(BLOCK), 0,
+                       s->user_stream, g, grid_type, Adiag, Ax, level_ptrs<float>(s, level));
   MG_HIP(hipGetLastError());
-  return end(s);
+  return 0;
 }
 
 extern "C" int mgpcg_load_rhs(mgpcg_solver_t* s, const float* b) {
   MG_REQUIRE(s != nullptr, "solver is NULL");
   MG_REQUIRE(b != nullptr, "b is NULL");
-  if (begin(s)) return -2;
+  MG_HIP(hipSetDevice(s->device));
   const Geo& g = s->geo[0];
-  const int nb = 2 * nblk(g.size);
   if (s->precision == F64)
-    hipLaunchKernelGGL((k_pack_vec<double>), dim3(nb), dim3(BLOCK), 0, s->stream, g, b,
-                       (double*)s->b[0], (double*)s->b[1]);
+    hipLaunchKernelGGL((k_pack_vec<float, double>), dim3(nblk(g.size)), dim3(BLOCK), 0,
+                       s->user_stream, g, b, (double*)s->b[0], (double*)s->b[1]);
   else
-    hipLaunchKernelGGL((k_pack_vec<float>), dim3(nb), dim3(BLOCK), 0, s->stream, g, b,
-                       (float*)s->b[0], (float*)s->b[1]);
+    hipLaunchKernelGGL((k_pack_vec<float, float>), dim3(nblk(g.size)), dim3(BLOCK), 0,
+                       s->user_stream, g, b, (float*)s->b[0], (float*)s->b[1]);
   MG_HIP(hipGetLastError());
-  return end(s);
+  return 0;
 }
 
 extern "C" int mgpcg_solve(mgpcg_solver_t* s, int max_iters, double rel_tol, double abs_tol,
                            double eps, int refine_every, mgpcg_result_t* out) {
   MG_REQUIRE(s != nullptr, "solver is NULL");
-  if (begin(s)) return -2;
-  if (set_params(s, max_iters, rel_tol, abs_tol, eps, refine_every)) return -2;
-  if (s->use_graph && capture_iteration(s)) return -2;
-  dispatch<RunPre>(s);
-  MG_HIP(hipGetLastError());
-  // Bursts of iterations; the state of burst k is read while burst k+1 runs.
-  const long hard_cap = max_iters >= 0 ? (long)max_iters + 2 * s->burst + 2 : 1000000L;
-  long issued = 0;
-  int slot = 0;
-  auto issue_burst = [&](int sl) -> int {
-    for (int i = 0; i < s->burst; ++i)
-      if (enqueue_iteration(s)) return -2;
-    issued += s->burst;
-    MG_HIP(hipMemcpyAsync(&s->host_st[sl], s->st, sizeof(CGState), hipMemcpyDeviceToHost,
-                          s->stream));
-    MG_HIP(hipEventRecord(s->ev_slot[sl], s->stream));
-    return 0;
-  };
-  if (issue_burst(slot)) return -2;
-  CGState final_state;
-  for (;;) {
-    if (issued < hard_cap && issue_burst(slot ^ 1)) return -2;
-    MG_HIP(hipEventSynchronize(s->ev_slot[slot]));
-    if (s->host_st[slot].done || issued >= hard_cap) {
-      final_state = s->host_st[slot];
-      break;
-    }
-    slot ^= 1;
-  }
-  MG_HIP(hipStreamSynchronize(s->stream));
-  if (!final_state.done) {
-    final_state = s->host_st[slot ^ 1];
-  }
-  fill_result(final_state, out);
-  MG_HIP(hipGetLastError());
-  return end(s);
+  return solve_single(s, max_iters, rel_tol, abs_tol, eps, refine_every, out, false);
 }
 
 extern "C" int mgpcg_solve_hostsync(mgpcg_solver_t* s, int max_iters, double rel_tol,
                                     double abs_tol, double eps, int refine_every,
                                     mgpcg_result_t* out) {
   MG_REQUIRE(s != nullptr, "solver is NULL");
-  if (begin(s)) return -2;
-  if (set_params(s, max_iters, rel_tol, abs_tol, eps, refine_every)) return -2;
-  int rc = dispatch<RunHostSync>(s, max_iters);
-  if (rc) return rc;
-  MG_HIP(hipStreamSynchronize(s->stream));
-  fill_result(s->host_st[0], out);
-  MG_HIP(hipGetLastError());
-  return end(s);
+  return solve_single(s, max_iters, rel_tol, abs_tol, eps, refine_every, out, true);
 }
 
 extern "C" int mgpcg_store_x(mgpcg_solver_t* s, float* x_f32, double* x_f64) {
   MG_REQUIRE(s != nullptr, "solver is NULL");
   MG_REQUIRE((x_f32 != nullptr) != (x_f64 != nullptr), "exactly one output pointer required");
-  if (begin(s)) return -2;
+  MG_HIP(hipSetDevice(s->device));
   const Geo& g = s->geo[0];
-  const int nb = 2 * nblk(g.size);
+  const dim3 gr(nblk(g.size)), bl(BLOCK);
+  hipStream_t st = s->user_stream;
   if (s->xsz == 4) {
     if (x_f32)
-      hipLaunchKernelGGL((k_unpack_vec<float, float>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
-                         (const float*)s->x[0], (const float*)s->x[1], x_f32);
+      hipLaunchKernelGGL((k_unpack_vec<float, float>), gr, bl, 0, st, g, (const float*)s->x[0],
+                         (const float*)s->x[1], x_f32);
     else
-      hipLaunchKernelGGL((k_unpack_vec<float, double>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
-                         (const float*)s->x[0], (const float*)s->x[1], x_f64);
+      hipLaunchKernelGGL((k_unpack_vec<float, double>), gr, bl, 0, st, g, (const float*)s->x[0],
+                         (const float*)s->x[1], x_f64);
   } else {
     if (x_f32)
-      hipLaunchKernelGGL((k_unpack_vec<double, float>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
+      hipLaunchKernelGGL((k_unpack_vec<double, float>), gr, bl, 0, st, g,
                          (const double*)s->x[0], (const double*)s->x[1], x_f32);
     else
-      hipLaunchKernelGGL((k_unpack_vec<double, double>), dim3(nb), dim3(BLOCK), 0, s->stream,
-                         g, (const double*)s->x[0], (const double*)s->x[1], x_f64);
+      hipLaunchKernelGGL((k_unpack_vec<double, double>), gr, bl, 0, st, g,
+                         (const double*)s->x[0], (const double*)s->x[1], x_f64);
   }
   MG_HIP(hipGetLastError());
-  return end(s);
+  return 0;
 }
 
-// Debug / parity access: move a level vector between canonical and internal layout.
+// ===========================================================================
+// C-ABI: kernel-level access for parity tests
+// ===========================================================================
 static int vec_slots(mgpcg_solver* s, int field, int level, void** c0, void** c1, size_t* esz) {
   MG_REQUIRE(level >= 0 && level < s->L, "level %d out of range", level);
   *esz = s->esz;
@@ -710,10 +1085,13 @@ static int vec_slots(mgpcg_solver* s, int field, int level, void** c0, void** c1
       *c1 = s->lv[level].z[1];
       return 0;
     case MGPCG_FIELD_P:
+    case MGPCG_FIELD_P_NEXT: {
       MG_REQUIRE(level == 0, "p lives on level 0");
-      *c0 = s->p[0];
-      *c1 = s->p[1];
+      const int par = field == MGPCG_FIELD_P ? 0 : 1;
+      *c0 = s->p[par][0];
+      *c1 = s->p[par][1];
       return 0;
+    }
     case MGPCG_FIELD_AP:
       MG_REQUIRE(level == 0, "Ap lives on level 0");
       *c0 = s->Ap[0];
@@ -737,30 +1115,15 @@ extern "C" int mgpcg_debug_get(mgpcg_solver_t* s, int field, int level, void* ca
   size_t esz;
   int rc = vec_slots(s, field, level, &c0, &c1, &esz);
   if (rc) return rc;
-  if (begin(s)) return -2;
   const Geo& g = s->geo[level];
-  const int nb = 2 * nblk(g.size);
   if (esz == 4)
-    hipLaunchKernelGGL((k_unpack_vec<float, float>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
-                       (const float*)c0, (const float*)c1, (float*)canon);
+    hipLaunchKernelGGL((k_unpack_vec<float, float>), dim3(nblk(g.size)), dim3(BLOCK), 0,
+                       s->user_stream, g, (const float*)c0, (const float*)c1, (float*)canon);
   else
-    hipLaunchKernelGGL((k_unpack_vec<double, double>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
-                       (const double*)c0, (const double*)c1, (double*)canon);
+    hipLaunchKernelGGL((k_unpack_vec<double, double>), dim3(nblk(g.size)), dim3(BLOCK), 0,
+                       s->user_stream, g, (const double*)c0, (const double*)c1, (double*)canon);
   MG_HIP(hipGetLastError());
-  return end(s);
-}
-
-template <typename T>
-__global__ void k_pack_any(Geo g, const T* __restrict__ src, T* d0, T* d1) {
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int c = lb & 1;
-  const int t = (lb >> 1) * BLOCK + threadIdx.x;
-  if (t >= g.size) return;
-  Slot sl = decode(g, t, c);
-  T v = (T)0;
-  if (sl.interior) v = src[canon_index(g, sl.i, sl.j, sl.k)];
-  (c ? d1 : d0)[t] = v;
+  return 0;
 }
 
 extern "C" int mgpcg_debug_set(mgpcg_solver_t* s, int field, int level, const void* canon) {
@@ -769,41 +1132,333 @@ extern "C" int mgpcg_debug_set(mgpcg_solver_t* s, int field, int level, const vo
   size_t esz;
   int rc = vec_slots(s, field, level, &c0, &c1, &esz);
   if (rc) return rc;
-  if (begin(s)) return -2;
   const Geo& g = s->geo[level];
-  const int nb = 2 * nblk(g.size);
   if (esz == 4)
-    hipLaunchKernelGGL((k_pack_any<float>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
-                       (const float*)canon, (float*)c0, (float*)c1);
+    hipLaunchKernelGGL((k_pack_vec<float, float>), dim3(nblk(g.size)), dim3(BLOCK), 0,
+                       s->user_stream, g, (const float*)canon, (float*)c0, (float*)c1);
   else
-    hipLaunchKernelGGL((k_pack_any<double>), dim3(nb), dim3(BLOCK), 0, s->stream, g,
-                       (const double*)canon, (double*)c0, (double*)c1);
+    hipLaunchKernelGGL((k_pack_vec<double, double>), dim3(nblk(g.size)), dim3(BLOCK), 0,
+                       s->user_stream, g, (const double*)canon, (double*)c0, (double*)c1);
   MG_HIP(hipGetLastError());
-  return end(s);
+  return 0;
 }
+
+namespace {
+
+// A one-item stage over level `level` of member 0 (item table slot 0).
+int prepare_op(Engine* e, const mgpcg_solver* s, int op, int level, Engine::StageRef* r) {
+  Item it;
+  it.desc = level;  // member 0: LevelDesc index = level
+  it.block0 = 0;
+  int nb = nblk(s->geo[level].size);
+  if (op == MGPCG_OP_RESTRICT) nb = nblk(s->geo[level + 1].size);
+  if (op == MGPCG_OP_COMPUTE_AP) it.desc = 0;  // SysDesc of member 0
+  MG_HIP(hipStreamSynchronize(e->stream));
+  MG_HIP(hipMemcpy(e->d_items, &it, sizeof(Item), hipMemcpyHostToDevice));
+  r->off = 0;
+  r->n = 1;
+  r->nblocks = nb;
+  e->dirty = true;  // the item table no longer matches the member set
+  return 0;
+}
+
+// Launch one stage on the prepared one-item table.
+template <typename T, typename TX, int DIM>
+struct RunOp {
+  static int run(Engine* e, int op, const Engine::StageRef& r, int arg) {
+    using Sd = Sched<T, TX, DIM>;
+    const int nb = r.nblocks;
+    switch (op) {
+      case MGPCG_OP_SMOOTH:
+        if (arg & 1) Sd::template stage<ST_SMOOTH1>(e, r);
+        else Sd::template stage<ST_SMOOTH0>(e, r);
+        return 0;
+      case MGPCG_OP_SMOOTH_PAIR_FROM_ZERO:
+        Sd::template stage<ST_PAIR_FROM_ZERO>(e, r);
+        return 0;
+      case MGPCG_OP_PROLONG_SMOOTH1:
+        Sd::template stage<ST_PROLONG_SMOOTH1>(e, r);
+        return 0;
+      case MGPCG_OP_RESTRICT:
+        Sd::restrict_(e, r);
+        return 0;
+      case MGPCG_OP_PROLONGATE:
+        Sd::template stage<ST_PROLONGATE>(e, r);
+        return 0;
+      case MGPCG_OP_COMPUTE_AP:
+        hipLaunchKernelGGL((k_cg_ap<T, TX, DIM>), dim3(std::min(nb, g_max_grid)), dim3(BLOCK), 0,
+                           e->stream, Sd::sys(e), stage_of(e, r));
+        return 0;
+      default:
+        mg_set_error("unknown op %d", op);
+        return -1;
+    }
+  }
+};
+
+}  // namespace
 
 extern "C" int mgpcg_debug_op(mgpcg_solver_t* s, int op, int level, int arg) {
   MG_REQUIRE(s != nullptr, "solver is NULL");
   MG_REQUIRE(level >= 0 && level < s->L, "level %d out of range", level);
-  if ((op == MGPCG_OP_RESTRICT || op == MGPCG_OP_PROLONGATE))
+  if (op == MGPCG_OP_RESTRICT || op == MGPCG_OP_PROLONGATE || op == MGPCG_OP_PROLONG_SMOOTH1)
     MG_REQUIRE(level < s->L - 1, "no coarser level below %d", level);
-  if (begin(s)) return -2;
-  if (op == MGPCG_OP_COMPUTE_AP) {
-    // the fused dot needs a fresh reduction ticket
-    CGState init;
-    memset(&init, 0, sizeof(init));
-    init.precision = s->precision;
-    init.eps = 1e-12;
-    CGState* h = &s->host_st[1];
-    MG_HIP(hipStreamSynchronize(s->stream));
-    *h = init;
-    MG_HIP(hipMemcpyAsync(s->st, h, sizeof(CGState), hipMemcpyHostToDevice, s->stream));
+  Engine* e = nullptr;
+  if (single_engine(s, &e)) return -1;
+  if (engine_begin(e)) return -2;
+  // a fresh state (not done, iteration 0, beta 0) and a one-member table
+  for (Member& M : e->mem) {
+    M.active = 0;
+    M.s = nullptr;
   }
-  int rc = dispatch<RunOp>(s, op, level, arg);
-  if (rc) return rc;
+  e->mem.clear();
+  const SolveParams prm{500, 1e-12, 1e-14, 1e-12, 10};
+  if (engine_add(e, s, 0, prm) < 0) return -2;
+  if (engine_build_items(e)) return -2;
+  MG_HIP(hipStreamSynchronize(e->stream));
+  if (op == MGPCG_OP_VCYCLE) {
+    dispatch<RunVcycle>(e->precision, e->dim, e);
+  } else {
+    Engine::StageRef r;
+    if (prepare_op(e, s, op, level, &r)) return -2;
+    int rc = dispatch<RunOp>(e->precision, e->dim, e, op, r, arg);
+    if (rc) return rc;
+  }
   MG_HIP(hipGetLastError());
-  MG_HIP(hipStreamSynchronize(s->stream));
-  return end(s);
+  MG_HIP(hipStreamSynchronize(e->stream));
+  e->dirty = true;  // the item table was overwritten
+  return engine_end(e);
+}
+
+extern "C" int mgpcg_time_op(mgpcg_solver_t* s, int op, int level, int arg, int reps,
+                             double* ms_per_launch) {
+  MG_REQUIRE(s != nullptr && ms_per_launch != nullptr, "NULL argument");
+  MG_REQUIRE(reps >= 1, "reps must be >= 1");
+  MG_REQUIRE(op == MGPCG_OP_SMOOTH || op == MGPCG_OP_RESTRICT || op == MGPCG_OP_VCYCLE ||
+                 op == MGPCG_OP_PROLONG_SMOOTH1 || op == MGPCG_OP_SMOOTH_PAIR_FROM_ZERO,
+             "op %d cannot be timed", op);
+  // warm-up launch (also sets up the one-member table)
+  int rc = mgpcg_debug_op(s, op, level, arg);
+  if (rc) return rc;
+  Engine* e = s->engine;
+  hipEvent_t e0, e1;
+  MG_HIP(hipEventCreate(&e0));
+  MG_HIP(hipEventCreate(&e1));
+  Engine::StageRef r;
+  if (op != MGPCG_OP_VCYCLE) {
+    if (prepare_op(e, s, op, level, &r)) return -2;
+  } else if (engine_build_items(e)) {
+    return -2;
+  }
+  MG_HIP(hipStreamSynchronize(e->stream));
+  MG_HIP(hipEventRecord(e0, e->stream));
+  for (int i = 0; i < reps; ++i) {
+    if (op == MGPCG_OP_VCYCLE) {
+      dispatch<RunVcycle>(e->precision, e->dim, e);
+    } else {
+      rc = dispatch<RunOp>(e->precision, e->dim, e, op, r, arg);
+      if (rc) return rc;
+    }
+  }
+  MG_HIP(hipEventRecord(e1, e->stream));
+  MG_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  MG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_launch = (double)ms / reps;
+  MG_HIP(hipEventDestroy(e0));
+  MG_HIP(hipEventDestroy(e1));
+  e->dirty = true;
+  MG_HIP(hipGetLastError());
+  return 0;
+}
+
+// ===========================================================================
+// C-ABI: batches and the server
+// ===========================================================================
+struct mgpcg_batch {
+  Engine* e = nullptr;
+};
+
+extern "C" int mgpcg_batch_create(mgpcg_batch_t** out) {
+  MG_REQUIRE(out != nullptr, "out is NULL");
+  mgpcg_batch* b = new (std::nothrow) mgpcg_batch();
+  MG_REQUIRE(b != nullptr, "out of host memory");
+  b->e = new_engine();
+  if (!b->e) {
+    delete b;
+    return -2;
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int mgpcg_batch_destroy(mgpcg_batch_t* b) {
+  if (!b) return 0;
+  delete b->e;
+  delete b;
+  return 0;
+}
+
+extern "C" int mgpcg_batch_set_stream(mgpcg_batch_t* b, void* stream) {
+  MG_REQUIRE(b != nullptr, "batch is NULL");
+  b->e->user_stream = (hipStream_t)stream;
+  return 0;
+}
+
+extern "C" int mgpcg_batch_solve(mgpcg_batch_t* b, mgpcg_solver_t* const* systems, int n,
+                                 int max_iters, double rel_tol, double abs_tol, double eps,
+                                 int refine_every, mgpcg_result_t* results) {
+  MG_REQUIRE(b != nullptr && systems != nullptr, "NULL argument");
+  MG_REQUIRE(n >= 1, "n must be >= 1");
+  for (int i = 0; i < n; ++i) MG_REQUIRE(systems[i] != nullptr, "systems[%d] is NULL", i);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j)
+      MG_REQUIRE(systems[i] != systems[j], "system %d appears twice in the batch", i);
+  Engine* e = b->e;
+  if (engine_begin(e)) return -2;
+  e->mem.clear();
+  e->dirty = true;
+  for (int i = 0; i < n; ++i) {
+    const SolveParams prm{max_iters, rel_tol, abs_tol, eps, refine_every};
+    const int slot = engine_add(e, systems[i], i, prm);
+    if (slot < 0) return slot == -1 ? -1 : -2;
+  }
+  std::vector<Finished> done;
+  const long cap = max_iters >= 0 ? (long)max_iters + 4 : 10000000L;
+  if (engine_run(e, cap, &done)) return -2;
+  if (results) {
+    for (const Finished& f : done) fill_result(f.st, &results[e->mem[f.member].tag]);
+    // systems the host cap stopped
+    for (const Member& M : e->mem)
+      if (M.active) {
+        CGState st;
+        MG_HIP(hipMemcpy(&st, M.s->st, sizeof(CGState), hipMemcpyDeviceToHost));
+        fill_result(st, &results[M.tag]);
+      }
+  }
+  e->mem.clear();
+  e->dirty = true;
+  MG_HIP(hipGetLastError());
+  return engine_end(e);
+}
+
+// The server: continuous batching.  Systems are submitted at any time; each
+// call of mgpcg_server_step admits the submitted ones (they join at the next
+// iteration boundary), enqueues one iteration for every member and retires
+// the ones whose state (read one iteration behind) says they are done.
+struct mgpcg_server {
+  Engine* e = nullptr;
+  int slot = 0;
+  bool inflight = false;
+  std::deque<std::pair<mgpcg_solver*, int64_t>> pending;
+  std::vector<std::pair<int64_t, mgpcg_result_t>> finished;
+  struct Params {
+    int max_iters = 500;
+    double rel_tol = 1e-12, abs_tol = 1e-14, eps = 1e-12;
+    int refine_every = 10;
+  } prm;
+};
+
+extern "C" int mgpcg_server_create(int max_iters, double rel_tol, double abs_tol, double eps,
+                                   int refine_every, mgpcg_server_t** out) {
+  MG_REQUIRE(out != nullptr, "out is NULL");
+  mgpcg_server* sv = new (std::nothrow) mgpcg_server();
+  MG_REQUIRE(sv != nullptr, "out of host memory");
+  sv->e = new_engine();
+  if (!sv->e) {
+    delete sv;
+    return -2;
+  }
+  sv->prm.max_iters = max_iters;
+  sv->prm.rel_tol = rel_tol;
+  sv->prm.abs_tol = abs_tol;
+  sv->prm.eps = eps;
+  sv->prm.refine_every = refine_every;
+  *out = sv;
+  return 0;
+}
+
+extern "C" int mgpcg_server_destroy(mgpcg_server_t* sv) {
+  if (!sv) return 0;
+  delete sv->e;
+  delete sv;
+  return 0;
+}
+
+extern "C" int mgpcg_server_set_stream(mgpcg_server_t* sv, void* stream) {
+  MG_REQUIRE(sv != nullptr, "server is NULL");
+  sv->e->user_stream = (hipStream_t)stream;
+  return 0;
+}
+
+extern "C" int mgpcg_server_submit(mgpcg_server_t* sv, mgpcg_solver_t* s, int64_t tag) {
+  MG_REQUIRE(sv != nullptr && s != nullptr, "NULL argument");
+  for (const Member& M : sv->e->mem)
+    MG_REQUIRE(!(M.active && M.s == s), "system is already being solved");
+  sv->pending.push_back({s, tag});
+  return 0;
+}
+
+extern "C" int mgpcg_server_step(mgpcg_server_t* sv, int* n_active) {
+  MG_REQUIRE(sv != nullptr, "server is NULL");
+  Engine* e = sv->e;
+  // newly submitted systems were assembled on the caller's stream: order the
+  // engine after it (only then, so assembly and iterations overlap otherwise)
+  if (!sv->pending.empty() && engine_begin(e)) return -2;
+  // admit submitted systems at this iteration boundary
+  while (!sv->pending.empty()) {
+    auto pr = sv->pending.front();
+    sv->pending.pop_front();
+    // free member slots of retired systems are reused
+    for (Member& M : e->mem)
+      if (!M.active) M.s = nullptr;
+    const SolveParams prm{sv->prm.max_iters, sv->prm.rel_tol, sv->prm.abs_tol, sv->prm.eps,
+                          sv->prm.refine_every};
+    if (engine_add(e, pr.first, pr.second, prm) < 0) return -2;
+  }
+  std::vector<Finished> done;
+  if (engine_active(e) > 0) {
+    if (engine_enqueue(e, sv->slot, false)) return -2;
+    if (sv->inflight && engine_collect(e, sv->slot ^ 1, &done)) return -2;
+    sv->inflight = true;
+    sv->slot ^= 1;
+  } else if (sv->inflight) {
+    if (engine_collect(e, sv->slot ^ 1, &done)) return -2;
+    sv->inflight = false;
+  }
+  for (const Finished& f : done) {
+    mgpcg_result_t r;
+    fill_result(f.st, &r);
+    sv->finished.push_back({e->mem[f.member].tag, r});
+  }
+  if (n_active) *n_active = engine_active(e);
+  // no ordering of the caller's stream after the engine: results are released
+  // only after the host waited for the readback recorded behind their last
+  // kernels, so the caller may read x at once
+  MG_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int mgpcg_server_poll(mgpcg_server_t* sv, int64_t* tags, mgpcg_result_t* results,
+                                 int cap, int* n_out) {
+  MG_REQUIRE(sv != nullptr && n_out != nullptr, "NULL argument");
+  int n = std::min(cap, (int)sv->finished.size());
+  for (int i = 0; i < n; ++i) {
+    if (tags) tags[i] = sv->finished[i].first;
+    if (results) results[i] = sv->finished[i].second;
+  }
+  sv->finished.erase(sv->finished.begin(), sv->finished.begin() + n);
+  *n_out = n;
+  return 0;
+}
+
+extern "C" int mgpcg_server_pending(const mgpcg_server_t* sv, int* n_pending, int* n_active,
+                                    int* inflight) {
+  MG_REQUIRE(sv != nullptr, "server is NULL");
+  if (n_pending) *n_pending = (int)sv->pending.size();
+  if (n_active) *n_active = engine_active(sv->e);
+  if (inflight) *inflight = sv->inflight ? 1 : 0;
+  return 0;
 }
 
 extern "C" int mgpcg_coarsen(void* stream, int dim, const int32_t* fine_res,

@@ -204,3 +204,10 @@ class MGPCGPoissonSolver:
     def debug_op(self, op, level=0, arg=0):
         _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
         _lib.check(self._lib.mgpcg_debug_op(self._h, op, level, arg))
+
+    def time_op(self, op, level=0, arg=0, reps=50):
+        """Average milliseconds of one launch of a stage (HIP events on the handle's stream)."""
+        ms = ctypes.c_double()
+        _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
+        _lib.check(self._lib.mgpcg_time_op(self._h, op, level, arg, reps, ctypes.byref(ms)))
+        return ms.value

@@ -83,7 +83,9 @@ def test_smooth_bit_exact(case, phase):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_smooth_from_zero(case):
+def test_smooth_pair_from_zero(case):
+    """z.fill(0); smooth(l, 0); smooth(l, 1) in one kernel (colour 0 recomputed
+    on the fly inside the colour-1 stencil)."""
     import _lib
 
     res, L = case
@@ -95,11 +97,39 @@ def test_smooth_from_zero(case):
         r = rng.standard_normal(shape).astype(np.float32)
         s.debug_set(_lib.FIELD_R, l, torch.from_numpy(r))
         s.debug_set(_lib.FIELD_Z, l, torch.from_numpy(rng.standard_normal(shape).astype(np.float32)))
-        s.debug_op(_lib.OP_SMOOTH_FROM_ZERO, l, 0)
+        s.debug_op(_lib.OP_SMOOTH_PAIR_FROM_ZERO, l, 0)
         got = _get(s, _lib.FIELD_Z, l)
         z = np.zeros(shape, np.float32)
         core.smooth(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z, 0)
+        core.smooth(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z, 1)
         assert np.array_equal(got, z), f"level {l}"  # == treats -0 and +0 alike
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_prolong_smooth1_bit_exact(case):
+    """prolongate(l); smooth(l, 1) fused: every colour-1 FLUID value matches."""
+    import _lib
+
+    res, L = case
+    h = random_hierarchy(res, L, seed=16)
+    s = _solver(h)
+    rng = np.random.default_rng(17)
+    for l in range(L - 1):
+        shape = h["gt"][l].shape
+        r = rng.standard_normal(shape).astype(np.float32)
+        z = rng.standard_normal(shape).astype(np.float32)
+        zc = rng.standard_normal(h["gt"][l + 1].shape).astype(np.float32)
+        s.debug_set(_lib.FIELD_R, l, torch.from_numpy(r))
+        s.debug_set(_lib.FIELD_Z, l, torch.from_numpy(z))
+        s.debug_set(_lib.FIELD_Z, l + 1, torch.from_numpy(zc))
+        s.debug_op(_lib.OP_PROLONG_SMOOTH1, l, 0)
+        got = _get(s, _lib.FIELD_Z, l)
+        zr = z.copy()
+        core.prolongate(zr, zc)
+        core.smooth(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, zr, 1)
+        colour = np.indices(shape).sum(axis=0) & 1
+        upd = (h["gt"][l] == 0) & (colour == 1)
+        assert np.array_equal(got[upd], zr[upd]), f"level {l}"
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
@@ -144,23 +174,32 @@ def test_prolongate_bit_exact(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_compute_Ap_bit_exact(case):
+def test_update_p_compute_Ap_bit_exact(case):
+    """p_next = z + 0 * p (update_p with beta = 0), Ap = A p_next."""
     import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=10)
     s = _solver(h)
-    p = np.random.default_rng(11).standard_normal(res).astype(np.float32)
+    rng = np.random.default_rng(11)
+    z = rng.standard_normal(res).astype(np.float32)
+    p = rng.standard_normal(res).astype(np.float32)
+    s.debug_set(_lib.FIELD_Z, 0, torch.from_numpy(z))
     s.debug_set(_lib.FIELD_P, 0, torch.from_numpy(p))
     s.debug_set(_lib.FIELD_AP, 0, torch.zeros(res))
     s.debug_op(_lib.OP_COMPUTE_AP, 0, 0)
+    fl = h["gt"][0] == 0
+    pn = _get(s, _lib.FIELD_P_NEXT, 0)
+    assert np.array_equal(pn[fl], z[fl])
+    pref = np.where(fl, z + np.float32(0.0) * p, p).astype(np.float32)
+    ref = core.compute_Ap(h["gt"][0], h["Adiag"][0], h["Ax"][0], pref)
     got = _get(s, _lib.FIELD_AP, 0)
-    ref = core.compute_Ap(h["gt"][0], h["Adiag"][0], h["Ax"][0], p)
-    assert np.array_equal(got, ref)
+    assert np.array_equal(got[fl], ref[fl])
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 def test_vcycle_bit_exact(case):
+    """Every FLUID value of z = M r; non-FLUID z is inert (unprolonged)."""
     import _lib
 
     res, L = case
@@ -171,7 +210,32 @@ def test_vcycle_bit_exact(case):
     s.debug_op(_lib.OP_VCYCLE, 0, 0)
     got = _get(s, _lib.FIELD_Z, 0)
     ref, _, _ = core.v_cycle(h, r0, nu=2, bottom=10)
-    assert np.array_equal(got, ref)
+    fl = h["gt"][0] == 0
+    assert np.array_equal(got[fl], ref[fl])
+
+
+@pytest.mark.parametrize("case", CASES[1:3], ids=_case_id)
+@pytest.mark.parametrize("coarse_max", [0, 1 << 30])
+def test_vcycle_schedule_variants(case, coarse_max):
+    """All levels on batched kernels (coarse_max 0) or all in the
+    one-workgroup coarse kernel: the same FLUID values."""
+    import _lib
+
+    res, L = case
+    lib = _lib.load()
+    _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, coarse_max))
+    try:
+        h = random_hierarchy(res, L, seed=18)
+        s = _solver(h)
+    finally:
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, 4096))
+    r0 = np.random.default_rng(19).standard_normal(res).astype(np.float32)
+    s.debug_set(_lib.FIELD_R, 0, torch.from_numpy(r0))
+    s.debug_op(_lib.OP_VCYCLE, 0, 0)
+    got = _get(s, _lib.FIELD_Z, 0)
+    ref, _, _ = core.v_cycle(h, r0, nu=2, bottom=10)
+    fl = h["gt"][0] == 0
+    assert np.array_equal(got[fl], ref[fl])
 
 
 @pytest.mark.parametrize("case", CASES[:2], ids=_case_id)
@@ -186,4 +250,5 @@ def test_vcycle_f64_matches_oracle(case):
     s.debug_op(_lib.OP_VCYCLE, 0, 0)
     got = _get(s, _lib.FIELD_Z, 0)
     ref, _, _ = core.v_cycle(h, r0, nu=2, bottom=10)
-    assert np.array_equal(got, ref)
+    fl = h["gt"][0] == 0
+    assert np.array_equal(got[fl], ref[fl])

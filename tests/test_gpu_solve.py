@@ -66,16 +66,91 @@ def test_hostsync_loop_equals_device_loop():
     assert np.array_equal(xa, xb)
 
 
-@pytest.mark.parametrize("use_graph,burst", [(0, 1), (1, 1), (1, 4), (1, 16)])
-def test_launch_modes_agree(use_graph, burst):
+@pytest.mark.parametrize("coarse_max,max_grid", [(0, 2048), (1 << 30, 2048), (4096, 8),
+                                                 (4096, 64)])
+def test_schedule_knobs_agree(coarse_max, max_grid):
+    """Where the coarse levels run and how many workgroups stride over the
+    virtual blocks must not change a single bit of x."""
     import _lib
 
     sysd = scenes.c2_3d(seed=1, n=64, L=4)
     xa, outa, _ = gpu_solve(sysd, "plain")
-    xb, outb, _ = gpu_solve(sysd, "plain",
-                            opts={_lib.OPT_USE_GRAPH: use_graph, _lib.OPT_BURST: burst})
+    lib = _lib.load()
+    try:
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, coarse_max))
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_MAX_GRID, max_grid))
+        xb, outb, _ = gpu_solve(sysd, "plain")
+    finally:
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, 4096))
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_MAX_GRID, 2048))
     assert outa["iters"] == outb["iters"]
     assert np.array_equal(xa, xb)
+
+
+def _assembled(sysd, precision):
+    import mgpcg
+    import systems
+
+    dev = systems.upload(sysd, "cuda:0")
+    s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], sysd["nu"], sysd["bottom"],
+                                 precision=precision, device="cuda:0")
+    systems.assemble(s, dev)
+    return s
+
+
+def _x(s):
+    return (s.x64 if s.x64 is not None else s.x).cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_batch_uneven_equals_alone(precision):
+    """A batch mixing the four system kinds at uneven sizes: every result is
+    bit-identical to the same system solved alone, and matches the oracle."""
+    import batch
+
+    mix = scenes.c4_batch(first_seed=100, count=8, L=4)
+    for sysd in mix:  # keep the test fast: halve the sides (still divisible by 8)
+        sysd.update(scenes.pool_sphere_system([n // 2 for n in sysd["res"]], sysd["seed"], L=4,
+                                              kind=sysd["kind"]))
+    solvers = [_assembled(sysd, precision) for sysd in mix]
+    res = batch.solve_batch(solvers, max_iters=500, rel_tol=1e-12)
+    xs = [_x(s) for s in solvers]
+    for sysd, xb, rb in zip(mix, xs, res):
+        print(f"case {sysd['name']} {precision}", flush=True)
+        xa, ra, _ = gpu_solve(sysd, precision)
+        assert ra["iters"] == rb["iters"] and ra["status"] == rb["status"]
+        assert np.array_equal(xa, xb)
+        xr, rr, _ = osys.solve(sysd, PREC[precision], max_iters=500, rel_tol=1e-12)
+        assert abs(rb["iters"] - rr["iters"]) <= 1 and rb["status"] == rr["status"]
+        assert rel_l2(xb, xr) <= 1e-5
+
+
+def test_server_continuous_batching_equals_alone():
+    """Requests joining at different iterations (continuous batching) give the
+    same bits as solving alone; slots and hierarchies are reused."""
+    import serve
+    import systems
+
+    reqs = [scenes.c5_request(1000 + i, lo=32, hi=64, L=3) for i in range(6)]
+    srv = serve.Server(hbm_budget_bytes=4 << 30, precision="mixed", n_levels=3,
+                       max_active=3, device="cuda:0")
+    devs = [systems.upload(r, "cuda:0") for r in reqs]
+    done = []
+    i = 0
+    while len(done) < len(reqs):
+        if i < len(reqs):  # one new request per step: staggered admission
+            srv.submit(devs[i], meta=i)
+            i += 1
+        for rq in srv.step():
+            rq["x"] = _x(rq["solver"])
+            srv.retire(rq)
+            done.append(rq)
+    assert srv.max_members >= 2  # they really overlapped
+    for rq in done:
+        sysd = reqs[rq["meta"]]
+        xa, ra, _ = gpu_solve(sysd, "mixed")
+        assert ra["iters"] == rq["result"]["iters"]
+        assert np.array_equal(xa, rq["x"])
 
 
 def test_mixed_error_vs_f64():
