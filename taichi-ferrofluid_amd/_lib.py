@@ -91,15 +91,27 @@ class MgpcgError(RuntimeError):
     pass
 
 
+_path = LIB_PATH
+
+
+def use_library(path):
+    """Select another build of the same C-ABI before the first load (the
+    tests' CPU emulation build); the product always uses libmgpcg.so."""
+    global _path
+    if _lib is not None and path != _path:
+        raise MgpcgError("the library is already loaded")
+    _path = path
+
+
 def load():
     """Load libmgpcg.so (raises if it is missing: there is no CPU fallback)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if not os.path.exists(_path):
             raise MgpcgError(
-                f"{LIB_PATH} is missing; build it with `python taichi-ferrofluid_amd/build.py` "
+                f"{_path} is missing; build it with `python taichi-ferrofluid_amd/build.py` "
                 "(or __graft_entry__.build())")
-        lib = ctypes.CDLL(LIB_PATH)
+        lib = ctypes.CDLL(_path)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
             fn.restype = res
@@ -132,7 +144,16 @@ def int3(vals):
 
 
 def stream_handle(device=None):
+    """torch's current HIP stream of `device` (NULL for CPU tensors, which only
+    the CPU emulation build accepts)."""
+    if device is not None and torch.device(device).type == "cpu":
+        return ctypes.c_void_p(None)
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def synchronize(device):
+    if torch.device(device).type != "cpu":
+        torch.cuda.synchronize(device)
 
 
 def hip_runtimes_mapped():

@@ -345,8 +345,9 @@ int engine_configure(Engine* e, const mgpcg_solver* s) {
   e->L = s->L;
   e->nu = s->nu;
   e->bottom = s->bottom;
-  e->level_sz = align_up(level_desc_size(e->precision), 16);
-  e->sys_sz = align_up(sys_desc_size(e->precision), 16);
+  // table strides are exactly the device-side array strides (descs[i])
+  e->level_sz = level_desc_size(e->precision);
+  e->sys_sz = sys_desc_size(e->precision);
   e->st_slot.assign(e->L, {});
   e->st_restrict.assign(e->L, {});
   e->st_bottom.assign(1, {});

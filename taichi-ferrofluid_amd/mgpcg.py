@@ -191,14 +191,14 @@ class MGPCGPoissonSolver:
         t = t.to(device=self.device, dtype=self._vec_dtype(field)).contiguous()
         _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
         _lib.check(self._lib.mgpcg_debug_set(self._h, field, level, _lib.ptr(t)))
-        torch.cuda.synchronize(self.device)
+        _lib.synchronize(self.device)
 
     def debug_get(self, field, level):
         out = torch.zeros(level_shape(self.res, level), dtype=self._vec_dtype(field),
                           device=self.device)
         _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
         _lib.check(self._lib.mgpcg_debug_get(self._h, field, level, _lib.ptr(out)))
-        torch.cuda.synchronize(self.device)
+        _lib.synchronize(self.device)
         return out
 
     def debug_op(self, op, level=0, arg=0):

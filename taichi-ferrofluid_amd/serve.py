@@ -191,7 +191,7 @@ def closed_loop(server, make_scene, total, concurrency, keep_x=False):
         for req in server.step():
             server.retire(req, keep_x=keep_x)
             records.append(req)
-    torch.cuda.synchronize(server.device)
+    _lib.synchronize(server.device)
     return records, time.perf_counter() - t0
 
 
@@ -216,7 +216,7 @@ def poisson_arrivals(server, scenes_dev, rate, total, seed=0, keep_x=False):
                 records.append(req)
         elif issued < total:
             time.sleep(max(0.0, min(arrive[issued] - now, 0.001)))
-    torch.cuda.synchronize(server.device)
+    _lib.synchronize(server.device)
     return records, time.perf_counter() - t0
 
 

@@ -10,8 +10,34 @@ for p in (PKG, ROOT):
         sys.path.insert(0, p)
 
 
+EMU_LIB = os.path.join(ROOT, "tests", "cpu_emu", "libmgpcg_emu.so")
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libmgpcg.so)")
+
+
+@pytest.fixture(scope="session", params=[pytest.param("cuda:0", marks=pytest.mark.gpu), "emu"])
+def dev(request):
+    """Device of a parity test: the MI355X through libmgpcg.so ("cuda:0", marked
+    gpu), or the CPU emulation build of the same HIP sources ("emu" -> torch
+    CPU tensors), which runs every kernel and the engine bounds-checked on the
+    host.  One process loads one of the two libraries."""
+    import _lib
+
+    if request.param == "emu":
+        if _lib._lib is not None and _lib._path != EMU_LIB:
+            pytest.skip("libmgpcg.so already loaded in this process")
+        if not os.path.exists(EMU_LIB):
+            sys.path.insert(0, os.path.join(ROOT, "tests", "cpu_emu"))
+            import build_emu
+
+            build_emu.build()
+        _lib.use_library(EMU_LIB)
+        return "cpu"
+    if _lib._lib is not None and _lib._path == EMU_LIB:
+        pytest.skip("the CPU emulation build is loaded in this process")
+    return "cuda:0"
 
 
 def gpu_available():

@@ -1,0 +1,142 @@
+// CPU emulation of the HIP subset libmgpcg uses -- TEST INFRASTRUCTURE ONLY.
+//
+// tests/cpu_emu/build_emu.py compiles the library's own HIP sources as host
+// C++ with this directory first on the include path, producing
+// tests/cpu_emu/libmgpcg_emu.so.  Kernels then run on the CPU: every GPU
+// thread is a coroutine, __syncthreads and wave shuffles are emulated, and
+// every "device" allocation is surrounded by guard pages, so an out-of-bounds
+// access faults (or is reported by AddressSanitizer in the driver build) on
+// the CPU, where it can be debugged.  The product never loads this build.
+#pragma once
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <cmath>
+#include <tuple>
+#include <utility>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+typedef enum hipError_t {
+  hipSuccess = 0,
+  hipErrorInvalidValue = 1,
+  hipErrorOutOfMemory = 2,
+  hipErrorLaunchFailure = 719,
+} hipError_t;
+
+typedef enum hipMemcpyKind {
+  hipMemcpyHostToHost = 0,
+  hipMemcpyHostToDevice = 1,
+  hipMemcpyDeviceToHost = 2,
+  hipMemcpyDeviceToDevice = 3,
+  hipMemcpyDefault = 4,
+} hipMemcpyKind;
+
+struct ihipStream_t;
+struct ihipEvent_t;
+typedef ihipStream_t* hipStream_t;
+typedef ihipEvent_t* hipEvent_t;
+
+#define hipStreamNonBlocking 1
+#define hipEventDisableTiming 2
+
+struct dim3 {
+  unsigned x, y, z;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+namespace hipemu {
+extern dim3 threadIdx, blockIdx, gridDim, blockDim;
+void syncthreads();
+double shfl_xor_d(double v, int mask);
+int shfl_xor_i(int v, int mask);
+float shfl_xor_f(float v, int mask);
+// run `body(thread)` for every thread of every block of the grid
+void run_grid(dim3 grid, dim3 block, void (*tramp)(void*), void* ctx);
+void* guarded_alloc(size_t bytes);
+void guarded_free(void* p);
+}  // namespace hipemu
+
+#define threadIdx (::hipemu::threadIdx)
+#define blockIdx (::hipemu::blockIdx)
+#define gridDim (::hipemu::gridDim)
+#define blockDim (::hipemu::blockDim)
+
+inline void __syncthreads() { hipemu::syncthreads(); }
+inline double __shfl_xor(double v, int mask, int width = 64) {
+  (void)width;
+  return hipemu::shfl_xor_d(v, mask);
+}
+inline float __shfl_xor(float v, int mask, int width = 64) {
+  (void)width;
+  return hipemu::shfl_xor_f(v, mask);
+}
+inline int __shfl_xor(int v, int mask, int width = 64) {
+  (void)width;
+  return hipemu::shfl_xor_i(v, mask);
+}
+inline void __threadfence() {}
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline unsigned __umulhi(unsigned a, unsigned b) {
+  return (unsigned)(((unsigned long long)a * b) >> 32);
+}
+
+// runtime API
+const char* hipGetErrorString(hipError_t e);
+hipError_t hipGetLastError(void);
+hipError_t hipGetDevice(int* d);
+hipError_t hipSetDevice(int d);
+hipError_t hipDeviceSynchronize(void);
+hipError_t hipMalloc(void** p, size_t bytes);
+template <typename T>
+inline hipError_t hipMalloc(T** p, size_t bytes) {
+  return hipMalloc((void**)p, bytes);
+}
+hipError_t hipFree(void* p);
+hipError_t hipMemset(void* p, int v, size_t bytes);
+hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind k);
+hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind k,
+                          hipStream_t s);
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned flags);
+hipError_t hipHostFree(void* p);
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned flags);
+hipError_t hipStreamDestroy(hipStream_t s);
+hipError_t hipStreamSynchronize(hipStream_t s);
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
+hipError_t hipEventCreate(hipEvent_t* e);
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
+hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
+hipError_t hipEventDestroy(hipEvent_t e);
+
+// kernel launch: synchronous, every block's threads as coroutines
+namespace hipemu {
+template <typename F, typename Tup>
+struct LaunchCtx {
+  F f;
+  Tup args;
+};
+template <typename F, typename Tup>
+void tramp(void* p) {
+  auto* c = (LaunchCtx<F, Tup>*)p;
+  std::apply(c->f, c->args);
+}
+template <typename... KArgs, typename... Args>
+void launch(void (*kernel)(KArgs...), dim3 grid, dim3 block, Args&&... args) {
+  typedef std::tuple<typename std::decay<KArgs>::type...> Tup;
+  LaunchCtx<void (*)(KArgs...), Tup> ctx{kernel, Tup(std::forward<Args>(args)...)};
+  run_grid(grid, block, &tramp<void (*)(KArgs...), Tup>, &ctx);
+}
+}  // namespace hipemu
+
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
+  ::hipemu::launch(kernel, dim3(grid), dim3(block), ##__VA_ARGS__)

@@ -1,9 +1,11 @@
-"""Kernel-level parity of the HIP library against the oracle (GPU).
+"""Kernel-level parity of the HIP library against the oracle.
 
 Each stage of the V-cycle and the CG operator runs once through the C-ABI
 (mgpcg_debug_set / _op / _get) on seeded inputs and is compared with the
 oracle's restatement of the same reference kernel.  FP32 results must be bit
 identical: both sides use the same operation order without contraction.
+Every test runs on the MI355X (`dev` = "cuda:0", marked gpu) and on the CPU
+emulation build of the same sources (`dev` = "cpu").
 """
 import numpy as np
 import pytest
@@ -11,8 +13,6 @@ import torch
 
 from helpers import random_hierarchy
 from oracle import core
-
-pytestmark = pytest.mark.gpu
 
 CASES = [
     ((32, 24), 3),       # 2D
@@ -22,13 +22,13 @@ CASES = [
 ]
 
 
-def _solver(h, precision="plain"):
+def _solver(h, dev, precision="plain"):
     import mgpcg
 
     res = list(h["gt"][0].shape)
     dim = len(res)
     L = len(h["gt"])
-    s = mgpcg.MGPCGPoissonSolver(dim, res, L, 2, 10, precision=precision, device="cuda:0")
+    s = mgpcg.MGPCGPoissonSolver(dim, res, L, 2, 10, precision=precision, device=dev)
     for l in range(L):
         s.grid_type[l].copy_(torch.from_numpy(h["gt"][l]))
         s.Adiag[l].copy_(torch.from_numpy(h["Adiag"][l].astype(np.float32)))
@@ -46,29 +46,29 @@ def _case_id(c):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_coarsen_bit_exact(case):
+def test_coarsen_bit_exact(dev, case):
     import _lib
 
     res, _ = case
     rng = np.random.default_rng(1)
     fine = rng.integers(0, 3, res).astype(np.int8)
-    t = torch.from_numpy(fine).cuda()
-    out = torch.empty([n // 2 for n in res], dtype=torch.int8, device="cuda")
+    t = torch.from_numpy(fine).to(dev)
+    out = torch.empty([n // 2 for n in res], dtype=torch.int8, device=dev)
     lib = _lib.load()
-    _lib.check(lib.mgpcg_coarsen(_lib.stream_handle(), len(res), _lib.int3(res), _lib.ptr(t),
+    _lib.check(lib.mgpcg_coarsen(_lib.stream_handle(dev), len(res), _lib.int3(res), _lib.ptr(t),
                                  _lib.ptr(out)))
-    torch.cuda.synchronize()
+    _lib.synchronize(dev)
     assert np.array_equal(out.cpu().numpy(), core.coarsen(fine))
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 @pytest.mark.parametrize("phase", [0, 1])
-def test_smooth_bit_exact(case, phase):
+def test_smooth_bit_exact(dev, case, phase):
     import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=2)
-    s = _solver(h)
+    s = _solver(h, dev)
     rng = np.random.default_rng(3)
     for l in range(L):
         shape = h["gt"][l].shape
@@ -83,14 +83,14 @@ def test_smooth_bit_exact(case, phase):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_smooth_pair_from_zero(case):
+def test_smooth_pair_from_zero(dev, case):
     """z.fill(0); smooth(l, 0); smooth(l, 1) in one kernel (colour 0 recomputed
     on the fly inside the colour-1 stencil)."""
     import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=4)
-    s = _solver(h)
+    s = _solver(h, dev)
     rng = np.random.default_rng(5)
     for l in range(L):
         shape = h["gt"][l].shape
@@ -106,13 +106,13 @@ def test_smooth_pair_from_zero(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_prolong_smooth1_bit_exact(case):
+def test_prolong_smooth1_bit_exact(dev, case):
     """prolongate(l); smooth(l, 1) fused: every colour-1 FLUID value matches."""
     import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=16)
-    s = _solver(h)
+    s = _solver(h, dev)
     rng = np.random.default_rng(17)
     for l in range(L - 1):
         shape = h["gt"][l].shape
@@ -133,12 +133,12 @@ def test_prolong_smooth1_bit_exact(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_restrict_bit_exact(case):
+def test_restrict_bit_exact(dev, case):
     import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=6)
-    s = _solver(h)
+    s = _solver(h, dev)
     rng = np.random.default_rng(7)
     for l in range(L - 1):
         shape = h["gt"][l].shape
@@ -153,12 +153,12 @@ def test_restrict_bit_exact(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_prolongate_bit_exact(case):
+def test_prolongate_bit_exact(dev, case):
     import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=8)
-    s = _solver(h)
+    s = _solver(h, dev)
     rng = np.random.default_rng(9)
     for l in range(L - 1):
         shape = h["gt"][l].shape
@@ -174,13 +174,13 @@ def test_prolongate_bit_exact(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_update_p_compute_Ap_bit_exact(case):
+def test_update_p_compute_Ap_bit_exact(dev, case):
     """p_next = z + 0 * p (update_p with beta = 0), Ap = A p_next."""
     import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=10)
-    s = _solver(h)
+    s = _solver(h, dev)
     rng = np.random.default_rng(11)
     z = rng.standard_normal(res).astype(np.float32)
     p = rng.standard_normal(res).astype(np.float32)
@@ -198,13 +198,13 @@ def test_update_p_compute_Ap_bit_exact(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_vcycle_bit_exact(case):
+def test_vcycle_bit_exact(dev, case):
     """Every FLUID value of z = M r; non-FLUID z is inert (unprolonged)."""
     import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=12)
-    s = _solver(h)
+    s = _solver(h, dev)
     r0 = np.random.default_rng(13).standard_normal(res).astype(np.float32)
     s.debug_set(_lib.FIELD_R, 0, torch.from_numpy(r0))
     s.debug_op(_lib.OP_VCYCLE, 0, 0)
@@ -216,7 +216,7 @@ def test_vcycle_bit_exact(case):
 
 @pytest.mark.parametrize("case", CASES[1:3], ids=_case_id)
 @pytest.mark.parametrize("coarse_max", [0, 1 << 30])
-def test_vcycle_schedule_variants(case, coarse_max):
+def test_vcycle_schedule_variants(dev, case, coarse_max):
     """All levels on batched kernels (coarse_max 0) or all in the
     one-workgroup coarse kernel: the same FLUID values."""
     import _lib
@@ -226,7 +226,7 @@ def test_vcycle_schedule_variants(case, coarse_max):
     _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, coarse_max))
     try:
         h = random_hierarchy(res, L, seed=18)
-        s = _solver(h)
+        s = _solver(h, dev)
     finally:
         _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, 4096))
     r0 = np.random.default_rng(19).standard_normal(res).astype(np.float32)
@@ -239,12 +239,12 @@ def test_vcycle_schedule_variants(case, coarse_max):
 
 
 @pytest.mark.parametrize("case", CASES[:2], ids=_case_id)
-def test_vcycle_f64_matches_oracle(case):
+def test_vcycle_f64_matches_oracle(dev, case):
     import _lib
 
     res, L = case
     h = core.to_f64(random_hierarchy(res, L, seed=14))
-    s = _solver(h, precision="f64")
+    s = _solver(h, dev, precision="f64")
     r0 = np.random.default_rng(15).standard_normal(res)
     s.debug_set(_lib.FIELD_R, 0, torch.from_numpy(r0))
     s.debug_op(_lib.OP_VCYCLE, 0, 0)

@@ -1,0 +1,190 @@
+"""Whole-solve parity of the HIP solver against the oracle.
+
+Bar: pressures within 1e-5 relative L2 of the oracle at 1e-6 relative
+residual (rel_tol 1e-12 on squared norms, mgpcg.py:164), iteration count
+within +-1, same status; MIXED additionally within 1e-5 of an FP64 solve
+converged to 1e-12 relative residual.  Batched and served solves must be
+bit-identical to the same system solved alone.
+
+`dev` = "cuda:0" runs on the MI355X (marked gpu) at the workload sizes;
+`dev` = "cpu" runs the CPU emulation build of the same HIP sources at
+reduced sizes.
+"""
+import numpy as np
+import pytest
+
+import scenes
+from helpers import rel_l2
+from oracle import core
+from oracle import systems as osys
+
+PREC = {"plain": core.PLAIN, "mixed": core.MIXED, "f64": core.F64}
+
+
+def gpu_solve(sysd, dev, precision="plain", hostsync=False, max_iters=500, rel_tol=1e-12):
+    import _lib
+    import mgpcg
+    import systems
+
+    d = systems.upload(sysd, dev)
+    s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], sysd["nu"], sysd["bottom"],
+                                 precision=precision, device=dev)
+    systems.assemble(s, d)
+    if hostsync:
+        s.solve_hostsync(max_iters, rel_tol=rel_tol)
+    else:
+        s.solve(max_iters, rel_tol=rel_tol)
+    _lib.synchronize(dev)
+    return _x(s), dict(s.last), s
+
+
+def _x(s):
+    return (s.x64 if s.x64 is not None else s.x).cpu().numpy().astype(np.float64)
+
+
+def _emu(dev):
+    return dev == "cpu"
+
+
+SYSTEMS = {
+    "c1_2d": lambda emu: scenes.c1_2d(seed=0, n=64 if emu else 128),
+    "c2_3d_small": lambda emu: scenes.c2_3d(seed=1, n=32 if emu else 64, L=3 if emu else 4),
+    "c2_3d_128": lambda emu: None if emu else scenes.c2_3d(seed=1, n=128, L=5),
+}
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_solve_matches_oracle(dev, name, precision):
+    sysd = SYSTEMS[name](_emu(dev))
+    if sysd is None:
+        pytest.skip("workload size runs on the GPU only")
+    print(f"case {sysd['name']} {precision} on {dev}", flush=True)
+    x, out, _ = gpu_solve(sysd, dev, precision)
+    xr, outr, _ = osys.solve(sysd, PREC[precision], max_iters=500, rel_tol=1e-12)
+    assert out["status"] == 1 == outr["status"], (out, outr)
+    assert abs(out["iters"] - outr["iters"]) <= 1, (out["iters"], outr["iters"])
+    assert rel_l2(x, xr) <= 1e-5
+    assert out["rr"] < out["tol"]
+
+
+def test_hostsync_loop_equals_device_loop(dev):
+    sysd = SYSTEMS["c2_3d_small"](_emu(dev))
+    xa, outa, _ = gpu_solve(sysd, dev, "plain")
+    xb, outb, _ = gpu_solve(sysd, dev, "plain", hostsync=True)
+    assert outa["iters"] == outb["iters"]
+    assert np.array_equal(xa, xb)
+
+
+@pytest.mark.parametrize("coarse_max,max_grid", [(0, 2048), (1 << 30, 2048), (4096, 8),
+                                                 (4096, 64)])
+def test_schedule_knobs_agree(dev, coarse_max, max_grid):
+    """Where the coarse levels run and how many workgroups stride over the
+    virtual blocks must not change a single bit of x."""
+    import _lib
+
+    sysd = SYSTEMS["c2_3d_small"](_emu(dev))
+    xa, outa, _ = gpu_solve(sysd, dev, "plain")
+    lib = _lib.load()
+    try:
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, coarse_max))
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_MAX_GRID, max_grid))
+        xb, outb, _ = gpu_solve(sysd, dev, "plain")
+    finally:
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, 4096))
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_MAX_GRID, 2048))
+    assert outa["iters"] == outb["iters"]
+    assert np.array_equal(xa, xb)
+
+
+def test_mixed_error_vs_f64(dev):
+    sysd = SYSTEMS["c2_3d_small"](_emu(dev))
+    x, out, _ = gpu_solve(sysd, dev, "mixed")
+    x64, out64, _ = gpu_solve(sysd, dev, "f64", rel_tol=1e-24)
+    xo, outo, _ = osys.solve(sysd, core.F64, max_iters=500, rel_tol=1e-24)
+    assert out64["status"] == 1
+    assert rel_l2(x64, xo) < 1e-10
+    assert out["true_rr"] < out["tol"]
+    assert rel_l2(x, x64) <= 1e-5
+
+
+def test_max_iters_and_zero_rhs(dev):
+    sysd = scenes.c2_3d(seed=1, n=32, L=3)
+    x, out, _ = gpu_solve(sysd, dev, "plain", max_iters=1)
+    xr, outr, _ = osys.solve(sysd, core.PLAIN, max_iters=1)
+    assert out["status"] == 2 and out["iters"] == 1 == outr["iters"]
+    assert rel_l2(x, xr) <= 1e-6
+    x0, out0, _ = gpu_solve(sysd, dev, "plain", max_iters=0)
+    assert out0["iters"] == 0 and out0["status"] == 2 and not np.any(x0)
+    z = dict(sysd)
+    z["velocity"] = [np.zeros_like(v) for v in sysd["velocity"]]
+    z["p0"] = 0.0
+    xz, outz, _ = gpu_solve(z, dev, "plain")
+    assert outz["status"] == 1 and outz["iters"] == 0 and not np.any(xz)
+
+
+def _assembled(sysd, dev, precision):
+    import mgpcg
+    import systems
+
+    d = systems.upload(sysd, dev)
+    s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], sysd["nu"], sysd["bottom"],
+                                 precision=precision, device=dev)
+    systems.assemble(s, d)
+    return s
+
+
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_batch_uneven_equals_alone(dev, precision):
+    """A batch mixing the four system kinds at uneven sizes: every result is
+    bit-identical to the same system solved alone, and matches the oracle."""
+    import batch
+
+    emu = _emu(dev)
+    count, div, L = (4, 4, 3) if emu else (8, 2, 4)
+    mix = scenes.c4_batch(first_seed=100, count=count, L=L)
+    for sysd in mix:  # smaller sides (still divisible by 2^(L-1))
+        sysd.update(scenes.pool_sphere_system([n // div for n in sysd["res"]], sysd["seed"], L=L,
+                                              kind=sysd["kind"]))
+    solvers = [_assembled(sysd, dev, precision) for sysd in mix]
+    res = batch.solve_batch(solvers, max_iters=500, rel_tol=1e-12)
+    xs = [_x(s) for s in solvers]
+    for sysd, xb, rb in zip(mix, xs, res):
+        print(f"case {sysd['name']} {precision}", flush=True)
+        xa, ra, _ = gpu_solve(sysd, dev, precision)
+        assert ra["iters"] == rb["iters"] and ra["status"] == rb["status"]
+        assert np.array_equal(xa, xb)
+        xr, rr, _ = osys.solve(sysd, PREC[precision], max_iters=500, rel_tol=1e-12)
+        assert abs(rb["iters"] - rr["iters"]) <= 1 and rb["status"] == rr["status"]
+        assert rel_l2(xb, xr) <= 1e-5
+
+
+def test_server_continuous_batching_equals_alone(dev):
+    """Requests joining at different iterations (continuous batching) give the
+    same bits as solving alone; slots and hierarchies are reused."""
+    import serve
+    import systems
+
+    emu = _emu(dev)
+    nreq = 4 if emu else 6
+    lo, hi = (16, 32) if emu else (32, 64)
+    reqs = [scenes.c5_request(1000 + i, lo=lo, hi=hi, L=3) for i in range(nreq)]
+    srv = serve.Server(hbm_budget_bytes=4 << 30, precision="mixed", n_levels=3,
+                       max_active=3, device=dev)
+    devs = [systems.upload(r, dev) for r in reqs]
+    done = []
+    i = 0
+    while len(done) < len(reqs):
+        if i < len(reqs):  # one new request per step: staggered admission
+            srv.submit(devs[i], meta=i)
+            i += 1
+        for rq in srv.step():
+            rq["x"] = _x(rq["solver"])
+            srv.retire(rq)
+            done.append(rq)
+    assert srv.max_members >= 2  # they really overlapped
+    for rq in done:
+        sysd = reqs[rq["meta"]]
+        xa, ra, _ = gpu_solve(sysd, dev, "mixed")
+        assert ra["iters"] == rq["result"]["iters"]
+        assert np.array_equal(xa, rq["x"])

@@ -1,32 +1,31 @@
-"""Operator assembly on the device vs the numpy strategies (GPU).
+"""Operator assembly in the HIP library vs the numpy strategies.
 
 Pressure / magnetic / volume-control arithmetic has no transcendental
-functions and must match bit for bit.  psi is bit-exact too (sample only).
-Surface tension and calc_volume go through sin/cos (level_set.py:24-37),
-whose FP32 results may differ by an ulp between the GPU math library and
-numpy: those compare within 1e-5 relative.
+functions and must match bit for bit; psi (sample only) too.  Surface tension
+and calc_volume go through sin/cos (level_set.py:24-37), whose FP32 results
+may differ by an ulp between math libraries: those compare within 1e-5
+relative.  Runs on the MI355X (`dev` = "cuda:0", marked gpu) and on the CPU
+emulation build (`dev` = "cpu").
 """
 import numpy as np
 import pytest
 import torch
 
 import scenes
-from oracle import core
 from oracle import systems as osys
 from oracle import strategies as S
 
-pytestmark = pytest.mark.gpu
 
-
-def _assemble_gpu(sysd):
+def _assemble(sysd, dev):
+    import _lib
     import mgpcg
     import systems
 
-    dev = systems.upload(sysd, "cuda:0")
-    s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], 2, 10, device="cuda:0")
-    systems.assemble(s, dev)
-    torch.cuda.synchronize()
-    return s, dev
+    d = systems.upload(sysd, dev)
+    s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], 2, 10, device=dev)
+    systems.assemble(s, d)
+    _lib.synchronize(dev)
+    return s
 
 
 def _cmp(s, h, b, exact):
@@ -54,29 +53,52 @@ def _cmp(s, h, b, exact):
     lambda: scenes.pool_sphere_system([32, 48, 32], 6, L=4, kind="volume"),
     lambda: scenes.pool_sphere_system([32, 32, 48], 7, L=4, kind="psi"),
 ], ids=["c1_2d_64", "c2_32", "pressure_48x32x64", "volume_32x48x32", "psi_32x32x48"])
-def test_assembly_bit_exact(make):
+def test_assembly_bit_exact(dev, make):
     sysd = make()
-    s, _ = _assemble_gpu(sysd)
+    s = _assemble(sysd, dev)
     h, b = osys.assemble(sysd)
     _cmp(s, h, b, exact=True)
 
 
 @pytest.mark.parametrize("d", [0, 1, 2])
-def test_surface_tension_assembly(d):
+def test_surface_tension_assembly(dev, d):
     sysd = scenes.pool_sphere_system([32, 32, 32], 8, L=3, kind="surface")
     sysd["d"] = d
-    s, _ = _assemble_gpu(sysd)
+    s = _assemble(sysd, dev)
     h, b = osys.assemble(sysd)
     _cmp(s, h, b, exact=False)
 
 
-def test_calc_volume():
+def test_magnetic_b_bit_exact(dev):
+    import magnetic_force
+    import mgpcg
+    import systems
+
+    sysd = scenes.c2_3d(seed=3, n=32, L=3)
+    rng = np.random.default_rng(4)
+    H = []
+    for k in range(3):
+        shape = [n + (1 if a == k else 0) for a, n in enumerate(sysd["res"])]
+        H.append((1e3 * rng.uniform(-1, 1, shape)).astype(np.float32))
+    d = systems.upload(sysd, dev)
+    Hd = [torch.from_numpy(h).to(dev) for h in H]
+    s = mgpcg.MGPCGPoissonSolver(3, sysd["res"], 3, 2, 10, device=dev)
+    mu0, kk = 4 * np.pi * 1e-7, 5
+    st = magnetic_force.MagneticForceStrategy(3, d["velocity"], True, d["phi"], 1e-3, mu0, kk, Hd)
+    st.scale_A, st.scale_b = sysd["scale_A"], sysd["scale_b"]
+    s.reinitialize(d["cell_type"], st)
+    ref = S.magnetic_build_b(np.ascontiguousarray(sysd["cell_type"], np.int8), sysd["velocity"], H,
+                             sysd["scale_A"], sysd["scale_b"], 1e-3, mu0, kk)
+    assert np.array_equal(s.b.cpu().numpy(), ref)
+
+
+def test_calc_volume(dev):
     import volume_control
     from level_set import LevelSet
 
     sysd = scenes.c2_3d(seed=1, n=32, L=3)
-    phi = torch.from_numpy(sysd["phi"]).cuda()
-    ls = LevelSet(3, sysd["res"], sysd["dx"], phi=phi)
+    phi = torch.from_numpy(sysd["phi"]).to(dev)
+    ls = LevelSet(3, sysd["res"], sysd["dx"], phi=phi, device=dev)
     st = volume_control.PressureProjectWithVolumeControlStrategy(
         3, [], True, phi, 0.0, ls, sysd["dt"])
     vol = st.calc_volume()
