@@ -103,8 +103,35 @@ int orc_coarsen(int dim, const int *nfine, const int8_t *fine, int8_t *coarse) {
   return 0;
 }
 
-/* Deterministic FP64 masked dot: exact products, 4096-cell chunks in order. */
+/* Masked dot: each product exact (FP32 x FP32 fits FP64; TwoProd for FP64),
+ * summed as a double-double (TwoSum) over 4096-cell chunks in order, so the
+ * rounded result does not depend on the summation order (except at an exact
+ * rounding tie) and matches the HIP library's tree reduction. */
 #define DOT_CHUNK 4096
+
+typedef struct {
+  double hi, lo;
+} dd_t;
+
+static inline dd_t dd_add(dd_t a, dd_t b) {
+  const double s = a.hi + b.hi;
+  const double bp = s - a.hi;
+  double e = (a.hi - (s - bp)) + (b.hi - bp);
+  e = e + (a.lo + b.lo);
+  const double h = s + e;
+  const double l = e - (h - s);
+  dd_t r = {h, l};
+  return r;
+}
+static inline dd_t dd_prod_f32(float a, float b) {
+  dd_t r = {(double)a * (double)b, 0.0};
+  return r;
+}
+static inline dd_t dd_prod_f64(double a, double b) {
+  const double p = a * b;
+  dd_t r = {p, fma(a, b, -p)};
+  return r;
+}
 
 #define DEFINE_KERNELS(T, SFX)                                                                  \
   /* neighbor_sum (mgpcg.py:96-102) */                                                          \
@@ -213,22 +240,22 @@ int orc_coarsen(int dim, const int *nfine, const int8_t *fine, int8_t *coarse) {
         }                                                                                       \
   }                                                                                             \
                                                                                                 \
-  /* reduce (mgpcg.py:211-216): FLUID-masked dot, deterministic FP64 */                        \
+  /* reduce (mgpcg.py:211-216): FLUID-masked dot, double-double, deterministic */              \
   double orc_dot_##SFX(long N, const int8_t *gt, const T *p, const T *q) {                      \
     long nch = (N + DOT_CHUNK - 1) / DOT_CHUNK;                                                 \
-    double *part = (double *)malloc(sizeof(double) * (nch > 0 ? nch : 1));                      \
+    dd_t *part = (dd_t *)malloc(sizeof(dd_t) * (nch > 0 ? nch : 1));                            \
     _Pragma("omp parallel for schedule(static)")                                                \
     for (long ch = 0; ch < nch; ++ch) {                                                         \
-      double acc = 0.0;                                                                         \
+      dd_t acc = {0.0, 0.0};                                                                    \
       long e = (ch + 1) * DOT_CHUNK < N ? (ch + 1) * DOT_CHUNK : N;                             \
       for (long i = ch * DOT_CHUNK; i < e; ++i)                                                 \
-        if (gt[i] == FLUID) acc += (double)p[i] * (double)q[i];                                 \
+        if (gt[i] == FLUID) acc = dd_add(acc, dd_prod_##SFX(p[i], q[i]));                       \
       part[ch] = acc;                                                                           \
     }                                                                                           \
-    double s = 0.0;                                                                             \
-    for (long ch = 0; ch < nch; ++ch) s += part[ch];                                            \
+    dd_t s = {0.0, 0.0};                                                                        \
+    for (long ch = 0; ch < nch; ++ch) s = dd_add(s, part[ch]);                                  \
     free(part);                                                                                 \
-    return s;                                                                                   \
+    return s.hi + s.lo;                                                                         \
   }                                                                                             \
                                                                                                 \
   /* true residual r = b - A x in FP64 (reported in every mode) */                             \

@@ -116,6 +116,34 @@ __host__ __device__ __forceinline__ int slot_of(const Geo& g, int i, int j, int 
 }
 __host__ __device__ __forceinline__ int colour_of(int i, int j, int k) { return (i + j + k) & 1; }
 
+// Dot products: every product of two FP32 values is exact in FP64 (for FP64
+// operands TwoProd gives the exact product as a pair), and the sum is
+// accumulated as a double-double (Knuth's TwoSum), so the rounded FP64 result
+// does not depend on the summation order except at an exact rounding tie.
+// The CPU oracle uses the same accumulation, so both agree to the last bit
+// although they add in different orders.
+struct DD {
+  double hi, lo;
+};
+__host__ __device__ __forceinline__ DD dd_zero() { return DD{0.0, 0.0}; }
+__host__ __device__ __forceinline__ DD dd_add(DD a, DD b) {
+  const double s = a.hi + b.hi;
+  const double bp = s - a.hi;
+  double e = (a.hi - (s - bp)) + (b.hi - bp);
+  e = e + (a.lo + b.lo);
+  const double h = s + e;
+  const double l = e - (h - s);
+  return DD{h, l};
+}
+__host__ __device__ __forceinline__ DD dd_prod(float a, float b) {
+  return DD{(double)a * (double)b, 0.0};  // exact: 24 + 24 bits < 53
+}
+__host__ __device__ __forceinline__ DD dd_prod(double a, double b) {
+  const double p = a * b;
+  return DD{p, fma(a, b, -p)};
+}
+__host__ __device__ __forceinline__ double dd_value(DD a) { return a.hi + a.lo; }
+
 // Device-resident CG state of one solve (all scalars of mgpcg.py:150-209).
 struct CGState {
   double r0r0, tol, zr_old, zr_new, pAp, alpha, beta, rr, true_rr;

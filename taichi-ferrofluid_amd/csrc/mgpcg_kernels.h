@@ -55,7 +55,7 @@ struct SysDesc {
   T* p[2][2];  // [iteration parity][colour]
   T* Ap[2];
   CGState* st;
-  double* partials;  // one per virtual block of the level-0 CG kernels
+  DD* partials;  // one per virtual block of the level-0 CG kernels
   int nparts;
   int coarse_from;  // first level handled by the single-workgroup coarse kernel
   int L;
@@ -115,6 +115,36 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+__device__ __forceinline__ DD wave_sum_dd(DD v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    DD o;
+    o.hi = __shfl_xor(v.hi, off, 64);
+    o.lo = __shfl_xor(v.lo, off, 64);
+    v = dd_add(v, o);
+  }
+  return v;
+}
+
+// Block total (every thread gets it) of a double-double.
+template <int NT>
+__device__ __forceinline__ DD block_sum_dd(DD v) {
+  __shared__ double smh[NT / 64];
+  __shared__ double sml[NT / 64];
+  v = wave_sum_dd(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    smh[wid] = v.hi;
+    sml[wid] = v.lo;
+  }
+  __syncthreads();
+  DD t = dd_zero();
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t = dd_add(t, DD{smh[w], sml[w]});
+  __syncthreads();
+  return t;
 }
 
 template <int NT>
@@ -478,7 +508,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_init(const SysDesc<T, TX>* __restr
     const SysDesc<T, TX>& D = sys[item.desc];
     const int lb = it.vb - item.block0;
     const int t = lb * BLOCK + threadIdx.x;
-    double acc = 0.0;
+    DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -488,10 +518,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_init(const SysDesc<T, TX>* __restr
         D.p[0][c][t] = (T)0;
         D.p[1][c][t] = (T)0;
         D.Ap[c][t] = (T)0;
-        if (D.P.gt[c][t] == MG_FLUID) acc += (double)bv * (double)bv;
+        if (D.P.gt[c][t] == MG_FLUID) acc = dd_add(acc, dd_prod(bv, bv));
       }
     }
-    double s = block_sum<BLOCK>(acc);
+    DD s = block_sum_dd<BLOCK>(acc);
     if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
@@ -524,7 +554,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap(const SysDesc<T, TX>* __restric
       }
       return (T)((double)D.P.z[c][u] + beta * (double)pold[c][u]);
     };
-    double acc = 0.0;
+    DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -537,10 +567,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap(const SysDesc<T, TX>* __restric
         T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return upd(c ^ 1, u); });
         T ap = rr + ns;
         D.Ap[c][t] = ap;
-        acc += (double)pv * (double)ap;
+        acc = dd_add(acc, dd_prod(pv, ap));
       }
     }
-    double s = block_sum<BLOCK>(acc);
+    DD s = block_sum_dd<BLOCK>(acc);
     if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
@@ -562,7 +592,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     const int prec = st->precision;
     const bool refine = prec == MIXED && st->refine_every > 0 &&
                         st->since_refine + 1 >= st->refine_every;
-    double acc = 0.0;
+    DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -576,7 +606,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
           T ta = (T)a32 * apv;
           T rv = D.P.r[c][t] - ta;
           D.P.r[c][t] = rv;
-          acc += (double)rv * (double)rv;
+          acc = dd_add(acc, dd_prod(rv, rv));
         } else if (prec == F64) {
           const double alpha = st->alpha;
           T tp = (T)(alpha * (double)pv);
@@ -584,19 +614,19 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
           T ta = (T)(alpha * (double)apv);
           T rv = D.P.r[c][t] - ta;
           D.P.r[c][t] = rv;
-          acc += (double)rv * (double)rv;
+          acc = dd_add(acc, dd_prod(rv, rv));
         } else {
           const double alpha = st->alpha;
           D.x[c][t] = (TX)((double)D.x[c][t] + alpha * (double)pv);
           if (!refine) {
             T rv = (T)((double)D.P.r[c][t] - alpha * (double)apv);
             D.P.r[c][t] = rv;
-            acc += (double)rv * (double)rv;
+            acc = dd_add(acc, dd_prod(rv, rv));
           }
         }
       }
     }
-    double s = block_sum<BLOCK>(acc);
+    DD s = block_sum_dd<BLOCK>(acc);
     if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
@@ -613,7 +643,7 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
     if (st->done || !st->pending_true) continue;
     const int lb = it.vb - item.block0;
     const int t = lb * BLOCK + threadIdx.x;
-    double acc = 0.0;
+    DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -634,10 +664,10 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
         double rv = (double)D.b[c][t] - Axv;
         T r32 = (T)rv;
         D.P.r[c][t] = r32;
-        acc += (double)r32 * (double)r32;
+        acc = dd_add(acc, dd_prod(r32, r32));
       }
     }
-    double s = block_sum<BLOCK>(acc);
+    DD s = block_sum_dd<BLOCK>(acc);
     if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
@@ -654,13 +684,13 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
     if (D.st->done) continue;
     const int lb = it.vb - item.block0;
     const int t = lb * BLOCK + threadIdx.x;
-    double acc = 0.0;
+    DD acc = dd_zero();
     if (t < D.g.size) {
       smooth_slot<T, DIM>(D.g, D.P, 0, t);
-      if (D.P.gt[0][t] == MG_FLUID) acc += (double)D.P.z[0][t] * (double)D.P.r[0][t];
-      if (D.P.gt[1][t] == MG_FLUID) acc += (double)D.P.z[1][t] * (double)D.P.r[1][t];
+      if (D.P.gt[0][t] == MG_FLUID) acc = dd_add(acc, dd_prod(D.P.z[0][t], D.P.r[0][t]));
+      if (D.P.gt[1][t] == MG_FLUID) acc = dd_add(acc, dd_prod(D.P.z[1][t], D.P.r[1][t]));
     }
-    double s = block_sum<BLOCK>(acc);
+    DD s = block_sum_dd<BLOCK>(acc);
     if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
@@ -677,12 +707,12 @@ __global__ void __launch_bounds__(BLOCK) k_dot_zr(const SysDesc<T, TX>* __restri
     if (D.st->done) continue;
     const int lb = it.vb - item.block0;
     const int t = lb * BLOCK + threadIdx.x;
-    double acc = 0.0;
+    DD acc = dd_zero();
     if (t < D.g.size) {
-      if (D.P.gt[0][t] == MG_FLUID) acc += (double)D.P.z[0][t] * (double)D.P.r[0][t];
-      if (D.P.gt[1][t] == MG_FLUID) acc += (double)D.P.z[1][t] * (double)D.P.r[1][t];
+      if (D.P.gt[0][t] == MG_FLUID) acc = dd_add(acc, dd_prod(D.P.z[0][t], D.P.r[0][t]));
+      if (D.P.gt[1][t] == MG_FLUID) acc = dd_add(acc, dd_prod(D.P.z[1][t], D.P.r[1][t]));
     }
-    double s = block_sum<BLOCK>(acc);
+    DD s = block_sum_dd<BLOCK>(acc);
     if (threadIdx.x == 0) D.partials[lb] = s;
   }
 }
@@ -701,9 +731,9 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const SysDesc<T, TX>* __rest
   CGState* st = D.st;
   if (st->done) return;
   if (OP == FIN_TRUE && !st->pending_true) return;
-  double v = 0.0;
-  for (int i = threadIdx.x; i < D.nparts; i += BLOCK) v += D.partials[i];
-  const double tot = block_sum<BLOCK>(v);
+  DD v = dd_zero();
+  for (int i = threadIdx.x; i < D.nparts; i += BLOCK) v = dd_add(v, D.partials[i]);
+  const double tot = dd_value(block_sum_dd<BLOCK>(v));
   if (threadIdx.x != 0) return;
   if (OP == FIN_INIT) {
     // mgpcg.py:157-164

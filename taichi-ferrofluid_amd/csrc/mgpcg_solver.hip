@@ -67,7 +67,7 @@ struct mgpcg_solver {
   void* p[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   void* Ap[2] = {nullptr, nullptr};
   CGState* st = nullptr;
-  double* partials = nullptr;
+  DD* partials = nullptr;
   int nparts = 0;
   int coarse_from = 0;
   char* pool = nullptr;
@@ -897,7 +897,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
   }
   size_t ost = take(sizeof(CGState));
   s->nparts = nblk(g0.size);
-  size_t opart = take(sizeof(double) * s->nparts);
+  size_t opart = take(sizeof(DD) * s->nparts);
   s->pool_bytes = off;
 
   auto fail = [&](hipError_t err, const char* what) {
@@ -931,7 +931,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     s->Ap[c] = s->pool + oap[c];
   }
   s->st = (CGState*)(s->pool + ost);
-  s->partials = (double*)(s->pool + opart);
+  s->partials = (DD*)(s->pool + opart);
   if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "hipDeviceSynchronize");
   *out = s;
   return 0;
