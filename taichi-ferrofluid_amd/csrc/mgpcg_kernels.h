@@ -340,6 +340,27 @@ __device__ __forceinline__ void zero_slot(const LevelPtrs<T>& P, int t) {
   P.z[1][t] = (T)0;
 }
 
+// z.fill(0); smooth(l, 0) on colour 0 only: z0 = r0 / Adiag0 (0 off FLUID).
+// The colour-1 values left from before are not zeroed: the next half-sweep
+// rewrites every colour-1 FLUID value without reading it, and the others are
+// inert (zero couplings).
+template <typename T>
+__device__ __forceinline__ void first_half_sweep_slot(const LevelPtrs<T>& P, int t) {
+  P.z[0][t] = P.gt[0][t] == MG_FLUID ? P.r[0][t] / P.Adiag[0][t] : (T)0;
+}
+
+// prolongate(l) on colour 0 only (mgpcg.py:120-123): the colour-1 FLUID values
+// are rewritten by the smooth(l, 1) that follows without reading them.
+template <typename T>
+__device__ __forceinline__ void prolongate0_slot(const Geo& g, const Geo& gc, T* z0,
+                                                 const T* zc0, const T* zc1, int t) {
+  Slot s = decode(g, t, 0);
+  if (!s.interior) return;
+  const int pi = s.i >> 1, pj = s.j >> 1, pk = s.k >> 1;
+  const int ps = slot_of(gc, pi, pj, pk);
+  z0[t] = z0[t] + (colour_of(pi, pj, pk) ? zc1[ps] : zc0[ps]);
+}
+
 // ---------------------------------------------------------------------------
 // batched stage kernels over level descriptors
 // ---------------------------------------------------------------------------
@@ -350,6 +371,8 @@ enum StageOp {
   ST_PROLONG_SMOOTH1 = 3,  // prolongate(l); smooth(l, 1)
   ST_ZERO = 4,
   ST_PROLONGATE = 5,
+  ST_FIRST = 6,       // fill(0); smooth(l, 0), colour 0 only
+  ST_PROLONGATE0 = 7,  // prolongate(l), colour 0 only
 };
 
 template <typename T, int DIM, int OP>
@@ -368,6 +391,9 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     else if (OP == ST_PROLONG_SMOOTH1)
       prolong_smooth1_slot<T, DIM>(D.g, D.gc, D.P, D.C.z[0], D.C.z[1], t);
     else if (OP == ST_ZERO) zero_slot<T>(D.P, t);
+    else if (OP == ST_FIRST) first_half_sweep_slot<T>(D.P, t);
+    else if (OP == ST_PROLONGATE0)
+      prolongate0_slot<T>(D.g, D.gc, D.P.z[0], D.C.z[0], D.C.z[1], t);
     else if (OP == ST_PROLONGATE) {
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 0, t);
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 1, t);
@@ -375,7 +401,32 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
   }
 }
 
-// restrict(l): virtual blocks cover the coarse level's slots, both colours per thread.
+// restrict(l), child-parallel: 16 lanes per coarse slot -- 8 per coarse
+// colour, one per child (o0, o1, o2) in lexicographic order.  Every lane
+// forms its child's residual (0 when the child is not FLUID, which leaves the
+// sum unchanged); the first lane of the eight gathers them with shuffles and
+// adds them in the reference order.  Virtual blocks cover 16 coarse slots.
+constexpr int RESTRICT_SLOTS_PER_BLOCK = BLOCK / 16;
+
+template <typename T, int DIM>
+__device__ __forceinline__ T child_res(const Geo& gf, const LevelPtrs<T>& F, const Slot& s,
+                                       int ch) {
+  const int o0 = (ch >> 2) & 1, o1 = (ch >> 1) & 1, o2 = ch & 1;
+  if (DIM == 2 && o0) return (T)0;
+  const int i = (DIM == 3) ? 2 * s.i + o0 : 0;
+  const int j = 2 * s.j + o1, k = 2 * s.k + o2;
+  const int c = colour_of(i, j, k);
+  const int t = slot_of(gf, i, j, k);
+  if (F.gt[c][t] != MG_FLUID) return (T)0;
+  const int q = (k + 1) & 1;  // padded k' = k + 1 = 2m + q
+  Nbr<DIM> n = neighbours<DIM>(gf, t, q);
+  const T* zo = F.z[c ^ 1];
+  T Az = F.Adiag[c][t] * F.z[c][t];
+  T ns = nsum<T, DIM>(F, c, t, n, [&](int u) { return zo[u]; });
+  Az = Az + ns;
+  return F.r[c][t] - Az;
+}
+
 template <typename T, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_restrict(const LevelDesc<T>* __restrict__ descs,
                                                     Stage S) {
@@ -384,10 +435,28 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(const LevelDesc<T>* __restri
     const Item item = S.items[ii];
     const LevelDesc<T>& D = descs[item.desc];
     if (D.st->done) continue;
-    const int tc = (it.vb - item.block0) * BLOCK + threadIdx.x;
-    if (tc >= D.gc.size) continue;
-    restrict_slot<T, DIM>(D.g, D.P, D.gc, D.C.r[0], D.C.r[1], 0, tc);
-    restrict_slot<T, DIM>(D.g, D.P, D.gc, D.C.r[0], D.C.r[1], 1, tc);
+    const int tc = (it.vb - item.block0) * RESTRICT_SLOTS_PER_BLOCK + (threadIdx.x >> 4);
+    const int cc = (threadIdx.x >> 3) & 1;
+    const int ch = threadIdx.x & 7;
+    T res = (T)0;
+    bool interior = false;
+    Slot s;
+    if (tc < D.gc.size) {
+      s = decode(D.gc, tc, cc);
+      interior = s.interior;
+      if (interior) res = child_res<T, DIM>(D.g, D.P, s, ch);
+    }
+    // gather the eight children into lane ch == 0 through LDS (all lanes take part)
+    __shared__ T xch[BLOCK];
+    xch[threadIdx.x] = res;
+    __syncthreads();
+    if (ch == 0 && interior) {
+      T sum = (T)0;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) sum = sum + xch[threadIdx.x + o];
+      (cc ? D.C.r[1] : D.C.r[0])[tc] = sum;
+    }
+    __syncthreads();  // xch is reused by the next virtual block
   }
 }
 
@@ -565,6 +634,69 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap(const SysDesc<T, TX>* __restric
         pnew[c][t] = pv;
         T rr = D.P.Adiag[c][t] * pv;
         T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return upd(c ^ 1, u); });
+        T ap = rr + ns;
+        D.Ap[c][t] = ap;
+        acc = dd_add(acc, dd_prod(pv, ap));
+      }
+    }
+    DD s = block_sum_dd<BLOCK>(acc);
+    if (threadIdx.x == 0) D.partials[lb] = s;
+  }
+}
+
+// update_p alone (mgpcg.py:234-238): p_next = z + beta p on FLUID cells.
+template <typename T, typename TX>
+__global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __restrict__ sys,
+                                                       Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const SysDesc<T, TX>& D = sys[item.desc];
+    const CGState* st = D.st;
+    if (st->done) continue;
+    const int t = (it.vb - item.block0) * BLOCK + threadIdx.x;
+    if (t >= D.g.size) continue;
+    const int par = st->iter & 1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (D.P.gt[c][t] != MG_FLUID) continue;
+      const T pold = D.p[par][c][t];
+      T pv;
+      if (st->precision == PLAIN) {
+        T tt = (T)st->beta32 * pold;
+        pv = D.P.z[c][t] + tt;
+      } else {
+        pv = (T)((double)D.P.z[c][t] + st->beta * (double)pold);
+      }
+      D.p[par ^ 1][c][t] = pv;
+    }
+  }
+}
+
+// compute_Ap (mgpcg.py:218-224) on the p written by k_cg_update_p, partial p.Ap.
+template <typename T, typename TX, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __restrict__ sys,
+                                                      Stage S) {
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const SysDesc<T, TX>& D = sys[item.desc];
+    const CGState* st = D.st;
+    if (st->done) continue;
+    const int lb = it.vb - item.block0;
+    const int t = lb * BLOCK + threadIdx.x;
+    const int par = (st->iter & 1) ^ 1;
+    DD acc = dd_zero();
+    if (t < D.g.size) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (D.P.gt[c][t] != MG_FLUID) continue;
+        const int q = row_q(D.g, t, c);
+        Nbr<DIM> n = neighbours<DIM>(D.g, t, q);
+        const T* po = D.p[par][c ^ 1];
+        const T pv = D.p[par][c][t];
+        T rr = D.P.Adiag[c][t] * pv;
+        T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return po[u]; });
         T ap = rr + ns;
         D.Ap[c][t] = ap;
         acc = dd_add(acc, dd_prod(pv, ap));

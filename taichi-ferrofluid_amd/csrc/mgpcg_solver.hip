@@ -51,6 +51,9 @@ struct LevelBuf {
 
 int g_coarse_max_slots = 4096;  // levels at most this large go to k_coarse
 int g_max_grid = 2048;          // workgroups of a persistent stage launch
+int g_fuse_first = 1;    // first pre-smoothing pair in one kernel (else two)
+int g_fuse_prolong = 1;  // prolongation inside the first post-smoothing half-sweep
+int g_fuse_p = 1;        // update_p inside compute_Ap
 
 }  // namespace
 
@@ -80,6 +83,9 @@ struct mgpcg_solver {
 namespace {
 
 inline int nblk(int n) { return (n + BLOCK - 1) / BLOCK; }
+inline int rblk(int coarse_slots) {
+  return (coarse_slots + RESTRICT_SLOTS_PER_BLOCK - 1) / RESTRICT_SLOTS_PER_BLOCK;
+}
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 template <typename T>
@@ -439,7 +445,7 @@ int engine_build_items(Engine* e) {
               [l, L](int m) { return m * L + l; });
     add_stage(e->st_restrict[l],
               [l, L](const Member& M) { return l < M.s->coarse_from && l + 1 < L; },
-              [l, L](const Member& M) { return l + 1 < L ? nblk(M.s->geo[l + 1].size) : 0; },
+              [l, L](const Member& M) { return l + 1 < L ? rblk(M.s->geo[l + 1].size) : 0; },
               [l, L](int m) { return m * L + l; });
   }
   // members whose bottom level is too large for the coarse kernel
@@ -535,7 +541,12 @@ struct Sched {
     for (int l = 0; l + 1 < L; ++l) {
       const Engine::StageRef& S = e->st_slot[l];
       if (nu > 0) {
-        stage<ST_PAIR_FROM_ZERO>(e, S);
+        if (g_fuse_first) {
+          stage<ST_PAIR_FROM_ZERO>(e, S);
+        } else {
+          stage<ST_FIRST>(e, S);
+          stage<ST_SMOOTH1>(e, S);
+        }
         for (int i = 1; i < nu; ++i) {
           stage<ST_SMOOTH0>(e, S);
           stage<ST_SMOOTH1>(e, S);
@@ -570,7 +581,12 @@ struct Sched {
     for (int l = L - 2; l >= 0; --l) {
       const Engine::StageRef& S = e->st_slot[l];
       if (nu > 0) {
-        stage<ST_PROLONG_SMOOTH1>(e, S);
+        if (g_fuse_prolong) {
+          stage<ST_PROLONG_SMOOTH1>(e, S);
+        } else {
+          stage<ST_PROLONGATE0>(e, S);
+          stage<ST_SMOOTH1>(e, S);
+        }
         for (int i = 0; i < nu; ++i) {
           if (i > 0) stage<ST_SMOOTH1>(e, S);
           const bool last = (i == nu - 1);
@@ -602,8 +618,15 @@ struct Sched {
       sync();
     }
     if (e->st_cg.nblocks) {
-      hipLaunchKernelGGL((k_cg_ap<T, TX, DIM>), dim3(grid_of(e->st_cg)), dim3(BLOCK), 0,
-                         e->stream, sys(e), stage_of(e, e->st_cg));
+      if (g_fuse_p) {
+        hipLaunchKernelGGL((k_cg_ap<T, TX, DIM>), dim3(grid_of(e->st_cg)), dim3(BLOCK), 0,
+                           e->stream, sys(e), stage_of(e, e->st_cg));
+      } else {
+        hipLaunchKernelGGL((k_cg_update_p<T, TX>), dim3(grid_of(e->st_cg)), dim3(BLOCK), 0,
+                           e->stream, sys(e), stage_of(e, e->st_cg));
+        hipLaunchKernelGGL((k_cg_ap_only<T, TX, DIM>), dim3(grid_of(e->st_cg)), dim3(BLOCK), 0,
+                           e->stream, sys(e), stage_of(e, e->st_cg));
+      }
       fin<FIN_ALPHA>(e, e->fin_cg);
       sync();
       hipLaunchKernelGGL((k_cg_xr<T, TX>), dim3(grid_of(e->st_cg)), dim3(BLOCK), 0, e->stream,
@@ -968,6 +991,15 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
       MG_REQUIRE(value >= 8, "grid limit must be >= 8");
       g_max_grid = value;
       return 0;
+    case MGPCG_OPT_FUSE_FIRST:
+      g_fuse_first = value != 0;
+      return 0;
+    case MGPCG_OPT_FUSE_PROLONG:
+      g_fuse_prolong = value != 0;
+      return 0;
+    case MGPCG_OPT_FUSE_P:
+      g_fuse_p = value != 0;
+      return 0;
     default:
       (void)s;
       mg_set_error("unknown option %d", option);
@@ -1152,7 +1184,7 @@ int prepare_op(Engine* e, const mgpcg_solver* s, int op, int level, Engine::Stag
   it.desc = level;  // member 0: LevelDesc index = level
   it.block0 = 0;
   int nb = nblk(s->geo[level].size);
-  if (op == MGPCG_OP_RESTRICT) nb = nblk(s->geo[level + 1].size);
+  if (op == MGPCG_OP_RESTRICT) nb = rblk(s->geo[level + 1].size);
   if (op == MGPCG_OP_COMPUTE_AP) it.desc = 0;  // SysDesc of member 0
   MG_HIP(hipStreamSynchronize(e->stream));
   MG_HIP(hipMemcpy(e->d_items, &it, sizeof(Item), hipMemcpyHostToDevice));

@@ -122,6 +122,20 @@ T shfl(std::vector<T>& buf, T v, int mask) {
   syncthreads();
   return r;
 }
+template <typename T>
+T shfl_src(std::vector<T>& buf, T v, int src, int width) {
+  const unsigned tid = g_cur->tid;
+  if (buf.size() < blockDim.x) buf.resize(blockDim.x);
+  buf[tid] = v;
+  syncthreads();
+  const unsigned base = tid & ~(unsigned)(width - 1);
+  const unsigned from = base + ((unsigned)src & (unsigned)(width - 1));
+  T r = from < blockDim.x ? buf[from] : v;
+  syncthreads();
+  return r;
+}
+float shfl_idx(float v, int src, int width) { return shfl_src(g_xf, v, src, width); }
+double shfl_idx(double v, int src, int width) { return shfl_src(g_xd, v, src, width); }
 double shfl_xor_d(double v, int mask) { return shfl(g_xd, v, mask); }
 float shfl_xor_f(float v, int mask) { return shfl(g_xf, v, mask); }
 int shfl_xor_i(int v, int mask) { return shfl(g_xi, v, mask); }

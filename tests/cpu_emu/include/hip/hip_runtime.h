@@ -57,6 +57,8 @@ extern dim3 threadIdx, blockIdx, gridDim, blockDim;
 void syncthreads();
 double shfl_xor_d(double v, int mask);
 int shfl_xor_i(int v, int mask);
+float shfl_idx(float v, int src, int width);
+double shfl_idx(double v, int src, int width);
 float shfl_xor_f(float v, int mask);
 // run `body(thread)` for every thread of every block of the grid
 void run_grid(dim3 grid, dim3 block, void (*tramp)(void*), void* ctx);
@@ -81,6 +83,10 @@ inline float __shfl_xor(float v, int mask, int width = 64) {
 inline int __shfl_xor(int v, int mask, int width = 64) {
   (void)width;
   return hipemu::shfl_xor_i(v, mask);
+}
+template <typename T>
+inline T __shfl(T v, int src, int width = 64) {
+  return hipemu::shfl_idx(v, src, width);
 }
 inline void __threadfence() {}
 inline int min(int a, int b) { return a < b ? a : b; }

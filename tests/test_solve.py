@@ -48,7 +48,7 @@ def _emu(dev):
 
 SYSTEMS = {
     "c1_2d": lambda emu: scenes.c1_2d(seed=0, n=64 if emu else 128),
-    "c2_3d_small": lambda emu: scenes.c2_3d(seed=1, n=32 if emu else 64, L=3 if emu else 4),
+    "c2_3d_small": lambda emu: scenes.c2_3d(seed=1, n=24 if emu else 64, L=3 if emu else 4),
     "c2_3d_128": lambda emu: None if emu else scenes.c2_3d(seed=1, n=128, L=5),
 }
 
@@ -76,29 +76,45 @@ def test_hostsync_loop_equals_device_loop(dev):
     assert np.array_equal(xa, xb)
 
 
-@pytest.mark.parametrize("coarse_max,max_grid", [(0, 2048), (1 << 30, 2048), (4096, 8),
-                                                 (4096, 64)])
-def test_schedule_knobs_agree(dev, coarse_max, max_grid):
-    """Where the coarse levels run and how many workgroups stride over the
-    virtual blocks must not change a single bit of x."""
+KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 1, "fuse_prolong": 1,
+                 "fuse_p": 1}
+
+
+def set_knobs(**kw):
     import _lib
 
-    sysd = SYSTEMS["c2_3d_small"](_emu(dev))
-    xa, outa, _ = gpu_solve(sysd, dev, "plain")
     lib = _lib.load()
+    opt = {"coarse_max": _lib.OPT_COARSE_MAX_SLOTS, "max_grid": _lib.OPT_MAX_GRID,
+           "fuse_first": _lib.OPT_FUSE_FIRST, "fuse_prolong": _lib.OPT_FUSE_PROLONG,
+           "fuse_p": _lib.OPT_FUSE_P}
+    for k, v in kw.items():
+        _lib.check(lib.mgpcg_set_option(None, opt[k], v))
+
+
+@pytest.mark.parametrize("knobs", [
+    {"coarse_max": 0}, {"coarse_max": 1 << 30}, {"max_grid": 8}, {"max_grid": 64},
+    {"fuse_first": 0}, {"fuse_prolong": 0}, {"fuse_p": 0},
+    {"fuse_first": 0, "fuse_prolong": 0, "fuse_p": 0},
+], ids=lambda k: "-".join(f"{a}{b}" for a, b in k.items()))
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_schedule_knobs_agree(dev, knobs, precision):
+    """Where the coarse levels run, how many workgroups stride over the
+    virtual blocks and which stages are fused must not change a bit of x."""
+    if _emu(dev) and precision == "mixed" and len(knobs) < 3:
+        pytest.skip("on the emulator the all-unfused case covers MIXED")
+    sysd = SYSTEMS["c2_3d_small"](_emu(dev))
+    xa, outa, _ = gpu_solve(sysd, dev, precision)
     try:
-        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, coarse_max))
-        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_MAX_GRID, max_grid))
-        xb, outb, _ = gpu_solve(sysd, dev, "plain")
+        set_knobs(**knobs)
+        xb, outb, _ = gpu_solve(sysd, dev, precision)
     finally:
-        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_COARSE_MAX_SLOTS, 4096))
-        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_MAX_GRID, 2048))
+        set_knobs(**KNOB_DEFAULTS)
     assert outa["iters"] == outb["iters"]
     assert np.array_equal(xa, xb)
 
 
 def test_mixed_error_vs_f64(dev):
-    sysd = SYSTEMS["c2_3d_small"](_emu(dev))
+    sysd = scenes.c2_3d(seed=1, n=16, L=3) if _emu(dev) else SYSTEMS["c2_3d_small"](False)
     x, out, _ = gpu_solve(sysd, dev, "mixed")
     x64, out64, _ = gpu_solve(sysd, dev, "f64", rel_tol=1e-24)
     xo, outo, _ = osys.solve(sysd, core.F64, max_iters=500, rel_tol=1e-24)
