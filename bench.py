@@ -1,17 +1,30 @@
 """Benchmark: MGPCG solves/s with p50/p99 latency at 1e-6 relative residual.
 
 Workload (default): the 3D 128^3 ghost-fluid pressure system -- pool y < 0.3
-plus a sphere c=(0.5,0.65,0.5) r=0.15, theta >= 0.01, seeded U[-1,1] MAC
-velocity divergence as RHS, 5 levels, 2+2 red-black sweeps, 10 bottom
-sweeps, rel_tol 1e-12 on squared norms.  One step = one whole solve request:
-assemble b and the hierarchy on the device from the scene fields (already in
-HBM) and solve to tolerance.  The JSON line carries, besides the headline
-mixed-precision rate:
-  * the plain FP32 GPU MGPCG driven like the reference (host round trip after
-    every reduction) and the device-resident plain FP32 loop;
-  * roofline: the level-0 red-black half-sweep (the dominant kernel), its
-    algorithmic bytes per launch over its HIP-event-timed launch duration;
-  * cpu_baseline: the C restatement of mgpcg.py on the host cores (OpenMP).
+plus a sphere c=(0.5,0.65,0.5) r=0.15, theta >= 0.01 (c_max = 100), RHS the
+divergence of a seeded U[-1,1] MAC velocity, 5 levels, 2+2 red-black sweeps,
+10 bottom sweeps, rel_tol 1e-12 on squared norms (1e-6 relative residual).
+Scenes (4 seeds) are uploaded to HBM before timing.
+
+One step = one solve request: assemble b and the multigrid hierarchy on the
+device from the scene and solve it to tolerance.  Requests are served with
+continuous batching (serve.Server) in a closed loop that keeps `--concurrency`
+requests in flight; `value` is requests completed per second over the timed
+steps, p50/p99 the request latencies (submit to result).
+
+Also reported, on the same requests:
+  plain_fp32_gpu_solves_per_s   the plain FP32 GPU port structured like
+                                mgpcg.py (one launch per reference kernel,
+                                scalars on the host) -- the 1x baseline;
+  single_request                concurrency 1 (latency-bound) throughput;
+  error_vs_fp64                 relative L2 error of the mixed result against
+                                an FP64 MGPCG solve to 1e-12 relative residual;
+  roofline                      the level-0 red-black half-sweep, its
+                                algorithmic bytes over its launch durations
+                                measured with HIP events on the engine stream
+                                during the timed steps;
+  cpu_baseline                  the oracle's C restatement of mgpcg.py on the
+                                host cores (OpenMP), a bounded sample.
 """
 import argparse
 import json
@@ -33,16 +46,18 @@ SMOOTH_BYTES_PER_CELL_3D = 20.5
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=64, help="timed solve requests")
+    ap.add_argument("--warmup", type=int, default=8, help="untimed solve requests")
     ap.add_argument("--n", type=int, default=128, help="grid side of the workload")
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--seeds", type=int, default=4, help="distinct scenes cycled through")
+    ap.add_argument("--concurrency", type=int, default=8)
     ap.add_argument("--precision", default="mixed", choices=["plain", "mixed", "f64"])
-    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--baseline-steps", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only (used for profiling runs)")
     return ap.parse_args()
 
 
@@ -58,8 +73,6 @@ def pct(v, q):
 
 def cpu_baseline(n, levels, budget_s):
     """The oracle's C restatement of mgpcg.py (PLAIN) on every host core."""
-    import numpy as np  # noqa: F401
-
     import scenes
     from oracle import core
     from oracle import systems as osys
@@ -76,83 +89,68 @@ def cpu_baseline(n, levels, budget_s):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": count / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{count} solves of the same {n}^3 system (solve only, assembly excluded), "
-                      f"{dt:.1f} s, PLAIN FP32, OMP threads={threads}"}
+    return {"value": round(count / dt, 3), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{count} solves of the seed-1 {n}^3 system (solve only; assembly excluded),"
+                      f" {dt:.1f} s, PLAIN FP32, OpenMP threads={threads}"}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
-        print("bench.py: only --gpus 1 is supported (single-GPU solver; multi-GPU sharding "
-              "is a design note, not built)", file=sys.stderr)
+        print("bench.py: only --gpus 1 is supported: the solver runs on one GPU (how it would "
+              "shard is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
 
     import _lib
     import mgpcg
     import scenes
+    import serve
     import systems
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     n, L = args.n, args.levels
-    scenes_dev = [systems.upload(scenes.c2_3d(seed=1 + i, n=n, L=L), dev)
-                  for i in range(args.seeds)]
+    host_scenes = [scenes.c2_3d(seed=1 + i, n=n, L=L) for i in range(args.seeds)]
+    scenes_dev = [systems.upload(s, dev) for s in host_scenes]
     torch.cuda.synchronize()
+    budget = 64 << 30
 
-    def run(precision, steps, warmup, hostsync=False):
-        solver = mgpcg.MGPCGPoissonSolver(3, [n] * 3, L, 2, 10, precision=precision, device=dev)
-        lat, iters, solve_only = [], [], []
-        for i in range(warmup + steps):
-            sc = scenes_dev[i % len(scenes_dev)]
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            systems.assemble(solver, sc)
-            t1 = time.perf_counter()
-            if hostsync:
-                solver.solve_hostsync(500, rel_tol=1e-12)
-            else:
-                solver.solve(500, rel_tol=1e-12)
-            torch.cuda.synchronize()
-            t2 = time.perf_counter()
-            if i >= warmup:
-                lat.append(t2 - t0)
-                solve_only.append(t2 - t1)
-                iters.append(solver.last["iters"])
-                assert solver.last["status"] == _lib.CONVERGED, solver.last
-        return solver, lat, iters, solve_only
+    def run_served(concurrency, total, warmup, probe=False):
+        srv = serve.Server(budget, precision=args.precision, n_levels=L, device=dev)
+        mk = (lambda i: scenes_dev[i % len(scenes_dev)])
+        serve.closed_loop(srv, mk, warmup, concurrency)  # allocate hierarchies, warm caches
+        if probe:
+            srv.probe(True)
+        recs, secs = serve.closed_loop(srv, mk, total, concurrency)
+        pr = srv.probe(False) if probe else None
+        for r in recs:
+            assert r["result"]["status"] == _lib.CONVERGED, r["result"]
+        return recs, secs, pr, srv
 
-    # headline
-    torch.cuda.synchronize()
-    solver, lat, iters, solve_only = run(args.precision, args.steps, args.warmup)
-    total = sum(lat)
-    value = args.steps / total
+    # headline: continuous batching, closed loop
+    recs, secs, pr, srv = run_served(args.concurrency, args.steps, args.warmup, probe=True)
+    value = args.steps / secs
+    lat = serve.latency_stats(recs)
+    iters = sorted(set(r["result"]["iters"] for r in recs))
 
-    extra = {}
-    if not args.no_baselines:
-        _, lat_p, it_p, _ = run("plain", args.plain_steps, 2, hostsync=True)
-        _, lat_d, it_d, _ = run("plain", args.plain_steps, 2)
-        extra["plain_fp32_hostsync_solves_per_s"] = args.plain_steps / sum(lat_p)
-        extra["plain_fp32_device_loop_solves_per_s"] = args.plain_steps / sum(lat_d)
-        extra["speedup_vs_plain_hostsync"] = value / extra["plain_fp32_hostsync_solves_per_s"]
-        extra["iters_plain"] = sorted(set(it_p))
-
-    # roofline of the dominant kernel: level-0 half-sweep, timed live with HIP events
-    ms = solver.time_op(_lib.OP_SMOOTH, 0, 0, reps=100)
-    cells = n ** 3
-    alg_bytes = SMOOTH_BYTES_PER_CELL_3D * cells
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "mg::k_stage<float, 3, 0> (smooth(0, 0): level-0 red-black half-sweep)",
-                "alg_bytes_per_launch": alg_bytes, "launch_us": round(ms * 1e3, 3)}
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(n, L, args.cpu_seconds)
+    # roofline of the dominant kernel, from its launches inside the timed region
+    nl, ms_tot, cells_tot = pr
+    roofline = None
+    if nl > 0:
+        alg_bytes = SMOOTH_BYTES_PER_CELL_3D * cells_tot / nl
+        launch_s = ms_tot * 1e-3 / nl
+        achieved = alg_bytes / launch_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "mg::k_stage<float, 3, 0, 1> (smooth(0, 0), level-0 red-black "
+                              "half-sweep over every member system)",
+                    "alg_bytes_per_launch": round(alg_bytes), "launch_us": round(launch_s * 1e6, 3),
+                    "launches": nl}
+    del srv
 
     line = {
         "metric": "MGPCG solves/s + p50/p99 latency at 1e-6 rel. residual, 128^3-256^3, 1 GPU",
@@ -161,24 +159,70 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "ms_per_step": round(1e3 * secs / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"plain": "f32", "mixed": "f32+f64", "f64": "f64"}[args.precision],
         "data": "synthetic (seeded scenes, resident in HBM)",
-        "config": {"workload": f"3D {n}^3 ghost-fluid pool+sphere pressure solve, L={L}, "
-                               f"rel residual 1e-6, {args.precision} precision",
-                   "n": n, "levels": L, "seeds": args.seeds},
-        "p50_ms": round(1e3 * pct(lat, 0.5), 3),
-        "p99_ms": round(1e3 * pct(lat, 0.99), 3),
-        "solve_only_p50_ms": round(1e3 * pct(solve_only, 0.5), 3),
-        "iters": sorted(set(iters)),
+        "config": {"workload": f"3D {n}^3 ghost-fluid pool+sphere pressure solve requests, "
+                               f"L={L}, 2+2 sweeps, bottom 10, rel residual 1e-6, "
+                               f"{args.precision} precision, continuous batching with "
+                               f"{args.concurrency} requests in flight",
+                   "n": n, "levels": L, "seeds": args.seeds, "concurrency": args.concurrency},
+        "p50_ms": round(lat["p50_ms"], 3),
+        "p99_ms": round(lat["p99_ms"], 3),
+        "iters": iters,
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": None,
     }
-    line.update({k: (round(v, 3) if isinstance(v, float) else v) for k, v in extra.items()})
+
+    if not args.no_extras:
+        # single request in flight: the latency-bound rate
+        r1, s1, _, srv1 = run_served(1, max(4, args.steps // 8), 2)
+        line["single_request"] = {"solves_per_s": round(len(r1) / s1, 3),
+                                  **{k: round(v, 3) for k, v in serve.latency_stats(r1).items()}}
+        del srv1
+        # the plain FP32 GPU port structured like mgpcg.py (the 1x baseline)
+        s = mgpcg.MGPCGPoissonSolver(3, [n] * 3, L, 2, 10, precision="plain", device=dev)
+        for i in range(2):  # warm-up
+            systems.assemble(s, scenes_dev[i % len(scenes_dev)])
+            s.solve_reference_structured(500, rel_tol=1e-12)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it_p = []
+        for i in range(args.baseline_steps):
+            systems.assemble(s, scenes_dev[i % len(scenes_dev)])
+            out = s.solve_reference_structured(500, rel_tol=1e-12)
+            assert out["status"] == _lib.CONVERGED, out
+            it_p.append(out["iters"])
+        torch.cuda.synchronize()
+        base = args.baseline_steps / (time.perf_counter() - t0)
+        line["plain_fp32_gpu_solves_per_s"] = round(base, 3)
+        line["plain_fp32_gpu_iters"] = sorted(set(it_p))
+        line["speedup_vs_plain_fp32_gpu"] = round(value / base, 3)
+        del s
+        # error of a mixed result against FP64 MGPCG converged to 1e-12 relative residual
+        sm = mgpcg.MGPCGPoissonSolver(3, [n] * 3, L, 2, 10, precision=args.precision,
+                                      device=dev)
+        systems.assemble(sm, scenes_dev[0])
+        sm.solve(500, rel_tol=1e-12)
+        xm = (sm.x64 if sm.x64 is not None else sm.x).double()
+        s64 = mgpcg.MGPCGPoissonSolver(3, [n] * 3, L, 2, 10, precision="f64", device=dev)
+        systems.assemble(s64, scenes_dev[0])
+        s64.solve(2000, rel_tol=1e-24)
+        x64 = s64.x64
+        err = float(torch.linalg.norm(xm - x64) / torch.linalg.norm(x64))
+        true_rel = (sm.last["true_rr"] / sm.last["r0r0"]) ** 0.5 if sm.last["true_rr"] >= 0 \
+            else None
+        line["error_vs_fp64"] = {"rel_l2": err, "fp64_iters": s64.last["iters"],
+                                 "fp64_rel_residual": (s64.last["rr"] / s64.last["r0r0"]) ** 0.5,
+                                 "true_rel_residual": true_rel}
+        del sm, s64
+        if not args.no_cpu:
+            line["cpu_baseline"] = cpu_baseline(n, L, args.cpu_seconds)
     print(json.dumps(line), flush=True)
+    np.random.seed(0)
 
 
 if __name__ == "__main__":

@@ -109,6 +109,15 @@ int mgpcg_solve(mgpcg_solver_t* solver, int max_iters, double rel_tol, double ab
 int mgpcg_solve_hostsync(mgpcg_solver_t* solver, int max_iters, double rel_tol, double abs_tol,
                          double eps, int refine_every, mgpcg_result_t* out);
 
+/* The plain FP32 GPU port of mgpcg.py structured like the reference: one
+ * kernel launch per reference kernel (fills, smooth, restrict, prolongate,
+ * compute_Ap, reduce, update_xr, update_p), alpha / beta / the residual test
+ * on the host, and a device->host read of every reduced scalar.  MGPCG_PLAIN
+ * handles only.  It is the baseline the batched solver is measured against;
+ * its x is bit-identical to mgpcg_solve's in MGPCG_PLAIN. */
+int mgpcg_solve_reference_structured(mgpcg_solver_t* solver, int max_iters, double rel_tol,
+                                     double abs_tol, double eps, mgpcg_result_t* out);
+
 /* Copies x (mgpcg.py:39) into a canonical array; exactly one of the two
  * pointers is non-NULL. */
 int mgpcg_store_x(mgpcg_solver_t* solver, float* x_f32, double* x_f64);
@@ -148,6 +157,16 @@ int mgpcg_server_poll(mgpcg_server_t* server, int64_t* tags, mgpcg_result_t* res
                       int* n_out);
 int mgpcg_server_pending(const mgpcg_server_t* server, int* n_pending, int* n_active,
                          int* inflight);
+
+/* Measurement (no reference counterpart): enable = 1 starts recording HIP
+ * events on the engine's stream around every launch of the level-0
+ * red-black half-sweep (the dominant kernel, mg::k_stage<..., 0, 1>);
+ * enable = 0 stops and returns the number of launches, their summed device
+ * time and the summed level-0 cells they swept. */
+int mgpcg_server_probe(mgpcg_server_t* server, int enable, int* n_launches, double* total_ms,
+                       int64_t* total_cells);
+int mgpcg_batch_probe(mgpcg_batch_t* batch, int enable, int* n_launches, double* total_ms,
+                      int64_t* total_cells);
 
 /* Kernel-level access for parity tests (no reference counterpart): move a
  * level vector between canonical layout (float, or double for MGPCG_F64 and

@@ -52,6 +52,8 @@ SIGNATURES = {
     "mgpcg_coarsen": (_I, [_P, _I, _P, _P, _P]),
     "mgpcg_solve": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(MgpcgResult)]),
     "mgpcg_solve_hostsync": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(MgpcgResult)]),
+    "mgpcg_solve_reference_structured": (_I, [_P, _I, _D, _D, _D,
+                                              ctypes.POINTER(MgpcgResult)]),
     "mgpcg_store_x": (_I, [_P, _P, _P]),
     "mgpcg_debug_set": (_I, [_P, _I, _I, _P]),
     "mgpcg_debug_get": (_I, [_P, _I, _I, _P]),
@@ -80,6 +82,10 @@ SIGNATURES = {
     "mgpcg_server_poll": (_I, [_P, _P, _P, _I, ctypes.POINTER(_I)]),
     "mgpcg_server_pending": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                   ctypes.POINTER(_I)]),
+    "mgpcg_server_probe": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_D),
+                                ctypes.POINTER(ctypes.c_int64)]),
+    "mgpcg_batch_probe": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_D),
+                               ctypes.POINTER(ctypes.c_int64)]),
     "mgpcg_last_error": (ctypes.c_char_p, []),
     "mgpcg_version": (_I, []),
 }

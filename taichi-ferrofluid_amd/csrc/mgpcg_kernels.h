@@ -375,7 +375,9 @@ enum StageOp {
   ST_PROLONGATE0 = 7,  // prolongate(l), colour 0 only
 };
 
-template <typename T, int DIM, int OP>
+// L0 only separates the level-0 instantiation (the dominant launches) from
+// the coarser levels' in profiles; the code is the same.
+template <typename T, int DIM, int OP, int L0>
 __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict__ descs,
                                                  Stage S) {
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
@@ -942,6 +944,130 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const SysDesc<T, TX>* __rest
         st->status = MAX_ITERS;
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The plain FP32 GPU port of mgpcg.py, structured like the reference: one
+// launch per reference kernel on one system, scalars on the host.  It is the
+// baseline the batched solver is measured against; each kernel handles both
+// colours of one slot per thread.
+// ---------------------------------------------------------------------------
+enum RefVec { RV_B = 0, RV_X = 1, RV_P = 2, RV_AP = 3, RV_R0 = 4, RV_Z0 = 5 };
+
+template <typename T, typename TX>
+__device__ __forceinline__ T* ref_vec(const SysDesc<T, TX>& D, int which, int c) {
+  switch (which) {
+    case RV_B: return D.b[c];
+    case RV_P: return D.p[0][c];
+    case RV_AP: return D.Ap[c];
+    case RV_R0: return D.P.r[c];
+    default: return D.P.z[c];
+  }
+}
+
+// r[0].copy_from(b) (mgpcg.py:157)
+template <typename T, typename TX>
+__global__ void __launch_bounds__(BLOCK) k_ref_copy_b(const SysDesc<T, TX>* __restrict__ sys) {
+  const SysDesc<T, TX>& D = sys[0];
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= D.g.size) return;
+  D.P.r[0][t] = D.b[0][t];
+  D.P.r[1][t] = D.b[1][t];
+}
+
+// reduce(a, b) (mgpcg.py:211-216): per-block partials of the FLUID-masked dot
+template <typename T, typename TX>
+__global__ void __launch_bounds__(BLOCK) k_ref_dot(const SysDesc<T, TX>* __restrict__ sys, int a,
+                                                   int b) {
+  const SysDesc<T, TX>& D = sys[0];
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  DD acc = dd_zero();
+  if (t < D.g.size) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (D.P.gt[c][t] == MG_FLUID)
+        acc = dd_add(acc, dd_prod(ref_vec(D, a, c)[t], ref_vec(D, b, c)[t]));
+  }
+  DD s = block_sum_dd<BLOCK>(acc);
+  if (threadIdx.x == 0) D.partials[blockIdx.x] = s;
+}
+
+// the scalar of the last k_ref_dot, rounded to FP32 as the 0-d f32 field `sum`
+template <typename T, typename TX>
+__global__ void __launch_bounds__(BLOCK) k_ref_sum(const SysDesc<T, TX>* __restrict__ sys,
+                                                   double* out) {
+  const SysDesc<T, TX>& D = sys[0];
+  DD v = dd_zero();
+  for (int i = threadIdx.x; i < D.nparts; i += BLOCK) v = dd_add(v, D.partials[i]);
+  const double tot = dd_value(block_sum_dd<BLOCK>(v));
+  if (threadIdx.x == 0) *out = (double)(float)tot;
+}
+
+// compute_Ap (mgpcg.py:218-224)
+template <typename T, typename TX, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_ref_compute_Ap(const SysDesc<T, TX>* __restrict__ sys) {
+  const SysDesc<T, TX>& D = sys[0];
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= D.g.size) return;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (D.P.gt[c][t] != MG_FLUID) continue;
+    const int q = row_q(D.g, t, c);
+    Nbr<DIM> n = neighbours<DIM>(D.g, t, q);
+    const T* po = D.p[0][c ^ 1];
+    T r = D.P.Adiag[c][t] * D.p[0][c][t];
+    T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return po[u]; });
+    D.Ap[c][t] = r + ns;
+  }
+}
+
+// update_xr (mgpcg.py:226-232) with alpha from the 0-d f32 field
+template <typename T, typename TX>
+__global__ void __launch_bounds__(BLOCK) k_ref_update_xr(const SysDesc<T, TX>* __restrict__ sys) {
+  const SysDesc<T, TX>& D = sys[0];
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= D.g.size) return;
+  const T alpha = (T)D.st->alpha32;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (D.P.gt[c][t] != MG_FLUID) continue;
+    T tp = alpha * D.p[0][c][t];
+    D.x[c][t] = (TX)((T)D.x[c][t] + tp);
+    T ta = alpha * D.Ap[c][t];
+    D.P.r[c][t] = D.P.r[c][t] - ta;
+  }
+}
+
+// update_p (mgpcg.py:234-238) with beta from the 0-d f32 field
+template <typename T, typename TX>
+__global__ void __launch_bounds__(BLOCK) k_ref_update_p(const SysDesc<T, TX>* __restrict__ sys) {
+  const SysDesc<T, TX>& D = sys[0];
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= D.g.size) return;
+  const T beta = (T)D.st->beta32;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (D.P.gt[c][t] != MG_FLUID) continue;
+    T tt = beta * D.p[0][c][t];
+    D.p[0][c][t] = D.P.z[c][t] + tt;
+  }
+}
+
+// r.fill(0); z.fill(0) of a level (mgpcg.py:126, 132-133)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_ref_fill(const LevelDesc<T>* __restrict__ descs,
+                                                    int desc, int which) {
+  const LevelDesc<T>& D = descs[desc];
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= D.g.size) return;
+  if (which & 1) {
+    D.P.z[0][t] = (T)0;
+    D.P.z[1][t] = (T)0;
+  }
+  if (which & 2) {
+    D.P.r[0][t] = (T)0;
+    D.P.r[1][t] = (T)0;
   }
 }
 

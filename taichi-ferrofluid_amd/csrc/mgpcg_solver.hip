@@ -202,6 +202,13 @@ struct Engine {
   char* h_desc = nullptr;
   CGState* h_stinit = nullptr;
 
+  // launch-duration probe of the level-0 half-sweeps (HIP events on `stream`)
+  bool probe_on = false;
+  std::vector<hipEvent_t> probe_ev;  // begin/end pairs
+  std::vector<long long> probe_cells;
+  int probe_used = 0;  // pairs recorded
+  long long probe_cells_now = 0;
+
   ~Engine() { release(); }
   void release() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -221,6 +228,8 @@ struct Engine {
       if (h_states[i]) (void)hipHostFree(h_states[i]);
       if (ev_states[i]) (void)hipEventDestroy(ev_states[i]);
     }
+    for (hipEvent_t ev : probe_ev) (void)hipEventDestroy(ev);
+    probe_ev.clear();
     if (ev_user) (void)hipEventDestroy(ev_user);
     if (ev_done) (void)hipEventDestroy(ev_done);
     if (stream) (void)hipStreamDestroy(stream);
@@ -452,6 +461,9 @@ int engine_build_items(Engine* e) {
   add_stage(e->st_bottom[0], [L](const Member& M) { return M.s->coarse_from >= L; },
             [L](const Member& M) { return nblk(M.s->geo[L - 1].size); },
             [L](int m) { return m * L + L - 1; });
+  e->probe_cells_now = 0;  // level-0 cells of one level-0 half-sweep launch
+  for (const Member& M : e->mem)
+    if (M.active && M.s->coarse_from > 0) e->probe_cells_now += M.s->geo[0].ncells;
   // coarse kernel: desc = first coarse level, block0 = number of coarse levels
   e->st_coarse.off = (int)it.size();
   for (int m = 0; m < (int)e->mem.size(); ++m) {
@@ -500,6 +512,26 @@ int engine_build_items(Engine* e) {
   return 0;
 }
 
+// Probe: an event pair around a launch; pairs beyond the pool are not recorded.
+const int PROBE_POOL = 8192;
+void probe_begin(Engine* e) {
+  if (e->probe_used >= PROBE_POOL) return;
+  if ((int)e->probe_ev.size() < 2 * (e->probe_used + 1)) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    e->probe_ev.push_back(a);
+    e->probe_ev.push_back(b);
+  }
+  (void)hipEventRecord(e->probe_ev[2 * e->probe_used], e->stream);
+}
+void probe_end(Engine* e) {
+  if (e->probe_used >= PROBE_POOL || (int)e->probe_ev.size() < 2 * (e->probe_used + 1)) return;
+  (void)hipEventRecord(e->probe_ev[2 * e->probe_used + 1], e->stream);
+  e->probe_cells.resize(e->probe_used + 1);
+  e->probe_cells[e->probe_used] = e->probe_cells_now;
+  e->probe_used++;
+}
+
 inline Stage stage_of(const Engine* e, const Engine::StageRef& r) {
   Stage s;
   s.items = e->d_items + r.off;
@@ -518,10 +550,17 @@ struct Sched {
   static const SysDesc<T, TX>* sys(const Engine* e) { return (const SysDesc<T, TX>*)e->d_sys; }
 
   template <int OP>
-  static void stage(Engine* e, const Engine::StageRef& r) {
+  static void stage(Engine* e, const Engine::StageRef& r, bool level0 = false) {
     if (r.nblocks == 0) return;
-    hipLaunchKernelGGL((k_stage<T, DIM, OP>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
-                       levels(e), stage_of(e, r));
+    const bool probe = level0 && OP == ST_SMOOTH0 && e->probe_on;
+    if (probe) probe_begin(e);
+    if (level0)
+      hipLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
+                         levels(e), stage_of(e, r));
+    else
+      hipLaunchKernelGGL((k_stage<T, DIM, OP, 0>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
+                         levels(e), stage_of(e, r));
+    if (probe) probe_end(e);
   }
   static void restrict_(Engine* e, const Engine::StageRef& r) {
     if (r.nblocks == 0) return;
@@ -542,17 +581,17 @@ struct Sched {
       const Engine::StageRef& S = e->st_slot[l];
       if (nu > 0) {
         if (g_fuse_first) {
-          stage<ST_PAIR_FROM_ZERO>(e, S);
+          stage<ST_PAIR_FROM_ZERO>(e, S, l == 0);
         } else {
-          stage<ST_FIRST>(e, S);
-          stage<ST_SMOOTH1>(e, S);
+          stage<ST_FIRST>(e, S, l == 0);
+          stage<ST_SMOOTH1>(e, S, l == 0);
         }
         for (int i = 1; i < nu; ++i) {
-          stage<ST_SMOOTH0>(e, S);
-          stage<ST_SMOOTH1>(e, S);
+          stage<ST_SMOOTH0>(e, S, l == 0);
+          stage<ST_SMOOTH1>(e, S, l == 0);
         }
       } else {
-        stage<ST_ZERO>(e, S);
+        stage<ST_ZERO>(e, S, l == 0);
       }
       restrict_(e, e->st_restrict[l]);
     }
@@ -582,18 +621,18 @@ struct Sched {
       const Engine::StageRef& S = e->st_slot[l];
       if (nu > 0) {
         if (g_fuse_prolong) {
-          stage<ST_PROLONG_SMOOTH1>(e, S);
+          stage<ST_PROLONG_SMOOTH1>(e, S, l == 0);
         } else {
-          stage<ST_PROLONGATE0>(e, S);
-          stage<ST_SMOOTH1>(e, S);
+          stage<ST_PROLONGATE0>(e, S, l == 0);
+          stage<ST_SMOOTH1>(e, S, l == 0);
         }
         for (int i = 0; i < nu; ++i) {
-          if (i > 0) stage<ST_SMOOTH1>(e, S);
+          if (i > 0) stage<ST_SMOOTH1>(e, S, l == 0);
           const bool last = (i == nu - 1);
-          if (!(last && l == 0)) stage<ST_SMOOTH0>(e, S);
+          if (!(last && l == 0)) stage<ST_SMOOTH0>(e, S, l == 0);
         }
       } else {
-        stage<ST_PROLONGATE>(e, S);
+        stage<ST_PROLONGATE>(e, S, l == 0);
       }
     }
     // level 0's last half-sweep fused with reduce(z, r), or a plain dot
@@ -1103,6 +1142,147 @@ extern "C" int mgpcg_store_x(mgpcg_solver_t* s, float* x_f32, double* x_f64) {
 }
 
 // ===========================================================================
+// The plain FP32 GPU port structured like mgpcg.py (the baseline)
+// ===========================================================================
+namespace {
+
+template <typename T, typename TX, int DIM>
+struct RefPath {
+  static int run(Engine* e, mgpcg_solver* s, int max_iters, double rel_tol, double abs_tol,
+                 double eps, mgpcg_result_t* out) {
+    using Sd = Sched<T, TX, DIM>;
+    const int L = s->L, nu = s->nu;
+    const SysDesc<T, TX>* sysd = Sd::sys(e);
+    const LevelDesc<T>* lv = Sd::levels(e);
+    const int nb0 = nblk(s->geo[0].size);
+    // one item per level: LevelDesc l of member 0
+    std::vector<Item> it(L);
+    for (int l = 0; l < L; ++l) it[l] = Item{l, 0};
+    MG_HIP(hipStreamSynchronize(e->stream));
+    MG_HIP(hipMemcpy(e->d_items, it.data(), sizeof(Item) * L, hipMemcpyHostToDevice));
+    e->dirty = true;
+    double* dsum = (double*)e->d_states;  // scratch for the reduced scalar
+    auto lvl = [&](int l, bool coarse) {
+      Engine::StageRef r;
+      r.off = l;
+      r.n = 1;
+      r.nblocks = coarse ? rblk(s->geo[l + 1].size) : nblk(s->geo[l].size);
+      return r;
+    };
+    auto fill = [&](int l, int which) {
+      hipLaunchKernelGGL((k_ref_fill<T>), dim3(nblk(s->geo[l].size)), dim3(BLOCK), 0, e->stream,
+                         lv, l, which);
+    };
+    // reduce(a, b) followed by the read of self.sum[None] (a host round trip)
+    auto reduce = [&](int a, int b, double* v) -> int {
+      hipLaunchKernelGGL((k_ref_dot<T, TX>), dim3(nb0), dim3(BLOCK), 0, e->stream, sysd, a, b);
+      hipLaunchKernelGGL((k_ref_sum<T, TX>), dim3(1), dim3(BLOCK), 0, e->stream, sysd, dsum);
+      MG_HIP(hipMemcpyAsync(v, dsum, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+      MG_HIP(hipStreamSynchronize(e->stream));
+      return 0;
+    };
+    auto write_f32 = [&](float* dst, float v) -> int {
+      MG_HIP(hipMemcpyAsync(dst, &v, sizeof(float), hipMemcpyHostToDevice, e->stream));
+      MG_HIP(hipStreamSynchronize(e->stream));
+      return 0;
+    };
+    auto vcycle = [&]() {
+      fill(0, 1);  // z[0].fill(0)
+      for (int l = 0; l + 1 < L; ++l) {
+        for (int i = 0; i < nu; ++i) {
+          Sd::template stage<ST_SMOOTH0>(e, lvl(l, false));
+          Sd::template stage<ST_SMOOTH1>(e, lvl(l, false));
+        }
+        fill(l + 1, 3);  // r[l+1].fill(0); z[l+1].fill(0)
+        Sd::restrict_(e, lvl(l, true));
+      }
+      const int half = s->bottom / 2;
+      for (int i = 0; i < half; ++i) {
+        Sd::template stage<ST_SMOOTH0>(e, lvl(L - 1, false));
+        Sd::template stage<ST_SMOOTH1>(e, lvl(L - 1, false));
+      }
+      for (int i = 0; i < half; ++i) {
+        Sd::template stage<ST_SMOOTH1>(e, lvl(L - 1, false));
+        Sd::template stage<ST_SMOOTH0>(e, lvl(L - 1, false));
+      }
+      for (int l = L - 2; l >= 0; --l) {
+        Sd::template stage<ST_PROLONGATE>(e, lvl(l, false));
+        for (int i = 0; i < nu; ++i) {
+          Sd::template stage<ST_SMOOTH1>(e, lvl(l, false));
+          Sd::template stage<ST_SMOOTH0>(e, lvl(l, false));
+        }
+      }
+    };
+    float* alpha32 = &s->st->alpha32;
+    float* beta32 = &s->st->beta32;
+    if (write_f32(beta32, 0.0f)) return -2;
+    hipLaunchKernelGGL((k_ref_copy_b<T, TX>), dim3(nb0), dim3(BLOCK), 0, e->stream, sysd);
+    double r0r0 = 0, old_zr = 0, pAp = 0, rr = 0, new_zr = 0;
+    if (reduce(RV_R0, RV_R0, &r0r0)) return -2;
+    const double tol = std::max(abs_tol, r0r0 * rel_tol);
+    vcycle();
+    hipLaunchKernelGGL((k_ref_update_p<T, TX>), dim3(nb0), dim3(BLOCK), 0, e->stream, sysd);
+    if (reduce(RV_Z0, RV_R0, &old_zr)) return -2;
+    int iter = 0, status = MAX_ITERS;
+    double alpha = 0, beta = 0;
+    while (max_iters == -1 || iter < max_iters) {
+      hipLaunchKernelGGL((k_ref_compute_Ap<T, TX, DIM>), dim3(nb0), dim3(BLOCK), 0, e->stream,
+                         sysd);
+      if (reduce(RV_P, RV_AP, &pAp)) return -2;
+      alpha = old_zr / (pAp + eps);
+      if (write_f32(alpha32, (float)alpha)) return -2;
+      hipLaunchKernelGGL((k_ref_update_xr<T, TX>), dim3(nb0), dim3(BLOCK), 0, e->stream, sysd);
+      if (reduce(RV_R0, RV_R0, &rr)) return -2;
+      if (rr < tol) {
+        status = CONVERGED;
+        break;
+      }
+      vcycle();
+      if (reduce(RV_Z0, RV_R0, &new_zr)) return -2;
+      beta = new_zr / (old_zr + eps);
+      if (write_f32(beta32, (float)beta)) return -2;
+      hipLaunchKernelGGL((k_ref_update_p<T, TX>), dim3(nb0), dim3(BLOCK), 0, e->stream, sysd);
+      old_zr = new_zr;
+      iter++;
+    }
+    if (out) {
+      out->iters = iter;
+      out->status = status;
+      out->r0r0 = r0r0;
+      out->rr = rr;
+      out->true_rr = -1.0;
+      out->tol = tol;
+      out->alpha = alpha;
+      out->beta = beta;
+    }
+    MG_HIP(hipGetLastError());
+    return 0;
+  }
+};
+
+}  // namespace
+
+extern "C" int mgpcg_solve_reference_structured(mgpcg_solver_t* s, int max_iters, double rel_tol,
+                                                double abs_tol, double eps,
+                                                mgpcg_result_t* out) {
+  MG_REQUIRE(s != nullptr, "solver is NULL");
+  MG_REQUIRE(s->precision == PLAIN, "the reference-structured path is the FP32 baseline");
+  Engine* e = nullptr;
+  if (single_engine(s, &e)) return -1;
+  if (engine_begin(e)) return -2;
+  e->mem.clear();
+  const SolveParams prm{max_iters, rel_tol, abs_tol, eps, 0};
+  if (engine_add(e, s, 0, prm) < 0) return -2;
+  if (engine_build_items(e)) return -2;  // allocates the item table
+  int rc;
+  if (s->dim == 3) rc = RefPath<float, float, 3>::run(e, s, max_iters, rel_tol, abs_tol, eps, out);
+  else rc = RefPath<float, float, 2>::run(e, s, max_iters, rel_tol, abs_tol, eps, out);
+  if (rc) return rc;
+  MG_HIP(hipStreamSynchronize(e->stream));
+  return engine_end(e);
+}
+
+// ===========================================================================
 // C-ABI: kernel-level access for parity tests
 // ===========================================================================
 static int vec_slots(mgpcg_solver* s, int field, int level, void** c0, void** c1, size_t* esz) {
@@ -1492,6 +1672,44 @@ extern "C" int mgpcg_server_pending(const mgpcg_server_t* sv, int* n_pending, in
   if (n_active) *n_active = engine_active(sv->e);
   if (inflight) *inflight = sv->inflight ? 1 : 0;
   return 0;
+}
+
+namespace {
+// enable: start a fresh measurement; disable: stop and report it
+int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells) {
+  if (enable) {
+    e->probe_on = true;
+    e->probe_used = 0;
+    e->probe_cells.clear();
+    return 0;
+  }
+  e->probe_on = false;
+  MG_HIP(hipStreamSynchronize(e->stream));
+  double tot = 0.0;
+  long long c = 0;
+  for (int i = 0; i < e->probe_used; ++i) {
+    float ms = 0.0f;
+    MG_HIP(hipEventElapsedTime(&ms, e->probe_ev[2 * i], e->probe_ev[2 * i + 1]));
+    tot += ms;
+    c += e->probe_cells[i];
+  }
+  if (n) *n = e->probe_used;
+  if (total_ms) *total_ms = tot;
+  if (cells) *cells = c;
+  return 0;
+}
+}  // namespace
+
+extern "C" int mgpcg_server_probe(mgpcg_server_t* sv, int enable, int* n_launches,
+                                  double* total_ms, int64_t* total_cells) {
+  MG_REQUIRE(sv != nullptr, "server is NULL");
+  return engine_probe(sv->e, enable, n_launches, total_ms, total_cells);
+}
+
+extern "C" int mgpcg_batch_probe(mgpcg_batch_t* b, int enable, int* n_launches,
+                                 double* total_ms, int64_t* total_cells) {
+  MG_REQUIRE(b != nullptr, "batch is NULL");
+  return engine_probe(b->e, enable, n_launches, total_ms, total_cells);
 }
 
 extern "C" int mgpcg_coarsen(void* stream, int dim, const int32_t* fine_res,

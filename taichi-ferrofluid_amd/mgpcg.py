@@ -163,6 +163,18 @@ class MGPCGPoissonSolver:
         self.last = out.as_dict()
         return self.last
 
+    def solve_reference_structured(self, max_iters=-1, rel_tol=1e-12, abs_tol=1e-14, eps=1e-12):
+        """The plain FP32 GPU port structured like mgpcg.py (one launch per
+        reference kernel, scalars on the host): the measured 1x baseline."""
+        self.load()
+        out = _lib.MgpcgResult()
+        _lib.check(self._lib.mgpcg_solve_reference_structured(
+            self._h, int(max_iters), float(rel_tol), float(abs_tol), float(eps),
+            ctypes.byref(out)))
+        self._store_x()
+        self.last = out.as_dict()
+        return self.last
+
     def _store_x(self):
         _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
         _lib.check(self._lib.mgpcg_store_x(self._h, _lib.ptr(self.x), None))

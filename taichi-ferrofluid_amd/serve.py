@@ -169,6 +169,14 @@ class Server:
             req["x"] = s.x.clone()
         self._release(s)
 
+    def probe(self, enable):
+        """Start (True) / stop (False) timing the level-0 half-sweep launches;
+        stopping returns (launches, device ms, level-0 cells swept)."""
+        n, ms, cells = ctypes.c_int(), ctypes.c_double(), ctypes.c_int64()
+        _lib.check(self._lib.mgpcg_server_probe(self._h, int(bool(enable)), ctypes.byref(n),
+                                                ctypes.byref(ms), ctypes.byref(cells)))
+        return n.value, ms.value, cells.value
+
     def busy(self):
         npend, nact, infl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _lib.check(self._lib.mgpcg_server_pending(self._h, ctypes.byref(npend),

@@ -76,6 +76,25 @@ def test_hostsync_loop_equals_device_loop(dev):
     assert np.array_equal(xa, xb)
 
 
+def test_reference_structured_equals_device_loop(dev):
+    """The plain baseline (one launch per reference kernel, scalars on the
+    host) computes exactly what the batched engine computes in PLAIN."""
+    import _lib
+    import mgpcg
+    import systems
+
+    sysd = SYSTEMS["c2_3d_small"](_emu(dev))
+    xa, outa, _ = gpu_solve(sysd, dev, "plain")
+    d = systems.upload(sysd, dev)
+    s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], sysd["nu"], sysd["bottom"],
+                                 precision="plain", device=dev)
+    systems.assemble(s, d)
+    outb = s.solve_reference_structured(500, rel_tol=1e-12)
+    _lib.synchronize(dev)
+    assert outb["iters"] == outa["iters"] and outb["status"] == outa["status"] == 1
+    assert np.array_equal(_x(s), xa)
+
+
 KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 1, "fuse_prolong": 1,
                  "fuse_p": 1}
 
