@@ -58,7 +58,26 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only (used for profiling runs)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a plain copy kernel over this many MiB (3 launches), the "
+                         "known-size reference for HBM byte counters")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "level0_smooth_c2_128_traffic.txt")
+
+
+def measured_traffic_per_cell():
+    """HBM bytes per swept cell of the level-0 half-sweep, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when it is absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_cell:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def pct(v, q):
@@ -102,7 +121,6 @@ def main():
               "shard is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import _lib
@@ -113,6 +131,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.copy_calibration_mib:
+        nel = args.copy_calibration_mib << 18
+        src = torch.ones(nel, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     n, L = args.n, args.levels
     host_scenes = [scenes.c2_3d(seed=1 + i, n=n, L=L) for i in range(args.seeds)]
     scenes_dev = [systems.upload(s, dev) for s in host_scenes]
@@ -144,12 +170,15 @@ def main():
         alg_bytes = SMOOTH_BYTES_PER_CELL_3D * cells_tot / nl
         launch_s = ms_tot * 1e-3 / nl
         achieved = alg_bytes / launch_s / 1e9
+        tpc = measured_traffic_per_cell()
+        traffic = round(tpc * cells_tot / nl) if tpc is not None else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": "mg::k_stage<float, 3, 0, 1> (smooth(0, 0), level-0 red-black "
                               "half-sweep over every member system)",
                     "alg_bytes_per_launch": round(alg_bytes), "launch_us": round(launch_s * 1e6, 3),
-                    "launches": nl}
+                    "launches": nl, "cells_per_launch": round(cells_tot / nl),
+                    "traffic_bytes_per_cell": tpc}
     del srv
 
     line = {
@@ -222,7 +251,6 @@ def main():
         if not args.no_cpu:
             line["cpu_baseline"] = cpu_baseline(n, L, args.cpu_seconds)
     print(json.dumps(line), flush=True)
-    np.random.seed(0)
 
 
 if __name__ == "__main__":

@@ -72,9 +72,9 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
                                         solved by one workgroup per system (default 4096);
                                         applies to handles created afterwards */
 #define MGPCG_OPT_MAX_GRID 4         /* workgroups of a persistent stage launch (default 2048) */
-#define MGPCG_OPT_FUSE_FIRST 5    /* 1: fill(0)+smooth(0)+smooth(1) in one kernel (default) */
-#define MGPCG_OPT_FUSE_PROLONG 6  /* 1: prolongate inside the next smooth(1) (default) */
-#define MGPCG_OPT_FUSE_P 7        /* 1: update_p inside compute_Ap (default) */
+#define MGPCG_OPT_FUSE_FIRST 5    /* 1: fill(0)+smooth(0)+smooth(1) in one kernel (default 0) */
+#define MGPCG_OPT_FUSE_PROLONG 6  /* 1: prolongate inside the next smooth(1) (default 0) */
+#define MGPCG_OPT_FUSE_P 7        /* 1: update_p inside compute_Ap (default 0) */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,

@@ -51,9 +51,9 @@ struct LevelBuf {
 
 int g_coarse_max_slots = 4096;  // levels at most this large go to k_coarse
 int g_max_grid = 2048;          // workgroups of a persistent stage launch
-int g_fuse_first = 1;    // first pre-smoothing pair in one kernel (else two)
-int g_fuse_prolong = 1;  // prolongation inside the first post-smoothing half-sweep
-int g_fuse_p = 1;        // update_p inside compute_Ap
+int g_fuse_first = 0;    // first pre-smoothing pair in one kernel (else two)
+int g_fuse_prolong = 0;  // prolongation inside the first post-smoothing half-sweep
+int g_fuse_p = 0;        // update_p inside compute_Ap
 
 }  // namespace
 

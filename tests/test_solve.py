@@ -95,8 +95,8 @@ def test_reference_structured_equals_device_loop(dev):
     assert np.array_equal(_x(s), xa)
 
 
-KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 1, "fuse_prolong": 1,
-                 "fuse_p": 1}
+KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 0, "fuse_prolong": 0,
+                 "fuse_p": 0}
 
 
 def set_knobs(**kw):
@@ -112,15 +112,15 @@ def set_knobs(**kw):
 
 @pytest.mark.parametrize("knobs", [
     {"coarse_max": 0}, {"coarse_max": 1 << 30}, {"max_grid": 8}, {"max_grid": 64},
-    {"fuse_first": 0}, {"fuse_prolong": 0}, {"fuse_p": 0},
-    {"fuse_first": 0, "fuse_prolong": 0, "fuse_p": 0},
+    {"fuse_first": 1}, {"fuse_prolong": 1}, {"fuse_p": 1},
+    {"fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1},
 ], ids=lambda k: "-".join(f"{a}{b}" for a, b in k.items()))
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
 def test_schedule_knobs_agree(dev, knobs, precision):
     """Where the coarse levels run, how many workgroups stride over the
     virtual blocks and which stages are fused must not change a bit of x."""
     if _emu(dev) and precision == "mixed" and len(knobs) < 3:
-        pytest.skip("on the emulator the all-unfused case covers MIXED")
+        pytest.skip("on the emulator the all-fused case covers MIXED")
     sysd = SYSTEMS["c2_3d_small"](_emu(dev))
     xa, outa, _ = gpu_solve(sysd, dev, precision)
     try:
