@@ -48,30 +48,59 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64, help="timed solve requests")
     ap.add_argument("--warmup", type=int, default=8, help="untimed solve requests")
-    ap.add_argument("--n", type=int, default=128, help="grid side of the workload")
-    ap.add_argument("--levels", type=int, default=5)
-    ap.add_argument("--seeds", type=int, default=4, help="distinct scenes cycled through")
-    ap.add_argument("--concurrency", type=int, default=8)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS),
+                    help="c2: 128^3 pool + sphere (the headline); c3: 256^3 pool + 8 spheres")
+    ap.add_argument("--n", type=int, default=None, help="grid side (default: the workload's)")
+    ap.add_argument("--levels", type=int, default=None)
+    ap.add_argument("--seeds", type=int, default=None, help="distinct scenes cycled through")
+    ap.add_argument("--concurrency", type=int, default=None, help="requests in flight")
     ap.add_argument("--precision", default="mixed", choices=["plain", "mixed", "f64"])
     ap.add_argument("--baseline-steps", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only (used for profiling runs)")
+    ap.add_argument("--probe-warmup", action="store_true",
+                    help="count the warm-up requests' level-0 sweeps too (counter runs, which "
+                         "attribute every dispatch of the process)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain copy kernel over this many MiB (3 launches), the "
                          "known-size reference for HBM byte counters")
-    return ap.parse_args()
+    a = ap.parse_args()
+    w = WORKLOADS[a.workload]
+    for k in ("n", "levels", "seeds", "concurrency"):
+        if getattr(a, k) is None:
+            setattr(a, k, w[k])
+    return a
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "level0_smooth_c2_128_traffic.txt")
+# Synthetic ghost-fluid pressure scenes (seeded, scenes.py): the 128^3 pool +
+# one sphere and the 256^3 pool + 8 spheres, theta >= 0.01, U[-1, 1] MAC
+# velocities.
+WORKLOADS = {
+    "c2": {"n": 128, "levels": 5, "seeds": 4, "concurrency": 8, "first_seed": 1,
+           "scene": "c2_3d", "what": "pool+sphere"},
+    "c3": {"n": 256, "levels": 6, "seeds": 2, "concurrency": 2, "first_seed": 2,
+           "scene": "c3_256", "what": "pool+8 spheres"},
+}
 
 
-def measured_traffic_per_cell():
+def make_scene(workload, i, n, levels):
+    import scenes
+
+    w = WORKLOADS[workload]
+    return getattr(scenes, w["scene"])(seed=w["first_seed"] + i, n=n, L=levels)
+
+
+def traffic_file(workload, n):
+    return os.path.join(ROOT, "profiles", f"level0_smooth_{workload}_{n}_traffic.txt")
+
+
+def measured_traffic_per_cell(workload, n):
     """HBM bytes per swept cell of the level-0 half-sweep, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when it is absent)."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(traffic_file(workload, n)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_cell:"):
                     return float(line.split(":", 1)[1])
@@ -90,14 +119,13 @@ def pct(v, q):
     return v[lo] + (v[hi] - v[lo]) * (k - lo)
 
 
-def cpu_baseline(n, levels, budget_s):
+def cpu_baseline(workload, n, levels, budget_s):
     """The oracle's C restatement of mgpcg.py (PLAIN) on every host core."""
-    import scenes
     from oracle import core
     from oracle import systems as osys
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    sysd = scenes.c2_3d(seed=1, n=n, L=levels)
+    sysd = make_scene(workload, 0, n, levels)
     h, b = osys.assemble(sysd)
     core.solve(h, b, max_iters=500, rel_tol=1e-12)  # warm-up
     t0 = time.perf_counter()
@@ -109,8 +137,52 @@ def cpu_baseline(n, levels, budget_s):
             break
     dt = time.perf_counter() - t0
     return {"value": round(count / dt, 3), "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{count} solves of the seed-1 {n}^3 system (solve only; assembly excluded),"
-                      f" {dt:.1f} s, PLAIN FP32, OpenMP threads={threads}"}
+            "sample": f"{count} solves of the {sysd['name']} system (seed {sysd['seed']}; solve "
+                      f"only, assembly excluded), {dt:.1f} s, PLAIN FP32, OpenMP "
+                      f"threads={threads}"}
+
+
+def measure_served(args, workload, n, L, seeds, concurrency, steps, warmup, dev):
+    """Closed-loop continuous batching of `steps` requests with `concurrency`
+    in flight; returns the throughput, latencies and the level-0 half-sweep
+    roofline measured over the timed requests."""
+    import _lib
+    import serve
+    import systems
+    import torch
+
+    scenes_dev = [systems.upload(make_scene(workload, i, n, L), dev) for i in range(seeds)]
+    torch.cuda.synchronize()
+    srv = serve.Server(64 << 30, precision=args.precision, n_levels=L, device=dev)
+    mk = (lambda i: scenes_dev[i % len(scenes_dev)])
+    if args.probe_warmup:
+        srv.probe(True)
+    serve.closed_loop(srv, mk, warmup, concurrency)  # allocate hierarchies, warm caches
+    if not args.probe_warmup:
+        srv.probe(True)
+    recs, secs = serve.closed_loop(srv, mk, steps, concurrency)
+    nl, ms_tot, cells_tot = srv.probe(False)
+    for r in recs:
+        assert r["result"]["status"] == _lib.CONVERGED, r["result"]
+    del srv
+    out = {"value": steps / secs, "secs": secs, "lat": serve.latency_stats(recs),
+           "iters": sorted(set(r["result"]["iters"] for r in recs)), "roofline": None,
+           "scenes": scenes_dev}
+    if nl > 0 and cells_tot > 0:
+        alg_bytes = SMOOTH_BYTES_PER_CELL_3D * cells_tot / nl
+        launch_s = ms_tot * 1e-3 / nl
+        achieved = alg_bytes / launch_s / 1e9
+        tpc = measured_traffic_per_cell(workload, n)
+        out["roofline"] = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(tpc * cells_tot / nl) if tpc is not None else None,
+            "kernel": "mg::k_stage<float, 3, 0, 1> (smooth(0, 0), level-0 red-black half-sweep "
+                      "over every member system)",
+            "alg_bytes_per_launch": round(alg_bytes), "launch_us": round(launch_s * 1e6, 3),
+            "launches": nl, "cells_per_launch": round(cells_tot / nl),
+            "traffic_bytes_per_cell": tpc}
+    return out
 
 
 def main():
@@ -125,8 +197,6 @@ def main():
 
     import _lib
     import mgpcg
-    import scenes
-    import serve
     import systems
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -139,79 +209,44 @@ def main():
             torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
-    n, L = args.n, args.levels
-    host_scenes = [scenes.c2_3d(seed=1 + i, n=n, L=L) for i in range(args.seeds)]
-    scenes_dev = [systems.upload(s, dev) for s in host_scenes]
-    torch.cuda.synchronize()
-    budget = 64 << 30
-
-    def run_served(concurrency, total, warmup, probe=False):
-        srv = serve.Server(budget, precision=args.precision, n_levels=L, device=dev)
-        mk = (lambda i: scenes_dev[i % len(scenes_dev)])
-        serve.closed_loop(srv, mk, warmup, concurrency)  # allocate hierarchies, warm caches
-        if probe:
-            srv.probe(True)
-        recs, secs = serve.closed_loop(srv, mk, total, concurrency)
-        pr = srv.probe(False) if probe else None
-        for r in recs:
-            assert r["result"]["status"] == _lib.CONVERGED, r["result"]
-        return recs, secs, pr, srv
+    wl, n, L = args.workload, args.n, args.levels
+    what = WORKLOADS[wl]["what"]
 
     # headline: continuous batching, closed loop
-    recs, secs, pr, srv = run_served(args.concurrency, args.steps, args.warmup, probe=True)
-    value = args.steps / secs
-    lat = serve.latency_stats(recs)
-    iters = sorted(set(r["result"]["iters"] for r in recs))
-
-    # roofline of the dominant kernel, from its launches inside the timed region
-    nl, ms_tot, cells_tot = pr
-    roofline = None
-    if nl > 0:
-        alg_bytes = SMOOTH_BYTES_PER_CELL_3D * cells_tot / nl
-        launch_s = ms_tot * 1e-3 / nl
-        achieved = alg_bytes / launch_s / 1e9
-        tpc = measured_traffic_per_cell()
-        traffic = round(tpc * cells_tot / nl) if tpc is not None else None
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "mg::k_stage<float, 3, 0, 1> (smooth(0, 0), level-0 red-black "
-                              "half-sweep over every member system)",
-                    "alg_bytes_per_launch": round(alg_bytes), "launch_us": round(launch_s * 1e6, 3),
-                    "launches": nl, "cells_per_launch": round(cells_tot / nl),
-                    "traffic_bytes_per_cell": tpc}
-    del srv
-
+    head = measure_served(args, wl, n, L, args.seeds, args.concurrency, args.steps, args.warmup,
+                          dev)
+    scenes_dev = head.pop("scenes")
     line = {
         "metric": "MGPCG solves/s + p50/p99 latency at 1e-6 rel. residual, 128^3-256^3, 1 GPU",
-        "value": round(value, 3),
+        "value": round(head["value"], 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * secs / args.steps, 3),
+        "ms_per_step": round(1e3 * head["secs"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"plain": "f32", "mixed": "f32+f64", "f64": "f64"}[args.precision],
         "data": "synthetic (seeded scenes, resident in HBM)",
-        "config": {"workload": f"3D {n}^3 ghost-fluid pool+sphere pressure solve requests, "
+        "config": {"workload": f"3D {n}^3 ghost-fluid {what} pressure solve requests, "
                                f"L={L}, 2+2 sweeps, bottom 10, rel residual 1e-6, "
                                f"{args.precision} precision, continuous batching with "
                                f"{args.concurrency} requests in flight",
                    "n": n, "levels": L, "seeds": args.seeds, "concurrency": args.concurrency},
-        "p50_ms": round(lat["p50_ms"], 3),
-        "p99_ms": round(lat["p99_ms"], 3),
-        "iters": iters,
-        "roofline": roofline,
+        "p50_ms": round(head["lat"]["p50_ms"], 3),
+        "p99_ms": round(head["lat"]["p99_ms"], 3),
+        "iters": head["iters"],
+        "roofline": head["roofline"],
         "cpu_baseline": None,
     }
 
     if not args.no_extras:
         # single request in flight: the latency-bound rate
-        r1, s1, _, srv1 = run_served(1, max(4, args.steps // 8), 2)
-        line["single_request"] = {"solves_per_s": round(len(r1) / s1, 3),
-                                  **{k: round(v, 3) for k, v in serve.latency_stats(r1).items()}}
-        del srv1
+        one = measure_served(args, wl, n, L, args.seeds, 1, max(4, args.steps // 8), 2, dev)
+        one.pop("scenes")
+        line["single_request"] = {"solves_per_s": round(one["value"], 3),
+                                  **{k: round(v, 3) for k, v in one["lat"].items()}}
         # the plain FP32 GPU port structured like mgpcg.py (the 1x baseline)
         s = mgpcg.MGPCGPoissonSolver(3, [n] * 3, L, 2, 10, precision="plain", device=dev)
         for i in range(2):  # warm-up
@@ -229,7 +264,7 @@ def main():
         base = args.baseline_steps / (time.perf_counter() - t0)
         line["plain_fp32_gpu_solves_per_s"] = round(base, 3)
         line["plain_fp32_gpu_iters"] = sorted(set(it_p))
-        line["speedup_vs_plain_fp32_gpu"] = round(value / base, 3)
+        line["speedup_vs_plain_fp32_gpu"] = round(head["value"] / base, 3)
         del s
         # error of a mixed result against FP64 MGPCG converged to 1e-12 relative residual
         sm = mgpcg.MGPCGPoissonSolver(3, [n] * 3, L, 2, 10, precision=args.precision,
@@ -247,9 +282,19 @@ def main():
         line["error_vs_fp64"] = {"rel_l2": err, "fp64_iters": s64.last["iters"],
                                  "fp64_rel_residual": (s64.last["rr"] / s64.last["r0r0"]) ** 0.5,
                                  "true_rel_residual": true_rel}
-        del sm, s64
+        del sm, s64, scenes_dev
+        if wl == "c2":
+            # the 256^3 workload, where a system's level 0 outgrows the Infinity Cache
+            c3 = WORKLOADS["c3"]
+            big = measure_served(args, "c3", c3["n"], c3["levels"], c3["seeds"],
+                                 c3["concurrency"], 8, 2, dev)
+            big.pop("scenes")
+            line["c3_256"] = {"solves_per_s": round(big["value"], 3),
+                              "p50_ms": round(big["lat"]["p50_ms"], 3),
+                              "p99_ms": round(big["lat"]["p99_ms"], 3), "iters": big["iters"],
+                              "concurrency": c3["concurrency"], "roofline": big["roofline"]}
         if not args.no_cpu:
-            line["cpu_baseline"] = cpu_baseline(n, L, args.cpu_seconds)
+            line["cpu_baseline"] = cpu_baseline(wl, n, L, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

@@ -34,7 +34,7 @@ extern "C" {
 #define MGPCG_ABI_VERSION 1
 
 /* precision modes */
-#define MGPCG_PLAIN 0 /* FP32 everywhere, FP32 scalars: the reference's arithmetic (G11) */
+#define MGPCG_PLAIN 0 /* FP32 everywhere, FP32 scalars: the reference's arithmetic (mgpcg.py:42-44) */
 #define MGPCG_MIXED 1 /* FP32 V-cycle/Laplacian, FP64 x, dots, scalars, true-residual refinement */
 #define MGPCG_F64 2   /* FP64 everywhere */
 

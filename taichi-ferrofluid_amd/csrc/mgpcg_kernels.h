@@ -71,6 +71,9 @@ struct Stage {
   const Item* items;
   int nitems;
   int nblocks;  // total virtual blocks
+  // measurement only: when non-NULL, a level-0 smooth(0) launch adds the
+  // level-0 cells of every system it sweeps (done systems are skipped)
+  unsigned long long* swept;
 };
 
 // ---------------------------------------------------------------------------
@@ -385,6 +388,10 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     const Item item = S.items[ii];
     const LevelDesc<T>& D = descs[item.desc];
     if (D.st->done) continue;
+    if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
+      if (S.swept != nullptr && it.vb == item.block0 && threadIdx.x == 0)
+        atomicAdd(S.swept, (unsigned long long)D.g.ncells);
+    }
     const int t = (it.vb - item.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
     if (OP == ST_SMOOTH0) smooth_slot<T, DIM>(D.g, D.P, 0, t);

@@ -205,9 +205,8 @@ struct Engine {
   // launch-duration probe of the level-0 half-sweeps (HIP events on `stream`)
   bool probe_on = false;
   std::vector<hipEvent_t> probe_ev;  // begin/end pairs
-  std::vector<long long> probe_cells;
   int probe_used = 0;  // pairs recorded
-  long long probe_cells_now = 0;
+  unsigned long long* d_swept = nullptr;  // level-0 cells swept by probed launches
 
   ~Engine() { release(); }
   void release() {
@@ -230,6 +229,8 @@ struct Engine {
     }
     for (hipEvent_t ev : probe_ev) (void)hipEventDestroy(ev);
     probe_ev.clear();
+    if (d_swept) (void)hipFree(d_swept);
+    d_swept = nullptr;
     if (ev_user) (void)hipEventDestroy(ev_user);
     if (ev_done) (void)hipEventDestroy(ev_done);
     if (stream) (void)hipStreamDestroy(stream);
@@ -461,9 +462,6 @@ int engine_build_items(Engine* e) {
   add_stage(e->st_bottom[0], [L](const Member& M) { return M.s->coarse_from >= L; },
             [L](const Member& M) { return nblk(M.s->geo[L - 1].size); },
             [L](int m) { return m * L + L - 1; });
-  e->probe_cells_now = 0;  // level-0 cells of one level-0 half-sweep launch
-  for (const Member& M : e->mem)
-    if (M.active && M.s->coarse_from > 0) e->probe_cells_now += M.s->geo[0].ncells;
   // coarse kernel: desc = first coarse level, block0 = number of coarse levels
   e->st_coarse.off = (int)it.size();
   for (int m = 0; m < (int)e->mem.size(); ++m) {
@@ -514,21 +512,26 @@ int engine_build_items(Engine* e) {
 
 // Probe: an event pair around a launch; pairs beyond the pool are not recorded.
 const int PROBE_POOL = 8192;
-void probe_begin(Engine* e) {
-  if (e->probe_used >= PROBE_POOL) return;
+// Returns whether the launch is recorded; a recorded launch also counts the
+// cells it sweeps into d_swept (on the device, so systems that finished on
+// the device but are not retired yet are not counted).
+bool probe_begin(Engine* e) {
+  if (e->probe_used >= PROBE_POOL || e->d_swept == nullptr) return false;
   if ((int)e->probe_ev.size() < 2 * (e->probe_used + 1)) {
     hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    if (hipEventCreate(&a) != hipSuccess) return false;
+    if (hipEventCreate(&b) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      return false;
+    }
     e->probe_ev.push_back(a);
     e->probe_ev.push_back(b);
   }
   (void)hipEventRecord(e->probe_ev[2 * e->probe_used], e->stream);
+  return true;
 }
 void probe_end(Engine* e) {
-  if (e->probe_used >= PROBE_POOL || (int)e->probe_ev.size() < 2 * (e->probe_used + 1)) return;
   (void)hipEventRecord(e->probe_ev[2 * e->probe_used + 1], e->stream);
-  e->probe_cells.resize(e->probe_used + 1);
-  e->probe_cells[e->probe_used] = e->probe_cells_now;
   e->probe_used++;
 }
 
@@ -537,6 +540,7 @@ inline Stage stage_of(const Engine* e, const Engine::StageRef& r) {
   s.items = e->d_items + r.off;
   s.nitems = r.n;
   s.nblocks = r.nblocks;
+  s.swept = nullptr;
   return s;
 }
 inline int grid_of(const Engine::StageRef& r) { return std::min(r.nblocks, g_max_grid); }
@@ -552,12 +556,13 @@ struct Sched {
   template <int OP>
   static void stage(Engine* e, const Engine::StageRef& r, bool level0 = false) {
     if (r.nblocks == 0) return;
-    const bool probe = level0 && OP == ST_SMOOTH0 && e->probe_on;
-    if (probe) probe_begin(e);
-    if (level0)
+    const bool probe = level0 && OP == ST_SMOOTH0 && e->probe_on && probe_begin(e);
+    if (level0) {
+      Stage st = stage_of(e, r);
+      if (probe) st.swept = e->d_swept;
       hipLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
-                         levels(e), stage_of(e, r));
-    else
+                         levels(e), st);
+    } else
       hipLaunchKernelGGL((k_stage<T, DIM, OP, 0>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
                          levels(e), stage_of(e, r));
     if (probe) probe_end(e);
@@ -1678,24 +1683,25 @@ namespace {
 // enable: start a fresh measurement; disable: stop and report it
 int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells) {
   if (enable) {
+    if (e->d_swept == nullptr) MG_HIP(hipMalloc(&e->d_swept, sizeof(unsigned long long)));
+    MG_HIP(hipMemsetAsync(e->d_swept, 0, sizeof(unsigned long long), e->stream));
     e->probe_on = true;
     e->probe_used = 0;
-    e->probe_cells.clear();
     return 0;
   }
   e->probe_on = false;
   MG_HIP(hipStreamSynchronize(e->stream));
   double tot = 0.0;
-  long long c = 0;
   for (int i = 0; i < e->probe_used; ++i) {
     float ms = 0.0f;
     MG_HIP(hipEventElapsedTime(&ms, e->probe_ev[2 * i], e->probe_ev[2 * i + 1]));
     tot += ms;
-    c += e->probe_cells[i];
   }
+  unsigned long long c = 0;
+  if (e->d_swept) MG_HIP(hipMemcpy(&c, e->d_swept, sizeof(c), hipMemcpyDeviceToHost));
   if (n) *n = e->probe_used;
   if (total_ms) *total_ms = tot;
-  if (cells) *cells = c;
+  if (cells) *cells = (int64_t)c;
   return 0;
 }
 }  // namespace

@@ -191,7 +191,7 @@ class MGPCGPoissonSolver:
         nbytes = ctypes.c_int64()
         _lib.check(self._lib.mgpcg_info(self._h, level, out, ctypes.byref(nbytes)))
         return {"P0": out[0], "P1": out[1], "W": out[2], "slots": out[3], "cells": out[4],
-                "partials": out[5], "device_bytes": nbytes.value}
+                "coarse_from": out[5], "device_bytes": nbytes.value}
 
     # -- kernel-level access (parity tests) --------------------------------------
     def _vec_dtype(self, field):

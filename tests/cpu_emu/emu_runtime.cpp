@@ -227,6 +227,10 @@ hipError_t hipMemset(void* p, int v, size_t bytes) {
   memset(p, v, bytes);
   return hipSuccess;
 }
+hipError_t hipMemsetAsync(void* p, int v, size_t bytes, hipStream_t) {
+  memset(p, v, bytes);
+  return hipSuccess;
+}
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind) {
   memmove(dst, src, bytes);
   return hipSuccess;

@@ -89,6 +89,12 @@ inline T __shfl(T v, int src, int width = 64) {
   return hipemu::shfl_idx(v, src, width);
 }
 inline void __threadfence() {}
+// the emulator runs every lane on one host thread, so a plain update is atomic
+inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
+  const unsigned long long old = *p;
+  *p = old + v;
+  return old;
+}
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline unsigned __umulhi(unsigned a, unsigned b) {
@@ -108,6 +114,7 @@ inline hipError_t hipMalloc(T** p, size_t bytes) {
 }
 hipError_t hipFree(void* p);
 hipError_t hipMemset(void* p, int v, size_t bytes);
+hipError_t hipMemsetAsync(void* p, int v, size_t bytes, hipStream_t s);
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind k);
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind k,
                           hipStream_t s);

@@ -209,15 +209,26 @@ def test_server_continuous_batching_equals_alone(dev):
     devs = [systems.upload(r, dev) for r in reqs]
     done = []
     i = 0
+    srv.probe(True)
     while len(done) < len(reqs):
         if i < len(reqs):  # one new request per step: staggered admission
             srv.submit(devs[i], meta=i)
             i += 1
         for rq in srv.step():
             rq["x"] = _x(rq["solver"])
+            rq["coarse_from"] = rq["solver"].info(0)["coarse_from"]
             srv.retire(rq)
             done.append(rq)
+    n_launch, ms, cells = srv.probe(False)
     assert srv.max_members >= 2  # they really overlapped
+    # the probe counts exactly the level-0 cells swept by smooth(0, 0): two
+    # launches per V-cycle (nu = 2), iters + 1 V-cycles per solve, none for
+    # members that finished on the device but were not yet retired, and none
+    # for systems small enough to run entirely in the coarse kernel
+    expect = sum((rq["result"]["iters"] + 1) * 2 * int(np.prod(reqs[rq["meta"]]["res"]))
+                 for rq in done if rq["coarse_from"] > 0)
+    assert cells == expect
+    assert n_launch > 0 and ms >= 0.0
     for rq in done:
         sysd = reqs[rq["meta"]]
         xa, ra, _ = gpu_solve(sysd, dev, "mixed")
