@@ -75,6 +75,9 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
 #define MGPCG_OPT_FUSE_FIRST 5    /* 1: fill(0)+smooth(0)+smooth(1) in one kernel (default 0) */
 #define MGPCG_OPT_FUSE_PROLONG 6  /* 1: prolongate inside the next smooth(1) (default 0) */
 #define MGPCG_OPT_FUSE_P 7        /* 1: update_p inside compute_Ap (default 0) */
+#define MGPCG_OPT_COMPACT 8       /* 1: levels whose nonzero off-diagonals are all one value
+                                     read per-cell coupling flags instead of Ax (default 1);
+                                     applies to systems solved or submitted afterwards */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,
@@ -88,6 +91,10 @@ int mgpcg_info(const mgpcg_solver_t* solver, int level, int32_t* out6, int64_t* 
  * full_reinitialize (mgpcg.py:79-94). */
 int mgpcg_load_level(mgpcg_solver_t* solver, int level, const int8_t* grid_type,
                      const float* Adiag, const float* Ax);
+/* Whether every nonzero off-diagonal coefficient of a loaded level has one
+ * value (then *coef is it, or 0 when the level has no coupling); waits for
+ * the handle's stream.  No reference counterpart: inspection only. */
+int mgpcg_level_coupling(mgpcg_solver_t* solver, int level, int* uniform, float* coef);
 /* Loads b (the strategy's build_b output, mgpcg.py:20). */
 int mgpcg_load_rhs(mgpcg_solver_t* solver, const float* b);
 

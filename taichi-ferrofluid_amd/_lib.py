@@ -17,7 +17,8 @@ RUNNING, CONVERGED, MAX_ITERS, BREAKDOWN = 0, 1, 2, 3
 FIELD_R, FIELD_Z, FIELD_P, FIELD_AP, FIELD_X, FIELD_P_NEXT = 0, 1, 2, 3, 4, 5
 (OP_SMOOTH, OP_SMOOTH_PAIR_FROM_ZERO, OP_RESTRICT, OP_PROLONGATE, OP_VCYCLE, OP_COMPUTE_AP,
  OP_PROLONG_SMOOTH1) = range(7)
-OPT_COARSE_MAX_SLOTS, OPT_MAX_GRID, OPT_FUSE_FIRST, OPT_FUSE_PROLONG, OPT_FUSE_P = 3, 4, 5, 6, 7
+(OPT_COARSE_MAX_SLOTS, OPT_MAX_GRID, OPT_FUSE_FIRST, OPT_FUSE_PROLONG, OPT_FUSE_P,
+ OPT_COMPACT) = 3, 4, 5, 6, 7, 8
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -49,6 +50,7 @@ SIGNATURES = {
     "mgpcg_info": (_I, [_P, _I, _P, _P]),
     "mgpcg_load_level": (_I, [_P, _I, _P, _P, _P]),
     "mgpcg_load_rhs": (_I, [_P, _P]),
+    "mgpcg_level_coupling": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float)]),
     "mgpcg_coarsen": (_I, [_P, _I, _P, _P, _P]),
     "mgpcg_solve": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(MgpcgResult)]),
     "mgpcg_solve_hostsync": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(MgpcgResult)]),

@@ -20,6 +20,13 @@
 //    z stays unprolonged; it is inert (every coupling to it is 0).
 //  * update_p (mgpcg.py:234-238) is fused into compute_Ap with p double-
 //    buffered by iteration parity; reduce(z, r) into the last half-sweep.
+//
+// Compact couplings: when every nonzero off-diagonal coefficient of a level
+// has the same FP32 bit pattern (the pressure-type operators: -scale_A on
+// every FLUID-FLUID face, pressure_project.py:55-57), the stencil reads a
+// per-slot flag byte instead of the 2 * dim coefficients and uses
+// (flag ? coef : 0).  That value is bit-identical to the stored coefficient,
+// and the same multiply and add follow, so the result does not change.
 #pragma once
 #include "mgpcg_common.h"
 
@@ -28,6 +35,17 @@ namespace mg {
 constexpr int BLOCK = 256;
 constexpr int COARSE_THREADS = 1024;
 
+// Nonzero-coupling summary of a level, accumulated on the device while the
+// level is packed: the smallest and largest FP32 bit pattern of its nonzero
+// Ax values (minbits = ~0, maxbits = 0 when there is none).
+struct LevelMeta {
+  uint32_t minbits, maxbits;
+};
+
+// flag byte of a slot: bit a = coupling to the upper neighbour on axis a is
+// nonzero (Ax[I][a]), bit NB_DN + a = coupling to the lower one (Ax[I - e_a][a])
+constexpr int NB_DN = 3;
+
 template <typename T>
 struct LevelPtrs {
   const int8_t* gt[2];
@@ -35,7 +53,30 @@ struct LevelPtrs {
   const T* Ax[3][2];  // [axis][colour]
   T* r[2];
   T* z[2];
+  const uint8_t* nb[2];    // coupling flags per slot
+  const LevelMeta* meta;   // NULL: always read Ax
 };
+
+// The level's single coupling value when its couplings are uniform.
+template <typename T>
+struct Coupling {
+  bool uniform;
+  T coef;
+};
+template <typename T>
+__device__ __forceinline__ Coupling<T> coupling(const LevelPtrs<T>& P) {
+  Coupling<T> k{false, (T)0};
+  if (P.meta != nullptr) {
+    const uint32_t mn = P.meta->minbits, mx = P.meta->maxbits;
+    if (mx == 0u) {
+      k.uniform = true;  // no nonzero coupling at all: every flag is 0
+    } else if (mn == mx) {
+      k.uniform = true;
+      k.coef = (T)__uint_as_float(mn);
+    }
+  }
+  return k;
+}
 
 // One level of one system.
 template <typename T>
@@ -204,6 +245,20 @@ template <typename T, int DIM, typename ZO>
 __device__ __forceinline__ T nsum(const LevelPtrs<T>& P, int c, int t, const Nbr<DIM>& n,
                                   ZO zo) {
   T ret = (T)0;
+  const Coupling<T> k = coupling(P);
+  if (k.uniform) {
+    const uint32_t f = P.nb[c][t];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      const T alo = ((f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
+      const T ahi = ((f >> a) & 1u) ? k.coef : (T)0;
+      T lo = alo * zo(n.lo[a]);
+      T hi = ahi * zo(n.hi[a]);
+      T s = lo + hi;
+      ret = ret + s;
+    }
+    return ret;
+  }
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     T lo = P.Ax[a][c ^ 1][n.lo[a]] * zo(n.lo[a]);
@@ -284,10 +339,14 @@ __device__ __forceinline__ void prolong_smooth1_slot(const Geo& g, const Geo& gc
     return z0[u] + zcv;
   };
   T ret = (T)0;
+  const Coupling<T> k = coupling(P);
+  const uint32_t f = k.uniform ? (uint32_t)P.nb[1][t] : 0u;
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    T lo = P.Ax[a][0][n.lo[a]] * zp(a, -1, n.lo[a]);
-    T hi = P.Ax[a][1][t] * zp(a, +1, n.hi[a]);
+    const T alo = k.uniform ? (((f >> (NB_DN + a)) & 1u) ? k.coef : (T)0) : P.Ax[a][0][n.lo[a]];
+    const T ahi = k.uniform ? (((f >> a) & 1u) ? k.coef : (T)0) : P.Ax[a][1][t];
+    T lo = alo * zp(a, -1, n.lo[a]);
+    T hi = ahi * zp(a, +1, n.hi[a]);
     T sm = lo + hi;
     ret = ret + sm;
   }
@@ -1085,29 +1144,95 @@ __device__ __forceinline__ long canon_index(const Geo& g, int i, int j, int k) {
   return ((long)i * g.n1 + j) * g.n2 + k;
 }
 
+__global__ void k_meta_init(LevelMeta* m) {
+  if (threadIdx.x == 0) {
+    m->minbits = 0xffffffffu;
+    m->maxbits = 0u;
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Packs one level from the canonical arrays, computes every slot's coupling
+// flags, and folds the level's nonzero coupling bit patterns into *meta.
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __restrict__ gt,
                                                       const float* __restrict__ Adiag,
                                                       const float* __restrict__ Ax,
-                                                      LevelPtrs<T> P) {
+                                                      LevelPtrs<T> P, LevelMeta* meta) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= g.size) return;
+  uint32_t mn = 0xffffffffu, mx = 0u;
+  if (t < g.size) {
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    Slot s = decode(g, t, c);
-    int8_t gv = MG_SOLID;
-    T ad = (T)0, a[3] = {(T)0, (T)0, (T)0};
-    if (s.interior) {
-      const long ci = canon_index(g, s.i, s.j, s.k);
-      gv = gt[ci];
-      ad = (T)Adiag[ci];
-      for (int k = 0; k < g.dim; ++k) a[k] = (T)Ax[ci * g.dim + k];
+    for (int c = 0; c < 2; ++c) {
+      Slot s = decode(g, t, c);
+      int8_t gv = MG_SOLID;
+      T ad = (T)0, a[3] = {(T)0, (T)0, (T)0};
+      uint32_t f = 0u;
+      if (s.interior) {
+        const long ci = canon_index(g, s.i, s.j, s.k);
+        gv = gt[ci];
+        ad = (T)Adiag[ci];
+        const int v[3] = {s.i, s.j, s.k};
+        for (int k = 0; k < g.dim; ++k) {
+          const float up = Ax[ci * g.dim + k];
+          a[k] = (T)up;
+          const uint32_t ub = __float_as_uint(up);
+          if (ub != 0u) {
+            f |= 1u << k;
+            mn = ub < mn ? ub : mn;
+            mx = ub > mx ? ub : mx;
+          }
+          // lower neighbour on original axis k (view axis k + 3 - dim)
+          const int va = k + 3 - g.dim;
+          if (v[va] > 0) {
+            int w[3] = {s.i, s.j, s.k};
+            w[va] -= 1;
+            const float dn = Ax[canon_index(g, w[0], w[1], w[2]) * g.dim + k];
+            if (__float_as_uint(dn) != 0u) f |= 1u << (NB_DN + k);
+          }
+        }
+      }
+      ((int8_t*)P.gt[c])[t] = gv;
+      ((T*)P.Adiag[c])[t] = ad;
+      for (int k = 0; k < g.dim; ++k) ((T*)P.Ax[k][c])[t] = a[k];
+      ((uint8_t*)P.nb[c])[t] = (uint8_t)f;
+      P.r[c][t] = (T)0;
+      P.z[c][t] = (T)0;
     }
-    ((int8_t*)P.gt[c])[t] = gv;
-    ((T*)P.Adiag[c])[t] = ad;
-    for (int k = 0; k < g.dim; ++k) ((T*)P.Ax[k][c])[t] = a[k];
-    P.r[c][t] = (T)0;
-    P.z[c][t] = (T)0;
+  }
+  __shared__ uint32_t smn[BLOCK / 64], smx[BLOCK / 64];
+  mn = wave_min_u32(mn);
+  mx = wave_max_u32(mx);
+  if ((threadIdx.x & 63) == 0) {
+    smn[threadIdx.x >> 6] = mn;
+    smx[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      mn = smn[w] < mn ? smn[w] : mn;
+      mx = smx[w] > mx ? smx[w] : mx;
+    }
+    if (mx != 0u) {
+      atomicMin(&meta->minbits, mn);
+      atomicMax(&meta->maxbits, mx);
+    }
   }
 }
 

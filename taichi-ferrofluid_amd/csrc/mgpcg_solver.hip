@@ -47,6 +47,8 @@ struct LevelBuf {
   void* Ax[3][2];
   void* r[2];
   void* z[2];
+  uint8_t* nb[2];    // coupling flags
+  LevelMeta* meta;   // nonzero-coupling summary, written when the level is loaded
 };
 
 int g_coarse_max_slots = 4096;  // levels at most this large go to k_coarse
@@ -54,6 +56,7 @@ int g_max_grid = 2048;          // workgroups of a persistent stage launch
 int g_fuse_first = 0;    // first pre-smoothing pair in one kernel (else two)
 int g_fuse_prolong = 0;  // prolongation inside the first post-smoothing half-sweep
 int g_fuse_p = 0;        // update_p inside compute_Ap
+int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
 
 }  // namespace
 
@@ -78,6 +81,7 @@ struct mgpcg_solver {
   hipStream_t user_stream = nullptr;
   Engine* engine = nullptr;  // private engine of the single-system API
   int device = 0;
+  bool full_coupling = false;  // descriptors written now read every Ax
 };
 
 namespace {
@@ -98,7 +102,9 @@ LevelPtrs<T> level_ptrs(const mgpcg_solver* s, int l) {
     P.r[c] = (T*)B.r[c];
     P.z[c] = (T*)B.z[c];
     for (int a = 0; a < 3; ++a) P.Ax[a][c] = (const T*)B.Ax[a][c];
+    P.nb[c] = B.nb[c];
   }
+  P.meta = (g_compact && !s->full_coupling) ? B.meta : nullptr;
   return P;
 }
 
@@ -933,7 +939,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     return o;
   };
   struct Offs {
-    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2];
+    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], meta;
   };
   std::vector<Offs> lo;
   s->coarse_from = n_levels;
@@ -950,7 +956,9 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       for (int a = 0; a < 3; ++a) o.Ax[a][c] = (a < dim) ? take(g.size * s->esz) : (size_t)-1;
       o.r[c] = take(g.size * s->esz);
       o.z[c] = take(g.size * s->esz);
+      o.nb[c] = take(g.size);
     }
+    o.meta = take(sizeof(LevelMeta));
     lo.push_back(o);
   }
   const Geo& g0 = s->geo[0];
@@ -983,7 +991,9 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
         B.Ax[a][c] = (a < dim) ? (void*)(s->pool + lo[l].Ax[a][c]) : nullptr;
       B.r[c] = s->pool + lo[l].r[c];
       B.z[c] = s->pool + lo[l].z[c];
+      B.nb[c] = (uint8_t*)(s->pool + lo[l].nb[c]);
     }
+    B.meta = (LevelMeta*)(s->pool + lo[l].meta);
     s->lv.push_back(B);
     const Geo& g = s->geo[l];
     // ghost cells are SOLID (the zero fill made them FLUID)
@@ -1044,6 +1054,9 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
     case MGPCG_OPT_FUSE_P:
       g_fuse_p = value != 0;
       return 0;
+    case MGPCG_OPT_COMPACT:
+      g_compact = value != 0;
+      return 0;
     default:
       (void)s;
       mg_set_error("unknown option %d", option);
@@ -1082,13 +1095,35 @@ extern "C" int mgpcg_load_level(mgpcg_solver_t* s, int level, const int8_t* grid
   MG_REQUIRE(grid_type && Adiag && Ax, "NULL field pointer");
   MG_HIP(hipSetDevice(s->device));
   const Geo& g = s->geo[level];
+  LevelMeta* meta = s->lv[level].meta;
+  hipLaunchKernelGGL(k_meta_init, dim3(1), dim3(64), 0, s->user_stream, meta);
   if (s->precision == F64)
     hipLaunchKernelGGL((k_pack_level<double>), dim3(nblk(g.size)), dim3(BLOCK), 0,
-                       s->user_stream, g, grid_type, Adiag, Ax, level_ptrs<double>(s, level));
+                       s->user_stream, g, grid_type, Adiag, Ax, level_ptrs<double>(s, level),
+                       meta);
   else
     hipLaunchKernelGGL((k_pack_level<float>), dim3(nblk(g.size)), dim3(BLOCK), 0,
-                       s->user_stream, g, grid_type, Adiag, Ax, level_ptrs<float>(s, level));
+                       s->user_stream, g, grid_type, Adiag, Ax, level_ptrs<float>(s, level),
+                       meta);
   MG_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int mgpcg_level_coupling(mgpcg_solver_t* s, int level, int* uniform, float* coef) {
+  MG_REQUIRE(s != nullptr, "solver is NULL");
+  MG_REQUIRE(level >= 0 && level < s->L, "level %d out of range [0, %d)", level, s->L);
+  MG_HIP(hipSetDevice(s->device));
+  MG_HIP(hipStreamSynchronize(s->user_stream));
+  LevelMeta m;
+  MG_HIP(hipMemcpy(&m, s->lv[level].meta, sizeof(m), hipMemcpyDeviceToHost));
+  const bool uni = m.maxbits == 0u || m.minbits == m.maxbits;
+  if (uniform) *uniform = uni ? 1 : 0;
+  if (coef) {
+    uint32_t bits = (uni && m.maxbits != 0u) ? m.minbits : 0u;
+    float f;
+    memcpy(&f, &bits, sizeof f);
+    *coef = f;
+  }
   return 0;
 }
 
@@ -1277,7 +1312,11 @@ extern "C" int mgpcg_solve_reference_structured(mgpcg_solver_t* s, int max_iters
   if (engine_begin(e)) return -2;
   e->mem.clear();
   const SolveParams prm{max_iters, rel_tol, abs_tol, eps, 0};
-  if (engine_add(e, s, 0, prm) < 0) return -2;
+  // the baseline reads every coefficient, as mgpcg.py does
+  s->full_coupling = true;
+  const int added = engine_add(e, s, 0, prm);
+  s->full_coupling = false;
+  if (added < 0) return -2;
   if (engine_build_items(e)) return -2;  // allocates the item table
   int rc;
   if (s->dim == 3) rc = RefPath<float, float, 3>::run(e, s, max_iters, rel_tol, abs_tol, eps, out);

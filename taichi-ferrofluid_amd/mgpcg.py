@@ -193,6 +193,13 @@ class MGPCGPoissonSolver:
         return {"P0": out[0], "P1": out[1], "W": out[2], "slots": out[3], "cells": out[4],
                 "coarse_from": out[5], "device_bytes": nbytes.value}
 
+    def level_coupling(self, level):
+        """(uniform, coef) of a loaded level: whether every nonzero off-diagonal
+        coefficient has one value, which the kernels then use with per-cell flags."""
+        u, c = ctypes.c_int(), ctypes.c_float()
+        _lib.check(self._lib.mgpcg_level_coupling(self._h, level, ctypes.byref(u), ctypes.byref(c)))
+        return bool(u.value), c.value
+
     # -- kernel-level access (parity tests) --------------------------------------
     def _vec_dtype(self, field):
         if self.precision == "f64" or (field == _lib.FIELD_X and self.precision != "plain"):

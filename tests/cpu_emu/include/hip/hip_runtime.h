@@ -95,6 +95,26 @@ inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v)
   *p = old + v;
   return old;
 }
+inline unsigned atomicMin(unsigned* p, unsigned v) {
+  const unsigned old = *p;
+  *p = v < old ? v : old;
+  return old;
+}
+inline unsigned atomicMax(unsigned* p, unsigned v) {
+  const unsigned old = *p;
+  *p = v > old ? v : old;
+  return old;
+}
+inline unsigned __float_as_uint(float f) {
+  unsigned u;
+  memcpy(&u, &f, sizeof u);
+  return u;
+}
+inline float __uint_as_float(unsigned u) {
+  float f;
+  memcpy(&f, &u, sizeof f);
+  return f;
+}
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline unsigned __umulhi(unsigned a, unsigned b) {

@@ -96,7 +96,7 @@ def test_reference_structured_equals_device_loop(dev):
 
 
 KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 0, "fuse_prolong": 0,
-                 "fuse_p": 0}
+                 "fuse_p": 0, "compact": 1}
 
 
 def set_knobs(**kw):
@@ -105,7 +105,7 @@ def set_knobs(**kw):
     lib = _lib.load()
     opt = {"coarse_max": _lib.OPT_COARSE_MAX_SLOTS, "max_grid": _lib.OPT_MAX_GRID,
            "fuse_first": _lib.OPT_FUSE_FIRST, "fuse_prolong": _lib.OPT_FUSE_PROLONG,
-           "fuse_p": _lib.OPT_FUSE_P}
+           "fuse_p": _lib.OPT_FUSE_P, "compact": _lib.OPT_COMPACT}
     for k, v in kw.items():
         _lib.check(lib.mgpcg_set_option(None, opt[k], v))
 
@@ -114,6 +114,7 @@ def set_knobs(**kw):
     {"coarse_max": 0}, {"coarse_max": 1 << 30}, {"max_grid": 8}, {"max_grid": 64},
     {"fuse_first": 1}, {"fuse_prolong": 1}, {"fuse_p": 1},
     {"fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1},
+    {"compact": 0}, {"compact": 0, "fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1},
 ], ids=lambda k: "-".join(f"{a}{b}" for a, b in k.items()))
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
 def test_schedule_knobs_agree(dev, knobs, precision):
@@ -167,6 +168,27 @@ def _assembled(sysd, dev, precision):
                                  precision=precision, device=dev)
     systems.assemble(s, d)
     return s
+
+
+@pytest.mark.parametrize("kind", ["pressure", "volume", "psi", "surface"])
+def test_level_coupling_detection(dev, kind):
+    """Pressure-type operators couple every FLUID pair with -scale_A at every
+    level (pressure_project.py:55-57), so every level is detected uniform
+    with that value; the potential and surface-tension couplings vary."""
+    emu = _emu(dev)
+    res = [16, 16, 32] if emu else [32, 48, 64]
+    sysd = scenes.pool_sphere_system(res, 7, L=3, kind=kind)
+    s = _assembled(sysd, dev, "plain")
+    s.load()
+    for l in range(sysd["L"]):
+        canon = s.Ax[l].cpu().numpy()
+        vals = np.unique(canon[canon.view(np.uint32) != 0])
+        uni, coef = s.level_coupling(l)
+        assert uni == (len(vals) <= 1), (l, vals[:4])
+        if kind in ("pressure", "volume"):
+            assert uni and coef == np.float32(-sysd["scale_A"])
+        elif l == 0:
+            assert not uni
 
 
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
