@@ -78,6 +78,8 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
 #define MGPCG_OPT_COMPACT 8       /* 1: levels whose nonzero off-diagonals are all one value
                                      read per-cell coupling flags instead of Ax (default 1);
                                      applies to systems solved or submitted afterwards */
+#define MGPCG_OPT_COARSE_LDS 9    /* 1: the coarse-level kernel keeps z in LDS when a system's
+                                     coarse levels fit (FP32 hierarchies; default 1) */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,

@@ -281,27 +281,36 @@ __device__ __forceinline__ int parent_slot(const Geo& gf, const Geo& gc, int t, 
 // ---------------------------------------------------------------------------
 // per-slot stage bodies (shared by the batched kernels and the coarse kernel)
 // ---------------------------------------------------------------------------
+// The slot bodies take the level's z colour arrays explicitly (z[0], z[1]):
+// the batched kernels pass the level's own arrays, the coarse kernel copies
+// held in LDS.
+
 // smooth(l, c) (mgpcg.py:104-109) at slot t of colour c.
 template <typename T, int DIM>
-__device__ __forceinline__ void smooth_slot(const Geo& g, const LevelPtrs<T>& P, int c, int t) {
+__device__ __forceinline__ void smooth_slot(const Geo& g, const LevelPtrs<T>& P, T* const* z,
+                                            int c, int t) {
   if (P.gt[c][t] != MG_FLUID) return;
   const int q = row_q(g, t, c);
   Nbr<DIM> n = neighbours<DIM>(g, t, q);
-  const T* zo = P.z[c ^ 1];
+  const T* zo = z[c ^ 1];
   T s = nsum<T, DIM>(P, c, t, n, [&](int u) { return zo[u]; });
   T num = P.r[c][t] - s;
-  P.z[c][t] = num / P.Adiag[c][t];
+  z[c][t] = num / P.Adiag[c][t];
+}
+template <typename T, int DIM>
+__device__ __forceinline__ void smooth_slot(const Geo& g, const LevelPtrs<T>& P, int c, int t) {
+  smooth_slot<T, DIM>(g, P, P.z, c, t);
 }
 
 // z.fill(0); smooth(l, 0); smooth(l, 1) (mgpcg.py:126-130 / 132-134) at slot t:
 // colour 0 from zero is r / Adiag, recomputed for every colour-1 stencil.
 template <typename T, int DIM>
 __device__ __forceinline__ void smooth_pair_from_zero_slot(const Geo& g, const LevelPtrs<T>& P,
-                                                           int t) {
+                                                           T* const* z, int t) {
   const bool f0 = P.gt[0][t] == MG_FLUID;
-  P.z[0][t] = f0 ? P.r[0][t] / P.Adiag[0][t] : (T)0;
+  z[0][t] = f0 ? P.r[0][t] / P.Adiag[0][t] : (T)0;
   if (P.gt[1][t] != MG_FLUID) {
-    P.z[1][t] = (T)0;
+    z[1][t] = (T)0;
     return;
   }
   const int q = row_q(g, t, 1);
@@ -311,19 +320,24 @@ __device__ __forceinline__ void smooth_pair_from_zero_slot(const Geo& g, const L
   };
   T s = nsum<T, DIM>(P, 1, t, n, z0);
   T num = P.r[1][t] - s;
-  P.z[1][t] = num / P.Adiag[1][t];
+  z[1][t] = num / P.Adiag[1][t];
+}
+template <typename T, int DIM>
+__device__ __forceinline__ void smooth_pair_from_zero_slot(const Geo& g, const LevelPtrs<T>& P,
+                                                           int t) {
+  smooth_pair_from_zero_slot<T, DIM>(g, P, P.z, t);
 }
 
 // prolongate(l); smooth(l, 1) (mgpcg.py:146-148) at colour-1 slot t: the
 // colour-0 neighbours read z0 + z_coarse[parent].
 template <typename T, int DIM>
 __device__ __forceinline__ void prolong_smooth1_slot(const Geo& g, const Geo& gc,
-                                                     const LevelPtrs<T>& P, const T* zc0,
-                                                     const T* zc1, int t) {
+                                                     const LevelPtrs<T>& P, T* const* z,
+                                                     const T* zc0, const T* zc1, int t) {
   if (P.gt[1][t] != MG_FLUID) return;
   const Slot c = decode(g, t, 1);  // FLUID cells are interior
   Nbr<DIM> n = neighbours<DIM>(g, t, c.q);
-  const T* z0 = P.z[0];
+  const T* z0 = z[0];
   // value of the colour-0 neighbour (a, dir) after prolongation; a ghost
   // neighbour (outside the level) has no parent and stays 0
   auto zp = [&](int a, int dir, int u) -> T {
@@ -351,7 +365,13 @@ __device__ __forceinline__ void prolong_smooth1_slot(const Geo& g, const Geo& gc
     ret = ret + sm;
   }
   T num = P.r[1][t] - ret;
-  P.z[1][t] = num / P.Adiag[1][t];
+  z[1][t] = num / P.Adiag[1][t];
+}
+template <typename T, int DIM>
+__device__ __forceinline__ void prolong_smooth1_slot(const Geo& g, const Geo& gc,
+                                                     const LevelPtrs<T>& P, const T* zc0,
+                                                     const T* zc1, int t) {
+  prolong_smooth1_slot<T, DIM>(g, gc, P, P.z, zc0, zc1, t);
 }
 
 // prolongate(l) alone (only when no post-smoothing follows): every interior cell.
@@ -370,7 +390,8 @@ __device__ __forceinline__ void prolongate_slot(const Geo& g, const Geo& gc, T* 
 // of colour cc: the residuals of the FLUID children, lexicographic order.
 template <typename T, int DIM>
 __device__ __forceinline__ void restrict_slot(const Geo& gf, const LevelPtrs<T>& F,
-                                              const Geo& gc, T* rc0, T* rc1, int cc, int tc) {
+                                              T* const* z, const Geo& gc, T* rc0, T* rc1,
+                                              int cc, int tc) {
   Slot s = decode(gc, tc, cc);
   if (!s.interior) return;
   T sum = (T)0;
@@ -385,8 +406,8 @@ __device__ __forceinline__ void restrict_slot(const Geo& gf, const LevelPtrs<T>&
         if (F.gt[c][t] != MG_FLUID) continue;
         const int q = (k + 1) & 1;  // padded k' = k + 1 = 2m + q
         Nbr<DIM> n = neighbours<DIM>(gf, t, q);
-        const T* zo = F.z[c ^ 1];
-        T Az = F.Adiag[c][t] * F.z[c][t];
+        const T* zo = z[c ^ 1];
+        T Az = F.Adiag[c][t] * z[c][t];
         T ns = nsum<T, DIM>(F, c, t, n, [&](int u) { return zo[u]; });
         Az = Az + ns;
         T res = F.r[c][t] - Az;
@@ -397,9 +418,13 @@ __device__ __forceinline__ void restrict_slot(const Geo& gf, const LevelPtrs<T>&
 
 // z <- 0 on both colours (a level that gets no smoothing).
 template <typename T>
+__device__ __forceinline__ void zero_slot(T* const* z, int t) {
+  z[0][t] = (T)0;
+  z[1][t] = (T)0;
+}
+template <typename T>
 __device__ __forceinline__ void zero_slot(const LevelPtrs<T>& P, int t) {
-  P.z[0][t] = (T)0;
-  P.z[1][t] = (T)0;
+  zero_slot<T>(P.z, t);
 }
 
 // z.fill(0); smooth(l, 0) on colour 0 only: z0 = r0 / Adiag0 (0 off FLUID).
@@ -533,44 +558,45 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(const LevelDesc<T>* __restri
 // (mgpcg.py:127-148 restricted to those levels), barriers between stages.
 // ---------------------------------------------------------------------------
 template <typename T, int DIM>
-__device__ void coarse_pre(const LevelDesc<T>& D, int nu) {
+__device__ void coarse_pre(const LevelDesc<T>& D, T* const* z, int nu) {
   const int n = D.g.size;
   if (nu <= 0) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(D.P, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(z, t);
     __syncthreads();
     return;
   }
   for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
-    smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
+    smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, z, t);
   __syncthreads();
   for (int i = 1; i < nu; ++i) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 1, t);
     __syncthreads();
   }
 }
 
+// z, zc: this level's and the next coarser level's z colour arrays
 template <typename T, int DIM>
-__device__ void coarse_post(const LevelDesc<T>& D, int nu) {
+__device__ void coarse_post(const LevelDesc<T>& D, T* const* z, T* const* zc, int nu) {
   const int n = D.g.size;
   if (nu <= 0) {
     for (int t = threadIdx.x; t < n; t += COARSE_THREADS) {
-      prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 0, t);
-      prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 1, t);
+      prolongate_slot<T>(D.g, D.gc, z, zc[0], zc[1], 0, t);
+      prolongate_slot<T>(D.g, D.gc, z, zc[0], zc[1], 1, t);
     }
     __syncthreads();
     return;
   }
   for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
-    prolong_smooth1_slot<T, DIM>(D.g, D.gc, D.P, D.C.z[0], D.C.z[1], t);
+    prolong_smooth1_slot<T, DIM>(D.g, D.gc, D.P, z, zc[0], zc[1], t);
   __syncthreads();
-  for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+  for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
   __syncthreads();
   for (int i = 1; i < nu; ++i) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 1, t);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
     __syncthreads();
   }
 }
@@ -578,35 +604,41 @@ __device__ void coarse_post(const LevelDesc<T>& D, int nu) {
 // bottom solve (mgpcg.py:136-142): half x (smooth 0, smooth 1) from z = 0,
 // then half x (smooth 1, smooth 0).
 template <typename T, int DIM>
-__device__ void coarse_bottom(const LevelDesc<T>& D, int bottom) {
+__device__ void coarse_bottom(const LevelDesc<T>& D, T* const* z, int bottom) {
   const int n = D.g.size;
   const int half = bottom / 2;
   if (half <= 0) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(D.P, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(z, t);
     __syncthreads();
     return;
   }
   for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
-    smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
+    smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, z, t);
   __syncthreads();
   for (int i = 1; i < half; ++i) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 1, t);
     __syncthreads();
   }
   for (int i = 0; i < half; ++i) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 1, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 1, t);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, 0, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
     __syncthreads();
   }
 }
 
+// LDS elements of the z copies of one system's coarse levels (48 KB of FP32)
+constexpr int COARSE_LDS_ELEMS = 12288;
+
 // One workgroup per item: item.desc = the system's LevelDesc of its first
 // coarse level, item.block0 = number of levels from there to the bottom
-// (a system's LevelDescs are contiguous).
-template <typename T, int DIM>
+// (a system's LevelDescs are contiguous).  With USE_LDS the z arrays of
+// those levels live in LDS for the whole kernel (the host checks that
+// 2 * sum(level slots) <= COARSE_LDS_ELEMS) and are copied out at the end;
+// r, the operator and the cell types are read from global memory.
+template <typename T, int DIM, bool USE_LDS>
 __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* __restrict__ descs,
                                                            const Item* __restrict__ items,
                                                            int nitems, int nu, int bottom) {
@@ -614,17 +646,55 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
   const int d0 = items[blockIdx.x].desc;
   const int nlev = items[blockIdx.x].block0;
   if (descs[d0].st->done) return;
+  __shared__ T zs[USE_LDS ? COARSE_LDS_ELEMS : 1];
+  // z colour arrays of coarse level k (offsets are prefix sums of 2 * size)
+  auto zof = [&](int k, T** z) {
+    if constexpr (USE_LDS) {
+      int off = 0;
+      for (int j = 0; j < k; ++j) off += 2 * descs[d0 + j].g.size;
+      z[0] = zs + off;
+      z[1] = zs + off + descs[d0 + k].g.size;
+    } else {
+      z[0] = descs[d0 + k].P.z[0];
+      z[1] = descs[d0 + k].P.z[1];
+    }
+  };
   for (int k = 0; k < nlev - 1; ++k) {
     const LevelDesc<T>& D = descs[d0 + k];
-    coarse_pre<T, DIM>(D, nu);
+    T* z[2];
+    zof(k, z);
+    coarse_pre<T, DIM>(D, z, nu);
     for (int tc = threadIdx.x; tc < D.gc.size; tc += COARSE_THREADS) {
-      restrict_slot<T, DIM>(D.g, D.P, D.gc, D.C.r[0], D.C.r[1], 0, tc);
-      restrict_slot<T, DIM>(D.g, D.P, D.gc, D.C.r[0], D.C.r[1], 1, tc);
+      restrict_slot<T, DIM>(D.g, D.P, z, D.gc, D.C.r[0], D.C.r[1], 0, tc);
+      restrict_slot<T, DIM>(D.g, D.P, z, D.gc, D.C.r[0], D.C.r[1], 1, tc);
     }
     __syncthreads();
   }
-  coarse_bottom<T, DIM>(descs[d0 + nlev - 1], bottom);
-  for (int k = nlev - 2; k >= 0; --k) coarse_post<T, DIM>(descs[d0 + k], nu);
+  {
+    T* z[2];
+    zof(nlev - 1, z);
+    coarse_bottom<T, DIM>(descs[d0 + nlev - 1], z, bottom);
+  }
+  for (int k = nlev - 2; k >= 0; --k) {
+    T* z[2];
+    T* zc[2];
+    zof(k, z);
+    zof(k + 1, zc);
+    coarse_post<T, DIM>(descs[d0 + k], z, zc, nu);
+  }
+  if constexpr (USE_LDS) {
+    // the finer level's prolongation reads the first coarse level's z; the
+    // others are copied too so every level's z holds its V-cycle value
+    for (int k = 0; k < nlev; ++k) {
+      const LevelDesc<T>& D = descs[d0 + k];
+      T* z[2];
+      zof(k, z);
+      for (int t = threadIdx.x; t < D.g.size; t += COARSE_THREADS) {
+        D.P.z[0][t] = z[0][t];
+        D.P.z[1][t] = z[1][t];
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -57,6 +57,7 @@ int g_fuse_first = 0;    // first pre-smoothing pair in one kernel (else two)
 int g_fuse_prolong = 0;  // prolongation inside the first post-smoothing half-sweep
 int g_fuse_p = 0;        // update_p inside compute_Ap
 int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
+int g_coarse_lds = 1;    // coarse kernel keeps z in LDS when it fits
 
 }  // namespace
 
@@ -188,6 +189,7 @@ struct Engine {
   };
   StageRef st_init, st_cg, fin_init, fin_cg, fin_all, st_dot_fused, st_dot_plain, st_coarse,
       st_gather;
+  bool coarse_lds = false;  // every coarse item's z fits the coarse kernel's LDS
   std::vector<StageRef> st_slot, st_restrict, st_bottom;  // per level
   int n_items_cap = 0;
   Item* d_items = nullptr;
@@ -470,6 +472,7 @@ int engine_build_items(Engine* e) {
             [L](int m) { return m * L + L - 1; });
   // coarse kernel: desc = first coarse level, block0 = number of coarse levels
   e->st_coarse.off = (int)it.size();
+  e->coarse_lds = g_coarse_lds && e->precision != F64;
   for (int m = 0; m < (int)e->mem.size(); ++m) {
     const Member& M = e->mem[m];
     if (!M.active || M.s->coarse_from >= L) continue;
@@ -477,6 +480,9 @@ int engine_build_items(Engine* e) {
     x.desc = m * L + M.s->coarse_from;
     x.block0 = L - M.s->coarse_from;
     it.push_back(x);
+    long elems = 0;
+    for (int l = M.s->coarse_from; l < L; ++l) elems += 2L * M.s->geo[l].size;
+    if (elems > COARSE_LDS_ELEMS) e->coarse_lds = false;
   }
   e->st_coarse.n = (int)it.size() - e->st_coarse.off;
   e->st_coarse.nblocks = e->st_coarse.n;
@@ -624,10 +630,21 @@ struct Sched {
         stage<ST_ZERO>(e, B);
       }
     }
-    if (e->st_coarse.n)
-      hipLaunchKernelGGL((k_coarse<T, DIM>), dim3(e->st_coarse.n), dim3(COARSE_THREADS), 0,
-                         e->stream, levels(e), (const Item*)(e->d_items + e->st_coarse.off),
-                         e->st_coarse.n, nu, e->bottom);
+    if (e->st_coarse.n) {
+      bool lds = false;
+      if constexpr (sizeof(T) == 4) {
+        lds = e->coarse_lds;
+        if (lds)
+          hipLaunchKernelGGL((k_coarse<T, DIM, true>), dim3(e->st_coarse.n),
+                             dim3(COARSE_THREADS), 0, e->stream, levels(e),
+                             (const Item*)(e->d_items + e->st_coarse.off), e->st_coarse.n, nu,
+                             e->bottom);
+      }
+      if (!lds)
+        hipLaunchKernelGGL((k_coarse<T, DIM, false>), dim3(e->st_coarse.n), dim3(COARSE_THREADS),
+                           0, e->stream, levels(e), (const Item*)(e->d_items + e->st_coarse.off),
+                           e->st_coarse.n, nu, e->bottom);
+    }
     for (int l = L - 2; l >= 0; --l) {
       const Engine::StageRef& S = e->st_slot[l];
       if (nu > 0) {
@@ -1056,6 +1073,9 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
       return 0;
     case MGPCG_OPT_COMPACT:
       g_compact = value != 0;
+      return 0;
+    case MGPCG_OPT_COARSE_LDS:
+      g_coarse_lds = value != 0;
       return 0;
     default:
       (void)s;
