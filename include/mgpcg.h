@@ -80,6 +80,9 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
                                      applies to systems solved or submitted afterwards */
 #define MGPCG_OPT_COARSE_LDS 9    /* 1: the coarse-level kernel keeps z in LDS when a system's
                                      coarse levels fit (FP32 hierarchies; default 1) */
+#define MGPCG_OPT_RESTRICT_ROWS 10 /* 1: restriction tiled by coarse rows with the fine
+                                      residuals in LDS (fine rows up to 512 cells; default 1),
+                                      0: one lane per child */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,

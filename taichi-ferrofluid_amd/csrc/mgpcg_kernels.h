@@ -494,6 +494,67 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
   }
 }
 
+// restrict(l) (mgpcg.py:111-118), row-tiled: a virtual block is one coarse
+// row (I, J) of a system.  The workgroup forms the residuals of the 2^(d-1)
+// fine rows (2I + o0, 2J + o1) -- consecutive lanes take consecutive k, so
+// both colour arrays are read in contiguous runs -- keeps them in LDS, then
+// sums every coarse cell's children in lexicographic order (o0, o1, o2)
+// (0 for a non-FLUID child, which leaves the sum unchanged).
+constexpr int RESTRICT_ROW_MAX = 512;  // longest fine row (n2) of this kernel
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __restrict__ descs,
+                                                         Stage S) {
+  constexpr int NR = (DIM == 3) ? 4 : 2;  // fine rows per coarse row
+  __shared__ T res[NR * RESTRICT_ROW_MAX];
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    const int ii = find_item(S.items, S.nitems, it.vb);
+    const Item item = S.items[ii];
+    const LevelDesc<T>& D = descs[item.desc];
+    if (D.st->done) continue;
+    const Geo& gf = D.g;
+    const Geo& gc = D.gc;
+    const int row = it.vb - item.block0;  // coarse row index I * nc1 + J
+    if (row >= gc.n0 * gc.n1) continue;
+    const int I = (DIM == 3) ? row / gc.n1 : 0;
+    const int J = (DIM == 3) ? row - I * gc.n1 : row;
+    const int n2 = gf.n2;
+    if (n2 > RESTRICT_ROW_MAX) continue;  // the host never selects this kernel then
+    const T* zp[2] = {D.P.z[0], D.P.z[1]};
+    for (int idx = threadIdx.x; idx < NR * n2; idx += BLOCK) {
+      const int rr = idx / n2;
+      const int k = idx - rr * n2;
+      const int i = (DIM == 3) ? 2 * I + (rr >> 1) : 0;
+      const int j = 2 * J + (rr & 1);
+      const int c = colour_of(i, j, k);
+      const int t = slot_of(gf, i, j, k);
+      T v = (T)0;
+      if (D.P.gt[c][t] == MG_FLUID) {
+        const int q = (k + 1) & 1;  // padded k' = k + 1 = 2m + q
+        Nbr<DIM> n = neighbours<DIM>(gf, t, q);
+        const T* zo = zp[c ^ 1];
+        T Az = D.P.Adiag[c][t] * zp[c][t];
+        T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return zo[u]; });
+        Az = Az + ns;
+        v = D.P.r[c][t] - Az;
+      }
+      res[idx] = v;
+    }
+    __syncthreads();
+    for (int K = threadIdx.x; K < gc.n2; K += BLOCK) {
+      T sum = (T)0;
+#pragma unroll
+      for (int o = 0; o < NR; ++o) {
+        sum = sum + res[o * n2 + 2 * K];
+        sum = sum + res[o * n2 + 2 * K + 1];
+      }
+      const int cc = colour_of(I, J, K);
+      (cc ? D.C.r[1] : D.C.r[0])[slot_of(gc, I, J, K)] = sum;
+    }
+    __syncthreads();  // res is reused by the next virtual block
+  }
+}
+
 // restrict(l), child-parallel: 16 lanes per coarse slot -- 8 per coarse
 // colour, one per child (o0, o1, o2) in lexicographic order.  Every lane
 // forms its child's residual (0 when the child is not FLUID, which leaves the

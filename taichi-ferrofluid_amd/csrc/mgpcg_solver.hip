@@ -58,6 +58,7 @@ int g_fuse_prolong = 0;  // prolongation inside the first post-smoothing half-sw
 int g_fuse_p = 0;        // update_p inside compute_Ap
 int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
 int g_coarse_lds = 1;    // coarse kernel keeps z in LDS when it fits
+int g_restrict_rows = 1;  // row-tiled restriction (else child-parallel)
 
 }  // namespace
 
@@ -190,6 +191,7 @@ struct Engine {
   StageRef st_init, st_cg, fin_init, fin_cg, fin_all, st_dot_fused, st_dot_plain, st_coarse,
       st_gather;
   bool coarse_lds = false;  // every coarse item's z fits the coarse kernel's LDS
+  bool restrict_rows = false;  // restriction stages are laid out for k_restrict_rows
   std::vector<StageRef> st_slot, st_restrict, st_bottom;  // per level
   int n_items_cap = 0;
   Item* d_items = nullptr;
@@ -448,6 +450,11 @@ int engine_build_items(Engine* e) {
     ref.nblocks = b0;
   };
   const int L = e->L;
+  // row-tiled restriction when every member's fine rows fit its LDS tile
+  e->restrict_rows = g_restrict_rows != 0;
+  for (const Member& M : e->mem)
+    if (M.active && M.s->geo[0].n2 > RESTRICT_ROW_MAX) e->restrict_rows = false;
+  const bool rows = e->restrict_rows;
   auto sysidx = [](int m) { return m; };
   auto lvl0 = [](const Member& M) { return nblk(M.s->geo[0].size); };
   auto one = [](const Member&) { return 1; };
@@ -463,7 +470,11 @@ int engine_build_items(Engine* e) {
               [l, L](int m) { return m * L + l; });
     add_stage(e->st_restrict[l],
               [l, L](const Member& M) { return l < M.s->coarse_from && l + 1 < L; },
-              [l, L](const Member& M) { return l + 1 < L ? rblk(M.s->geo[l + 1].size) : 0; },
+              [l, L, rows](const Member& M) {
+                if (l + 1 >= L) return 0;
+                const Geo& gc = M.s->geo[l + 1];
+                return rows ? gc.n0 * gc.n1 : rblk(gc.size);
+              },
               [l, L](int m) { return m * L + l; });
   }
   // members whose bottom level is too large for the coarse kernel
@@ -556,6 +567,11 @@ inline Stage stage_of(const Engine* e, const Engine::StageRef& r) {
   return s;
 }
 inline int grid_of(const Engine::StageRef& r) { return std::min(r.nblocks, g_max_grid); }
+// virtual blocks of one system's restriction into the coarse level gc, for the
+// kernel the engine's current layout selects
+inline int restrict_blocks(const Engine* e, const Geo& gc) {
+  return e->restrict_rows ? gc.n0 * gc.n1 : rblk(gc.size);
+}
 
 // ---------------------------------------------------------------------------
 // iteration schedule, templated on hierarchy type T, x type TX and dimension
@@ -581,8 +597,12 @@ struct Sched {
   }
   static void restrict_(Engine* e, const Engine::StageRef& r) {
     if (r.nblocks == 0) return;
-    hipLaunchKernelGGL((k_restrict<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
-                       levels(e), stage_of(e, r));
+    if (e->restrict_rows)
+      hipLaunchKernelGGL((k_restrict_rows<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
+                         levels(e), stage_of(e, r));
+    else
+      hipLaunchKernelGGL((k_restrict<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
+                         levels(e), stage_of(e, r));
   }
   template <int OP>
   static void fin(Engine* e, const Engine::StageRef& r) {
@@ -1077,6 +1097,9 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
     case MGPCG_OPT_COARSE_LDS:
       g_coarse_lds = value != 0;
       return 0;
+    case MGPCG_OPT_RESTRICT_ROWS:
+      g_restrict_rows = value != 0;
+      return 0;
     default:
       (void)s;
       mg_set_error("unknown option %d", option);
@@ -1226,7 +1249,7 @@ struct RefPath {
       Engine::StageRef r;
       r.off = l;
       r.n = 1;
-      r.nblocks = coarse ? rblk(s->geo[l + 1].size) : nblk(s->geo[l].size);
+      r.nblocks = coarse ? restrict_blocks(e, s->geo[l + 1]) : nblk(s->geo[l].size);
       return r;
     };
     auto fill = [&](int l, int which) {
@@ -1428,7 +1451,10 @@ int prepare_op(Engine* e, const mgpcg_solver* s, int op, int level, Engine::Stag
   it.desc = level;  // member 0: LevelDesc index = level
   it.block0 = 0;
   int nb = nblk(s->geo[level].size);
-  if (op == MGPCG_OP_RESTRICT) nb = rblk(s->geo[level + 1].size);
+  if (op == MGPCG_OP_RESTRICT) {
+    e->restrict_rows = g_restrict_rows && s->geo[0].n2 <= RESTRICT_ROW_MAX;
+    nb = restrict_blocks(e, s->geo[level + 1]);
+  }
   if (op == MGPCG_OP_COMPUTE_AP) it.desc = 0;  // SysDesc of member 0
   MG_HIP(hipStreamSynchronize(e->stream));
   MG_HIP(hipMemcpy(e->d_items, &it, sizeof(Item), hipMemcpyHostToDevice));

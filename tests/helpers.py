@@ -9,9 +9,11 @@ def level_shapes(res, L):
     return [tuple(n >> l for n in res) for l in range(L)]
 
 
-def random_hierarchy(res, L, seed, dtype=np.float32, p_fluid=0.7, p_air=0.2, sym_pos=True):
-    """Random cell types (coarsened by the oracle rule), random positive
-    couplings between FLUID neighbours and a diagonally dominant Adiag."""
+def random_hierarchy(res, L, seed, dtype=np.float32, p_fluid=0.7, p_air=0.2, sym_pos=True,
+                     coupling=None):
+    """Random cell types (coarsened by the oracle rule), random negative
+    couplings between FLUID neighbours (or the single value `coupling`, as
+    the pressure operators have) and a diagonally dominant Adiag."""
     from oracle import core
 
     rng = np.random.default_rng(seed)
@@ -33,7 +35,8 @@ def random_hierarchy(res, L, seed, dtype=np.float32, p_fluid=0.7, p_air=0.2, sym
             sl_lo[k] = slice(0, shape[k] - 1)
             sl_hi[k] = slice(1, shape[k])
             nb[tuple(sl_lo)] = fl[tuple(sl_lo)] & fl[tuple(sl_hi)]
-            ax[..., k] = np.where(nb, -rng.uniform(0.5, 1.5, shape), 0.0).astype(dtype)
+            val = -rng.uniform(0.5, 1.5, shape) if coupling is None else coupling
+            ax[..., k] = np.where(nb, val, 0.0).astype(dtype)
         off = np.zeros(shape, np.float64)
         for k in range(dim):
             off += np.abs(ax[..., k])

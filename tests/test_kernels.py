@@ -61,13 +61,14 @@ def test_coarsen_bit_exact(dev, case):
     assert np.array_equal(out.cpu().numpy(), core.coarsen(fine))
 
 
+@pytest.mark.parametrize("coupling", [None, -0.75], ids=["random", "uniform"])
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 @pytest.mark.parametrize("phase", [0, 1])
-def test_smooth_bit_exact(dev, case, phase):
+def test_smooth_bit_exact(dev, case, phase, coupling):
     import _lib
 
     res, L = case
-    h = random_hierarchy(res, L, seed=2)
+    h = random_hierarchy(res, L, seed=2, coupling=coupling)
     s = _solver(h, dev)
     rng = np.random.default_rng(3)
     for l in range(L):
@@ -132,24 +133,36 @@ def test_prolong_smooth1_bit_exact(dev, case):
         assert np.array_equal(got[upd], zr[upd]), f"level {l}"
 
 
+@pytest.mark.parametrize("rows", [1, 0], ids=["rows", "children"])
+@pytest.mark.parametrize("coupling", [None, -0.75], ids=["random", "uniform"])
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_restrict_bit_exact(dev, case):
+def test_restrict_bit_exact(dev, case, coupling, rows):
+    """Both restriction kernels (row-tiled, child-parallel), with stored and
+    with uniform (flag-byte) couplings, against the oracle bit for bit."""
     import _lib
 
     res, L = case
-    h = random_hierarchy(res, L, seed=6)
+    h = random_hierarchy(res, L, seed=6, coupling=coupling)
     s = _solver(h, dev)
-    rng = np.random.default_rng(7)
-    for l in range(L - 1):
-        shape = h["gt"][l].shape
-        r = rng.standard_normal(shape).astype(np.float32)
-        z = rng.standard_normal(shape).astype(np.float32)
-        s.debug_set(_lib.FIELD_R, l, torch.from_numpy(r))
-        s.debug_set(_lib.FIELD_Z, l, torch.from_numpy(z))
-        s.debug_op(_lib.OP_RESTRICT, l, 0)
-        got = _get(s, _lib.FIELD_R, l + 1)
-        ref = core.restrict(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z)
-        assert np.array_equal(got, ref), f"level {l}"
+    lib = _lib.load()
+    _lib.check(lib.mgpcg_set_option(None, _lib.OPT_RESTRICT_ROWS, rows))
+    try:
+        rng = np.random.default_rng(7)
+        for l in range(L - 1):
+            if coupling is not None:
+                assert s.level_coupling(l) == (True, np.float32(coupling)) or \
+                    not np.any(h["Ax"][l])
+            shape = h["gt"][l].shape
+            r = rng.standard_normal(shape).astype(np.float32)
+            z = rng.standard_normal(shape).astype(np.float32)
+            s.debug_set(_lib.FIELD_R, l, torch.from_numpy(r))
+            s.debug_set(_lib.FIELD_Z, l, torch.from_numpy(z))
+            s.debug_op(_lib.OP_RESTRICT, l, 0)
+            got = _get(s, _lib.FIELD_R, l + 1)
+            ref = core.restrict(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z)
+            assert np.array_equal(got, ref), f"level {l}"
+    finally:
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_RESTRICT_ROWS, 1))
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
@@ -174,12 +187,13 @@ def test_prolongate_bit_exact(dev, case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
-def test_update_p_compute_Ap_bit_exact(dev, case):
+@pytest.mark.parametrize("coupling", [None, -0.75], ids=["random", "uniform"])
+def test_update_p_compute_Ap_bit_exact(dev, case, coupling):
     """p_next = z + 0 * p (update_p with beta = 0), Ap = A p_next."""
     import _lib
 
     res, L = case
-    h = random_hierarchy(res, L, seed=10)
+    h = random_hierarchy(res, L, seed=10, coupling=coupling)
     s = _solver(h, dev)
     rng = np.random.default_rng(11)
     z = rng.standard_normal(res).astype(np.float32)

@@ -96,7 +96,7 @@ def test_reference_structured_equals_device_loop(dev):
 
 
 KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 0, "fuse_prolong": 0,
-                 "fuse_p": 0, "compact": 1, "coarse_lds": 1}
+                 "fuse_p": 0, "compact": 1, "coarse_lds": 1, "restrict_rows": 1}
 
 
 def set_knobs(**kw):
@@ -106,7 +106,7 @@ def set_knobs(**kw):
     opt = {"coarse_max": _lib.OPT_COARSE_MAX_SLOTS, "max_grid": _lib.OPT_MAX_GRID,
            "fuse_first": _lib.OPT_FUSE_FIRST, "fuse_prolong": _lib.OPT_FUSE_PROLONG,
            "fuse_p": _lib.OPT_FUSE_P, "compact": _lib.OPT_COMPACT,
-           "coarse_lds": _lib.OPT_COARSE_LDS}
+           "coarse_lds": _lib.OPT_COARSE_LDS, "restrict_rows": _lib.OPT_RESTRICT_ROWS}
     for k, v in kw.items():
         _lib.check(lib.mgpcg_set_option(None, opt[k], v))
 
@@ -117,6 +117,7 @@ def set_knobs(**kw):
     {"fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1},
     {"compact": 0}, {"compact": 0, "fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1},
     {"coarse_lds": 0}, {"coarse_lds": 0, "compact": 0, "fuse_prolong": 1},
+    {"restrict_rows": 0}, {"restrict_rows": 0, "compact": 0, "max_grid": 8},
 ], ids=lambda k: "-".join(f"{a}{b}" for a, b in k.items()))
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
 def test_schedule_knobs_agree(dev, knobs, precision):
