@@ -50,6 +50,7 @@ SYSTEMS = {
     "c1_2d": lambda emu: scenes.c1_2d(seed=0, n=64 if emu else 128),
     "c2_3d_small": lambda emu: scenes.c2_3d(seed=1, n=24 if emu else 64, L=3 if emu else 4),
     "c2_3d_128": lambda emu: None if emu else scenes.c2_3d(seed=1, n=128, L=5),
+    "c3_256": lambda emu: None if emu else scenes.c3_256(seed=2, n=256, L=6),
 }
 
 
@@ -66,6 +67,10 @@ def test_solve_matches_oracle(dev, name, precision):
     assert abs(out["iters"] - outr["iters"]) <= 1, (out["iters"], outr["iters"])
     assert rel_l2(x, xr) <= 1e-5
     assert out["rr"] < out["tol"]
+    if precision == "plain":
+        # FP32 arithmetic in the same order as the oracle, order-insensitive dots
+        assert out["iters"] == outr["iters"]
+        assert np.array_equal(x, xr)
 
 
 def test_hostsync_loop_equals_device_loop(dev):
