@@ -252,6 +252,22 @@ int mgpcg_surface_build_A(void* stream, int dim, const int32_t* res, int level, 
                           const float* phi, double dt, double dx, double sigma, float* Adiag,
                           float* Ax);
 
+/* ---------------------------------------------------------------------------
+ * The step after the solve: the pressure gradient applied to the MAC velocity
+ * (in place; velocity[k] has shape res + e_k, pressure is the solve's x).
+ * ------------------------------------------------------------------------- */
+/* FluidSimulator.apply_pressure (fluid_simulator.py:192-214); c_max is the
+ * ghost-fluid clamp (1e3 in the reference, :205/:211). */
+int mgpcg_apply_pressure(void* stream, int dim, const int32_t* res, const int8_t* cell_type,
+                         const float* phi, const float* pressure, float* const* velocity,
+                         double dt, double rho, double dx, double p0, int ghost_fluid_method,
+                         double c_max);
+/* FerrofluidSimulator.apply_magnetic_force (ferrofluid_simulator.py:114-137). */
+int mgpcg_apply_magnetic_force(void* stream, int dim, const int32_t* res,
+                               const int8_t* cell_type, const float* pressure,
+                               const float* const* H, float* const* velocity, double dt,
+                               double rho, double dx, double p0, double mu0, double k);
+
 const char* mgpcg_last_error(void);
 int mgpcg_version(void);
 

@@ -252,6 +252,74 @@ def magnetic_build_b(cell_type, velocity, H, scale_A, scale_b, p0, mu0, kk):
 
 
 # ----------------------------------------------------------------------------
+# The step after the solve: apply_pressure (fluid_simulator.py:192-214) and
+# apply_magnetic_force (ferrofluid_simulator.py:114-137)
+# ----------------------------------------------------------------------------
+def apply_pressure(cell_type, velocity, pressure, dt, rho, dx, p0, ghost_fluid_method=True,
+                   phi=None, c_max=1e3, H=None, mu0=None, kk=None):
+    """Returns the updated MAC velocity (a new list).  For each axis k and
+    cell I, the face u_k[I] between I_1 = I - e_k and I changes when either
+    side is FLUID: 0 next to SOLID (out of range is SOLID); fluid-air and
+    air-fluid faces take p0 on the AIR side (ghost-fluid weighted), or with
+    ``H`` given (apply_magnetic_force) p0 + 1/2 k mu0 |H|^2 at the AIR
+    cell's centre and no weight; fluid-fluid faces the plain jump."""
+    dim = cell_type.ndim
+    scale = f32(f32(dt) / f32(rho * dx))
+    p0f = f32(p0)
+    I = _grid(cell_type.shape)
+    out = [np.array(u, dtype=f32, copy=True) for u in velocity]
+    sl = tuple(slice(0, n) for n in cell_type.shape)
+    magnetic = H is not None
+    if magnetic:
+        coef = f32(1 / 2 * kk * mu0)
+
+        def p_air(shift_k):
+            pos = tuple((I[a] + (shift_k if a == k else 0)).astype(f32) + f32(0.5)
+                        for a in range(dim))
+            Hv = H_interp(H, pos)
+            hh = (Hv[0] * Hv[0]).astype(f32)
+            for a in range(1, dim):
+                hh = (hh + Hv[a] * Hv[a]).astype(f32)
+            return (p0f + (coef * hh).astype(f32)).astype(f32)
+
+    for k in range(dim):
+        u = out[k][sl].copy()
+        ti, tj = cell_type, _shift(cell_type, k, -1, SOLID)
+        pi, pj = pressure.astype(f32), _shift(pressure.astype(f32), k, -1, f32(0.0))
+        act = (tj == FLUID) | (ti == FLUID)
+        solid = act & ((tj == SOLID) | (ti == SOLID))
+        fa = act & ~solid & (ti == AIR)
+        af = act & ~solid & ~fa & (tj == AIR)
+        ff = act & ~solid & ~fa & ~af
+        new = u.copy()
+        new[solid] = f32(0.0)
+        if magnetic:
+            v_fa = (u - (scale * (p_air(0) - pj)).astype(f32)).astype(f32)
+            v_af = (u - (scale * (pi - p_air(-1))).astype(f32)).astype(f32)
+        elif ghost_fluid_method:
+            phj = _shift(phi, k, -1, f32(1.0))
+            c_fa = _gfm_c(phj, phi, c_max)
+            c_af = _gfm_c(phi, phj, c_max)
+            v_fa = (u - ((scale * (p0f - pj)).astype(f32) * c_fa).astype(f32)).astype(f32)
+            v_af = (u - ((scale * (pi - p0f)).astype(f32) * c_af).astype(f32)).astype(f32)
+        else:
+            v_fa = (u - (scale * (p0f - pj)).astype(f32)).astype(f32)
+            v_af = (u - (scale * (pi - p0f)).astype(f32)).astype(f32)
+        v_ff = (u - (scale * (pi - pj)).astype(f32)).astype(f32)
+        new = np.where(fa, v_fa, new)
+        new = np.where(af, v_af, new)
+        new = np.where(ff, v_ff, new)
+        out[k][sl] = new
+    return out
+
+
+def apply_magnetic_force(cell_type, velocity, pressure, H, dt, rho, dx, p0, mu0, kk):
+    """apply_magnetic_force (ferrofluid_simulator.py:114-137)."""
+    return apply_pressure(cell_type, velocity, pressure, dt, rho, dx, p0,
+                          ghost_fluid_method=False, H=H, mu0=mu0, kk=kk)
+
+
+# ----------------------------------------------------------------------------
 # PotentialFunctionStrategy (potential_function.py:6-60)
 # ----------------------------------------------------------------------------
 def _in_domain(coords, res):

@@ -72,6 +72,8 @@ SIGNATURES = {
     "mgpcg_calc_kappa": (_I, [_P, _I, _P, _P, _D, _P]),
     "mgpcg_surface_build_b": (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _D, _D, _D, _P]),
     "mgpcg_surface_build_A": (_I, [_P, _I, _P, _I, _I, _P, _D, _D, _D, _P, _P]),
+    "mgpcg_apply_pressure": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _D, _D, _I, _D]),
+    "mgpcg_apply_magnetic_force": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D]),
     "mgpcg_batch_create": (_I, [ctypes.POINTER(_P)]),
     "mgpcg_batch_destroy": (_I, [_P]),
     "mgpcg_batch_set_stream": (_I, [_P, _P]),

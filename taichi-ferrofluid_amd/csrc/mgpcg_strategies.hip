@@ -567,6 +567,81 @@ HField make_hfield(int dim, const int32_t* res, const float* const* H) {
   return F;
 }
 
+// ---------------------------------------------------------------------------
+// The step after the solve: FluidSimulator.apply_pressure
+// (fluid_simulator.py:192-214, mode 0) and FerrofluidSimulator.
+// apply_magnetic_force (ferrofluid_simulator.py:114-137, mode 1).  For every
+// cell I and axis k the face velocity u_k[I] between I - e_k and I is
+// updated when either side is FLUID: 0 next to SOLID (out of range counts as
+// SOLID), else minus scale times the pressure jump, the AIR side taking p0
+// (ghost-fluid weighted in mode 0, plus coef |H|^2 at the AIR cell's centre
+// in mode 1).  One thread per cell writes its k faces; the top faces
+// u_k[res_k] are not touched, as in the reference.
+// ---------------------------------------------------------------------------
+struct VelOut {
+  float* u[3];
+  Shape s[3];
+};
+
+__global__ void k_apply_pressure(Shape s, const int8_t* __restrict__ ct,
+                                 const float* __restrict__ phi, const float* __restrict__ p,
+                                 VelOut V, float scale, float p0, int gfm, float c_max, int mode,
+                                 HField H, float coef) {
+  const long t = (long)blockIdx.x * SB + threadIdx.x;
+  if (t >= s.cells()) return;
+  int I[3];
+  s.coords(t, I);
+  const int ti = ct[t];
+  for (int k = 0; k < s.dim; ++k) {
+    int J[3] = {I[0], I[1], I[2]};  // I_1 = I - e_k
+    J[k] -= 1;
+    const bool jin = s.inside(J);
+    const long tj = jin ? s.idx(J) : -1;
+    const int tj_type = jin ? ct[tj] : MG_SOLID;
+    if (!(tj_type == MG_FLUID || ti == MG_FLUID)) continue;
+    float* u = V.u[k] + V.s[k].idx(I);
+    if (tj_type == MG_SOLID || ti == MG_SOLID) {
+      *u = 0.0f;
+    } else if (ti == MG_AIR) {  // fluid (I_1) - air (I)
+      if (mode == 1) {
+        float pos[3];
+        for (int a = 0; a < s.dim; ++a) pos[a] = (float)I[a] + 0.5f;
+        const float pa = p0 + coef * hdoth(H, s.dim, pos);
+        *u = *u - scale * (pa - p[tj]);
+      } else if (gfm) {
+        const float c = gfm_c(phi[tj], phi[t], c_max);
+        *u = *u - scale * (p0 - p[tj]) * c;
+      } else {
+        *u = *u - scale * (p0 - p[tj]);
+      }
+    } else if (tj_type == MG_AIR) {  // air (I_1) - fluid (I)
+      if (mode == 1) {
+        float pos[3];
+        for (int a = 0; a < s.dim; ++a) pos[a] = (float)J[a] + 0.5f;
+        const float pa = p0 + coef * hdoth(H, s.dim, pos);
+        *u = *u - scale * (p[t] - pa);
+      } else if (gfm) {
+        const float c = gfm_c(phi[t], phi[tj], c_max);
+        *u = *u - scale * (p[t] - p0) * c;
+      } else {
+        *u = *u - scale * (p[t] - p0);
+      }
+    } else {  // fluid - fluid
+      *u = *u - scale * (p[t] - p[tj]);
+    }
+  }
+}
+
+VelOut make_vel_out(int dim, const int32_t* res, float* const* velocity) {
+  VelOut V;
+  memset(&V, 0, sizeof(V));
+  for (int k = 0; k < dim; ++k) {
+    V.u[k] = velocity[k];
+    V.s[k] = make_shape(dim, res, 0, k);
+  }
+  return V;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -727,6 +802,48 @@ extern "C" int mgpcg_surface_build_b(void* stream, int dim, const int32_t* res, 
   hipLaunchKernelGGL(k_surface_b, dim3(nb_for(s.cells())), dim3(SB), 0, (hipStream_t)stream, s,
                      d, u, Field{phi, s}, kappa, n, make_ls(dx), (float)dt, (float)dx,
                      (float)(dx * dx), (float)sigma, b);
+  MG_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int mgpcg_apply_pressure(void* stream, int dim, const int32_t* res,
+                                    const int8_t* cell_type, const float* phi,
+                                    const float* pressure, float* const* velocity, double dt,
+                                    double rho, double dx, double p0, int ghost_fluid_method,
+                                    double c_max) {
+  if (check_common(dim, res)) return -1;
+  MG_REQUIRE(cell_type && pressure && velocity, "NULL argument");
+  for (int k = 0; k < dim; ++k) MG_REQUIRE(velocity[k], "velocity[%d] is NULL", k);
+  MG_REQUIRE(!ghost_fluid_method || phi, "ghost-fluid method needs phi");
+  MG_REQUIRE(rho > 0.0 && dx > 0.0, "rho and dx must be positive");
+  Shape s = make_shape(dim, res);
+  HField none;
+  memset(&none, 0, sizeof(none));
+  // scale = dt / (rho * dx) with dt an f32 kernel argument (fluid_simulator.py:194)
+  const float scale = (float)dt / (float)(rho * dx);
+  hipLaunchKernelGGL(k_apply_pressure, dim3(nb_for(s.cells())), dim3(SB), 0, (hipStream_t)stream,
+                     s, cell_type, phi, pressure, make_vel_out(dim, res, velocity), scale,
+                     (float)p0, ghost_fluid_method, (float)c_max, 0, none, 0.0f);
+  MG_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int mgpcg_apply_magnetic_force(void* stream, int dim, const int32_t* res,
+                                          const int8_t* cell_type, const float* pressure,
+                                          const float* const* H, float* const* velocity,
+                                          double dt, double rho, double dx, double p0,
+                                          double mu0, double k) {
+  if (check_common(dim, res)) return -1;
+  MG_REQUIRE(cell_type && pressure && H && velocity, "NULL argument");
+  for (int a = 0; a < dim; ++a) MG_REQUIRE(velocity[a] && H[a], "NULL component %d", a);
+  MG_REQUIRE(rho > 0.0 && dx > 0.0, "rho and dx must be positive");
+  Shape s = make_shape(dim, res);
+  const float scale = (float)dt / (float)(rho * dx);
+  const float coef = (float)(1.0 / 2.0 * k * mu0);
+  hipLaunchKernelGGL(k_apply_pressure, dim3(nb_for(s.cells())), dim3(SB), 0, (hipStream_t)stream,
+                     s, cell_type, (const float*)nullptr, pressure,
+                     make_vel_out(dim, res, velocity), scale, (float)p0, 0, 0.0f, 1,
+                     make_hfield(dim, res, H), coef);
   MG_HIP(hipGetLastError());
   return 0;
 }
