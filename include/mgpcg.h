@@ -268,6 +268,22 @@ int mgpcg_apply_magnetic_force(void* stream, int dim, const int32_t* res,
                                const float* const* H, float* const* velocity, double dt,
                                double rho, double dx, double p0, double mu0, double k);
 
+/* ---------------------------------------------------------------------------
+ * The step before the solve: cell classification and redistancing.
+ * ------------------------------------------------------------------------- */
+/* FluidSimulator.apply_markers (fluid_simulator.py:151-159): non-SOLID cells
+ * become FLUID where phi <= 0, AIR elsewhere (cell_type in place). */
+int mgpcg_apply_markers(void* stream, int dim, const int32_t* res, const float* phi,
+                        int8_t* cell_type);
+/* FluidSimulator.init_boundary (fluid_simulator.py:337-341): SOLID border. */
+int mgpcg_init_boundary(void* stream, int dim, const int32_t* res, int8_t* cell_type);
+/* FastSweepingLevelSet.redistance (level_set.py:260-333), phi in place;
+ * phi_temp, snap (floats) and fixed (bytes) are caller-provided work arrays
+ * of the grid's size.  Neighbouring lines are read as they were before each
+ * sweep (deterministic; the reference's parallel lines race). */
+int mgpcg_redistance(void* stream, int dim, const int32_t* res, double dx, int repeat_times,
+                     float* phi, float* phi_temp, float* snap, int8_t* fixed);
+
 const char* mgpcg_last_error(void);
 int mgpcg_version(void);
 

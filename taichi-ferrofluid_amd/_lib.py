@@ -74,6 +74,9 @@ SIGNATURES = {
     "mgpcg_surface_build_A": (_I, [_P, _I, _P, _I, _I, _P, _D, _D, _D, _P, _P]),
     "mgpcg_apply_pressure": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _D, _D, _I, _D]),
     "mgpcg_apply_magnetic_force": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D]),
+    "mgpcg_apply_markers": (_I, [_P, _I, _P, _P, _P]),
+    "mgpcg_init_boundary": (_I, [_P, _I, _P, _P]),
+    "mgpcg_redistance": (_I, [_P, _I, _P, _D, _I, _P, _P, _P, _P]),
     "mgpcg_batch_create": (_I, [ctypes.POINTER(_P)]),
     "mgpcg_batch_destroy": (_I, [_P]),
     "mgpcg_batch_set_stream": (_I, [_P, _P]),

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 OUT = os.path.join(HERE, "libmgpcg.so")
 
-SOURCES = ["mgpcg_solver.hip", "mgpcg_strategies.hip"]
+SOURCES = ["mgpcg_solver.hip", "mgpcg_strategies.hip", "mgpcg_levelset.hip"]
 
 HIPFLAGS = [
     "--offload-arch=gfx950",

@@ -15,7 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "taichi-ferrofluid_amd", "csrc")
-SOURCES = [os.path.join(CSRC, "mgpcg_solver.hip"), os.path.join(CSRC, "mgpcg_strategies.hip")]
+SOURCES = [os.path.join(CSRC, s) for s in
+           ("mgpcg_solver.hip", "mgpcg_strategies.hip", "mgpcg_levelset.hip")]
 RUNTIME = os.path.join(HERE, "emu_runtime.cpp")
 LIB = os.path.join(HERE, "libmgpcg_emu.so")
 

@@ -37,7 +37,8 @@ def _fields(res, seed, dev):
 
 
 def _t(a, dev):
-    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    # a copy: the device calls update in place, and on the CPU build .to() would share memory
+    return torch.from_numpy(np.array(a, copy=True)).to(dev)
 
 
 @pytest.mark.parametrize("res", [(24, 20), (12, 10, 14)], ids=["2d", "3d"])
