@@ -284,6 +284,16 @@ int mgpcg_init_boundary(void* stream, int dim, const int32_t* res, int8_t* cell_
 int mgpcg_redistance(void* stream, int dim, const int32_t* res, double dx, int repeat_times,
                      float* phi, float* phi_temp, float* snap, int8_t* fixed);
 
+/* FluidSimulator.extrap_velocity (fluid_simulator.py:240-270), velocity in
+ * place: per component, faces next to a FLUID cell are valid; `iterations`
+ * (10 in the reference) passes average the valid faces of each invalid face's
+ * 3^dim neighbourhood into it.  backup (floats), valid and valid_temp (bytes)
+ * are work arrays of at least the largest component's size.  Faces outside a
+ * component's shape count as invalid. */
+int mgpcg_extrap_velocity(void* stream, int dim, const int32_t* res, const int8_t* cell_type,
+                          float* const* velocity, int iterations, float* backup, int8_t* valid,
+                          int8_t* valid_temp);
+
 const char* mgpcg_last_error(void);
 int mgpcg_version(void);
 

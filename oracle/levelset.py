@@ -6,6 +6,8 @@ TEST INFRASTRUCTURE ONLY.
   redistance      FastSweepingLevelSet (level_set.py:260-333): target_surface
                   (level_set.py:66-86) and the Eikonal update
                   update_from_neighbor (level_set.py:88-112), FP32.
+  extrap_velocity fluid_simulator.py:240-270 (faces outside a component's
+                  own shape count as invalid).
 
 Sweep semantics (shared with the HIP kernels; the reference's parallel lines
 race): along the sweep axis a line is walked in order, so cells behind are
@@ -151,3 +153,41 @@ def redistance(phi, dx, repeat_times=2):
             for direction in (1, -1):
                 tmp = _sweep(phi, tmp, fixed, dx, ax, direction)
     return tmp
+
+
+def extrap_velocity(cell_type, velocity, iterations=10):
+    """-> new velocity list (mark_valid, then `iterations` diffuse passes)."""
+    dim = cell_type.ndim
+    out = []
+    for k in range(dim):
+        u = np.array(velocity[k], dtype=f32, copy=True)
+        shape = u.shape
+        # mark_valid: a cell on either side (I - e_k, I) is FLUID
+        fl = np.zeros([n + 2 for n in cell_type.shape], bool)  # padded: out of range = no
+        fl[tuple(slice(1, -1) for _ in range(dim))] = cell_type == FLUID
+        # padded coordinates are I + 1: cell I - e_k, then cell I
+        lo = [slice(1, n + 1) if a != k else slice(0, n) for a, n in enumerate(shape)]
+        hi = [slice(1, n + 1) for n in shape]
+        valid = (fl[tuple(lo)] | fl[tuple(hi)]).astype(np.int8)
+        for _ in range(iterations):
+            src = u.copy()
+            vt = valid.copy()
+            vp = np.zeros([n + 2 for n in shape], np.int8)
+            sp = np.zeros([n + 2 for n in shape], f32)
+            vp[tuple(slice(1, -1) for _ in range(dim))] = vt
+            sp[tuple(slice(1, -1) for _ in range(dim))] = src
+            tot = np.zeros(shape, f32)
+            cnt = np.zeros(shape, np.int32)
+            for off in np.ndindex(*([3] * dim)):  # C order over offsets -1..1
+                sl = tuple(slice(o, o + n) for o, n in zip(off, shape))
+                m = vp[sl] == 1
+                tot = np.where(m, (tot + sp[sl]).astype(f32), tot)
+                cnt = cnt + m
+            upd = (vt == 0) & (cnt > 0)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                avg = (tot / cnt.astype(f32)).astype(f32)
+            u = np.where(upd, avg, u).astype(f32)
+            valid = np.where(upd, np.int8(1), valid).astype(np.int8)
+        out.append(u)
+    return out
+

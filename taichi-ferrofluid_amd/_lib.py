@@ -77,6 +77,7 @@ SIGNATURES = {
     "mgpcg_apply_markers": (_I, [_P, _I, _P, _P, _P]),
     "mgpcg_init_boundary": (_I, [_P, _I, _P, _P]),
     "mgpcg_redistance": (_I, [_P, _I, _P, _D, _I, _P, _P, _P, _P]),
+    "mgpcg_extrap_velocity": (_I, [_P, _I, _P, _P, _P, _I, _P, _P, _P]),
     "mgpcg_batch_create": (_I, [ctypes.POINTER(_P)]),
     "mgpcg_batch_destroy": (_I, [_P]),
     "mgpcg_batch_set_stream": (_I, [_P, _P]),

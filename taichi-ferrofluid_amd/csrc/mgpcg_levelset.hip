@@ -167,6 +167,71 @@ __global__ void k_copy(long n, const float* __restrict__ src, float* __restrict_
   if (t < n) dst[t] = src[t];
 }
 
+// ---------------------------------------------------------------------------
+// FluidSimulator.extrap_velocity (fluid_simulator.py:240-270) for one MAC
+// component k (shape res + e_k): mark_valid -- a face is valid when a cell on
+// either side is FLUID (out of range is not) -- then `iterations` passes of
+// diffuse_quantity, each averaging the valid faces of the 3^dim block around
+// every invalid face (offsets in C order, FP32 sum then one division) into it
+// and marking it valid; each pass reads the velocities and flags as they were
+// before the pass.  Faces outside the component's own shape count as invalid
+// (the reference's shared `valid` field is res + 1 in every axis and keeps
+// other components' flags there, and it is read out of range at index -1).
+// ---------------------------------------------------------------------------
+__global__ void k_mark_valid(Grid f, Grid c, int k, const int8_t* __restrict__ ct,
+                             int8_t* __restrict__ valid) {
+  const long t = (long)blockIdx.x * LB + threadIdx.x;
+  if (t >= f.cells()) return;
+  int I[3];
+  coords(f, t, I);
+  auto fluid = [&](const int* J) {
+    for (int a = 0; a < c.dim; ++a)
+      if (J[a] < 0 || J[a] >= c.n[a]) return false;
+    long u = 0;
+    for (int a = 0; a < c.dim; ++a) u += J[a] * c.st[a];
+    return ct[u] == MG_FLUID;
+  };
+  int J[3] = {I[0], I[1], I[2]};
+  J[k] -= 1;
+  valid[t] = (fluid(J) || fluid(I)) ? 1 : 0;
+}
+
+__global__ void k_diffuse(Grid f, float* __restrict__ dst, const float* __restrict__ src,
+                          int8_t* __restrict__ valid_dst, const int8_t* __restrict__ valid) {
+  const long t = (long)blockIdx.x * LB + threadIdx.x;
+  if (t >= f.cells()) return;
+  if (valid[t] != 0) return;
+  int I[3];
+  coords(f, t, I);
+  float tot = 0.0f;
+  int cnt = 0;
+  const int nz = f.dim == 3 ? 3 : 1;
+  for (int o0 = -1; o0 <= 1; ++o0)
+    for (int o1 = -1; o1 <= 1; ++o1)
+      for (int oz = 0; oz < nz; ++oz) {
+        const int o2 = f.dim == 3 ? oz - 1 : 0;
+        const int J[3] = {I[0] + o0, I[1] + o1, I[2] + o2};
+        bool in = true;
+        for (int a = 0; a < f.dim; ++a) in = in && J[a] >= 0 && J[a] < f.n[a];
+        if (!in) continue;
+        long u = 0;
+        for (int a = 0; a < f.dim; ++a) u += J[a] * f.st[a];
+        if (valid[u] == 1) {
+          tot = tot + src[u];
+          cnt += 1;
+        }
+      }
+  if (cnt > 0) {
+    dst[t] = tot / (float)cnt;
+    valid_dst[t] = 1;
+  }
+}
+
+__global__ void k_copy8(long n, const int8_t* __restrict__ src, int8_t* __restrict__ dst) {
+  const long t = (long)blockIdx.x * LB + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
 inline unsigned nblk(long n) { return (unsigned)((n + LB - 1) / LB); }
 
 int check_grid(int dim, const int32_t* res) {
@@ -233,3 +298,33 @@ extern "C" int mgpcg_redistance(void* stream, int dim, const int32_t* res, doubl
   MG_HIP(hipGetLastError());
   return 0;
 }
+
+extern "C" int mgpcg_extrap_velocity(void* stream, int dim, const int32_t* res,
+                                     const int8_t* cell_type, float* const* velocity,
+                                     int iterations, float* backup, int8_t* valid,
+                                     int8_t* valid_temp) {
+  if (check_grid(dim, res)) return -1;
+  MG_REQUIRE(cell_type && velocity && backup && valid && valid_temp, "NULL argument");
+  for (int k = 0; k < dim; ++k) MG_REQUIRE(velocity[k], "velocity[%d] is NULL", k);
+  MG_REQUIRE(iterations >= 0, "iterations must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  const Grid c = make_grid(dim, res);
+  for (int k = 0; k < dim; ++k) {
+    int32_t fr[3] = {0, 0, 0};
+    for (int a = 0; a < dim; ++a) fr[a] = res[a] + (a == k ? 1 : 0);
+    const Grid f = make_grid(dim, fr);
+    const long n = f.cells();
+    hipLaunchKernelGGL(k_mark_valid, dim3(nblk(n)), dim3(LB), 0, s, f, c, k, cell_type, valid);
+    for (int it = 0; it < iterations; ++it) {
+      hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(LB), 0, s, n, (const float*)velocity[k],
+                         backup);
+      hipLaunchKernelGGL(k_copy8, dim3(nblk(n)), dim3(LB), 0, s, n, (const int8_t*)valid,
+                         valid_temp);
+      hipLaunchKernelGGL(k_diffuse, dim3(nblk(n)), dim3(LB), 0, s, f, velocity[k],
+                         (const float*)backup, valid, (const int8_t*)valid_temp);
+    }
+  }
+  MG_HIP(hipGetLastError());
+  return 0;
+}
+

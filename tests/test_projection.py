@@ -119,3 +119,46 @@ def test_projection_leaves_residual_divergence(dev, precision):
     # without the gradient step the identity fails by O(1): the check has teeth
     b_same = _pressure_b(sysd, dev, [u.clone() for u in d["velocity"]], ct, d["phi"])
     assert rel_l2(b_same[fluid], b0[fluid]) < 1e-6
+
+
+# -- extrap_velocity (fluid_simulator.py:240-270) ---------------------------------
+@pytest.mark.parametrize("res", [(20, 18), (12, 10, 14)], ids=["2d", "3d"])
+@pytest.mark.parametrize("iters", [1, 10])
+def test_extrap_velocity_matches_oracle(dev, res, iters):
+    import projection
+    from oracle import levelset as OL
+
+    rng = np.random.default_rng(7)
+    ct, _, _, vel, _ = _fields(res, 8, dev)
+    ct = np.where(rng.uniform(size=res) < 0.7, AIR, ct).astype(np.int8)  # sparse fluid
+    ref = OL.extrap_velocity(ct, vel, iters)
+    v = [_t(a, dev) for a in vel]
+    projection.extrap_velocity(v, _t(ct, dev), iters)
+    for k in range(len(res)):
+        assert np.array_equal(v[k].cpu().numpy(), ref[k]), k
+
+
+def test_extrap_velocity_closed_form():
+    """A fluid blob with one constant velocity extends that constant to every
+    face within reach and leaves the valid faces unchanged; faces beyond the
+    iteration count keep their values."""
+    from oracle import levelset as OL
+
+    res = (24, 24)
+    ct = np.full(res, AIR, np.int8)
+    ct[10:14, 10:14] = FLUID
+    vel = [np.full([n + (a == k) for a, n in enumerate(res)], 9.0, np.float32) for k in range(2)]
+    for k in range(2):
+        sl = [slice(10, 14), slice(10, 14)]
+        sl[k] = slice(10, 15)
+        vel[k][tuple(sl)] = 0.25 * (k + 1)
+    out = OL.extrap_velocity(ct, vel, 3)
+    for k in range(2):
+        c = 0.25 * (k + 1)
+        sl = [slice(10, 14), slice(10, 14)]
+        sl[k] = slice(10, 15)
+        assert np.all(out[k][tuple(sl)] == np.float32(c))  # valid faces unchanged
+        reach = [slice(7, 17), slice(7, 17)]
+        reach[k] = slice(7, 18)
+        assert np.all(out[k][tuple(reach)] == np.float32(c))  # three layers out
+        assert np.all(out[k][:5, :] == 9.0) and np.all(out[k][:, :5] == 9.0)  # beyond reach
