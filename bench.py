@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--seeds", type=int, default=None, help="distinct scenes cycled through")
     ap.add_argument("--concurrency", type=int, default=None, help="requests in flight")
     ap.add_argument("--precision", default="mixed", choices=["plain", "mixed", "f64"])
+    ap.add_argument("--admit-per-step", type=int, default=2,
+                    help="admissions per server step (0: no limit)")
     ap.add_argument("--baseline-steps", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -160,7 +162,8 @@ def measure_served(args, workload, n, L, seeds, concurrency, steps, warmup, dev)
 
     scenes_dev = [systems.upload(make_scene(workload, i, n, L), dev) for i in range(seeds)]
     torch.cuda.synchronize()
-    srv = serve.Server(64 << 30, precision=args.precision, n_levels=L, device=dev)
+    srv = serve.Server(64 << 30, precision=args.precision, n_levels=L,
+                       admit_per_step=args.admit_per_step or None, device=dev)
     mk = (lambda i: scenes_dev[i % len(scenes_dev)])
     if args.probe_warmup:
         srv.probe(True)
@@ -307,7 +310,8 @@ def main():
                                f"L={L}, 2+2 sweeps, bottom 10, rel residual 1e-6, "
                                f"{args.precision} precision, continuous batching with "
                                f"{args.concurrency} requests in flight",
-                   "n": n, "levels": L, "seeds": args.seeds, "concurrency": args.concurrency},
+                   "n": n, "levels": L, "seeds": args.seeds, "concurrency": args.concurrency,
+                   "admit_per_step": args.admit_per_step},
         "p50_ms": round(head["lat"]["p50_ms"], 3),
         "p99_ms": round(head["lat"]["p99_ms"], 3),
         "iters": head["iters"],

@@ -83,6 +83,8 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
 #define MGPCG_OPT_RESTRICT_ROWS 10 /* 1: restriction tiled by coarse rows with the fine
                                       residuals in LDS (fine rows up to 512 cells; default 1),
                                       0: one lane per child */
+#define MGPCG_OPT_PROBE_EVERY 11  /* the launch probe times one eligible launch in this
+                                     many (default 4) */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,
@@ -171,10 +173,11 @@ int mgpcg_server_pending(const mgpcg_server_t* server, int* n_pending, int* n_ac
                          int* inflight);
 
 /* Measurement (no reference counterpart): enable = 1 starts recording HIP
- * events on the engine's stream around every launch of the level-0
+ * events on the engine's stream around one in MGPCG_OPT_PROBE_EVERY launches of the level-0
  * red-black half-sweep (the dominant kernel, mg::k_stage<..., 0, 1>);
- * enable = 0 stops and returns the number of launches, their summed device
- * time and the summed level-0 cells they swept. */
+ * enable = 0 stops and returns the number of timed launches, their summed
+ * device time and the level-0 cells those launches swept (counted on the
+ * device: systems already finished are not swept). */
 int mgpcg_server_probe(mgpcg_server_t* server, int enable, int* n_launches, double* total_ms,
                        int64_t* total_cells);
 int mgpcg_batch_probe(mgpcg_batch_t* batch, int enable, int* n_launches, double* total_ms,

@@ -59,6 +59,7 @@ int g_fuse_p = 0;        // update_p inside compute_Ap
 int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
 int g_coarse_lds = 1;    // coarse kernel keeps z in LDS when it fits
 int g_restrict_rows = 1;  // row-tiled restriction (else child-parallel)
+int g_probe_every = 4;    // the launch probe times one eligible launch in this many
 
 }  // namespace
 
@@ -216,6 +217,7 @@ struct Engine {
   bool probe_on = false;
   std::vector<hipEvent_t> probe_ev;  // begin/end pairs
   int probe_used = 0;  // pairs recorded
+  int probe_seen = 0;  // eligible launches since the probe started
   unsigned long long* d_swept = nullptr;  // level-0 cells swept by probed launches
 
   ~Engine() { release(); }
@@ -538,7 +540,10 @@ const int PROBE_POOL = 8192;
 // Returns whether the launch is recorded; a recorded launch also counts the
 // cells it sweeps into d_swept (on the device, so systems that finished on
 // the device but are not retired yet are not counted).
+// Every g_probe_every-th eligible launch is timed: the event pair costs a few
+// microseconds of queue time, which the others are spared.
 bool probe_begin(Engine* e) {
+  if (e->probe_seen++ % g_probe_every != 0) return false;
   if (e->probe_used >= PROBE_POOL || e->d_swept == nullptr) return false;
   if ((int)e->probe_ev.size() < 2 * (e->probe_used + 1)) {
     hipEvent_t a, b;
@@ -1099,6 +1104,10 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
       return 0;
     case MGPCG_OPT_RESTRICT_ROWS:
       g_restrict_rows = value != 0;
+      return 0;
+    case MGPCG_OPT_PROBE_EVERY:
+      MG_REQUIRE(value >= 1, "probe period must be >= 1");
+      g_probe_every = value;
       return 0;
     default:
       (void)s;
@@ -1772,6 +1781,7 @@ int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells
     MG_HIP(hipMemsetAsync(e->d_swept, 0, sizeof(unsigned long long), e->stream));
     e->probe_on = true;
     e->probe_used = 0;
+    e->probe_seen = 0;
     return 0;
   }
   e->probe_on = false;

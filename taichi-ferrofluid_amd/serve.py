@@ -30,13 +30,17 @@ import systems
 class Server:
     def __init__(self, hbm_budget_bytes, precision="mixed", n_levels=5, nu=2, bottom=10,
                  max_iters=500, rel_tol=1e-12, abs_tol=1e-14, eps=1e-12, refine_every=10,
-                 max_active=None, device="cuda:0"):
+                 max_active=None, admit_per_step=None, device="cuda:0"):
         self.device = torch.device(device)
         self.budget = int(hbm_budget_bytes)
         self.precision = precision
         self.L, self.nu, self.bottom = n_levels, nu, bottom
         self.refine_every = refine_every
         self.max_active = max_active
+        # at most this many admissions per step: the host's assembly work then
+        # fits in the iteration the device is running, and a cohort that
+        # finished together re-enters staggered
+        self.admit_per_step = admit_per_step
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         _lib.check(self._lib.mgpcg_server_create(int(max_iters), float(rel_tol), float(abs_tol),
@@ -116,6 +120,9 @@ class Server:
         admitted = 0
         while self.queue:
             if self.max_active is not None and len(self.running) >= self.max_active:
+                break
+            if self.admit_per_step is not None and admitted >= self.admit_per_step and \
+                    self.running:
                 break
             req = self.queue[0]
             s = self._acquire(req["scene"]["res"])

@@ -111,7 +111,8 @@ def set_knobs(**kw):
     opt = {"coarse_max": _lib.OPT_COARSE_MAX_SLOTS, "max_grid": _lib.OPT_MAX_GRID,
            "fuse_first": _lib.OPT_FUSE_FIRST, "fuse_prolong": _lib.OPT_FUSE_PROLONG,
            "fuse_p": _lib.OPT_FUSE_P, "compact": _lib.OPT_COMPACT,
-           "coarse_lds": _lib.OPT_COARSE_LDS, "restrict_rows": _lib.OPT_RESTRICT_ROWS}
+           "coarse_lds": _lib.OPT_COARSE_LDS, "restrict_rows": _lib.OPT_RESTRICT_ROWS,
+           "probe_every": _lib.OPT_PROBE_EVERY}
     for k, v in kw.items():
         _lib.check(lib.mgpcg_set_option(None, opt[k], v))
 
@@ -239,6 +240,7 @@ def test_server_continuous_batching_equals_alone(dev):
     devs = [systems.upload(r, dev) for r in reqs]
     done = []
     i = 0
+    set_knobs(probe_every=1)
     srv.probe(True)
     while len(done) < len(reqs):
         if i < len(reqs):  # one new request per step: staggered admission
@@ -250,11 +252,12 @@ def test_server_continuous_batching_equals_alone(dev):
             srv.retire(rq)
             done.append(rq)
     n_launch, ms, cells = srv.probe(False)
+    set_knobs(probe_every=4)
     assert srv.max_members >= 2  # they really overlapped
-    # the probe counts exactly the level-0 cells swept by smooth(0, 0): two
-    # launches per V-cycle (nu = 2), iters + 1 V-cycles per solve, none for
-    # members that finished on the device but were not yet retired, and none
-    # for systems small enough to run entirely in the coarse kernel
+    # with every launch timed, the probe counts exactly the level-0 cells swept
+    # by smooth(0, 0): two launches per V-cycle (nu = 2), iters + 1 V-cycles
+    # per solve, none for members that finished on the device but were not
+    # yet retired, and none for systems run entirely by the coarse kernel
     expect = sum((rq["result"]["iters"] + 1) * 2 * int(np.prod(reqs[rq["meta"]]["res"]))
                  for rq in done if rq["coarse_from"] > 0)
     assert cells == expect
