@@ -69,6 +69,8 @@ def parse():
                     help="skip the 64-system mixed batch and the Poisson-arrival sweep")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only (used for profiling runs)")
+    ap.add_argument("--probe-every", type=int, default=4,
+                    help="time one level-0 sweep launch in this many (1 for counter runs)")
     ap.add_argument("--probe-warmup", action="store_true",
                     help="count the warm-up requests' level-0 sweeps too (counter runs, which "
                          "attribute every dispatch of the process)")
@@ -279,13 +281,20 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     if args.copy_calibration_mib:
+        # two known-size copies: 16-byte vector accesses (aligned views) and
+        # 4-byte scalar accesses (views offset by one element, which the
+        # elementwise kernel cannot vectorize) -- the access width of the
+        # solver's kernels
         nel = args.copy_calibration_mib << 18
-        src = torch.ones(nel, device=dev)
+        src = torch.ones(nel + 1, device=dev)
         dst = torch.empty_like(src)
         for _ in range(3):
-            torch.mul(src, 1.0, out=dst)
+            torch.mul(src[:nel], 1.0, out=dst[:nel])
+        for _ in range(3):
+            torch.add(src[1:], 0.0, out=dst[1:])
         torch.cuda.synchronize()
         del src, dst
+    _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_PROBE_EVERY, args.probe_every))
     wl, n, L = args.workload, args.n, args.levels
     what = WORKLOADS[wl]["what"]
 

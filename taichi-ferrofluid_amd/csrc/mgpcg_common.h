@@ -24,6 +24,31 @@
 #define MG_AIR 1
 #define MG_SOLID 2
 
+// Global (HBM) address space.  Pointers the kernels read from descriptor
+// tables are generic to the compiler, which then emits flat loads; accesses
+// through ldg / stg are global loads and stores.  Only for device memory
+// from hipMalloc, never for LDS.
+#ifndef MG_GLOBAL
+#define MG_GLOBAL __attribute__((address_space(1)))
+#endif
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const MG_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* p, T v) {
+  *(MG_GLOBAL T*)p = v;
+}
+// Makes a loaded value live at this point of the program, so the compiler
+// cannot sink its load into a later branch (all of a slot's loads are then in
+// flight together).  Emits no instruction.
+#ifndef MG_KEEP
+#define MG_KEEP(x) asm volatile("" ::"v"(x))
+#define MG_KEEP4(a, b, c, d) asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d))
+#define MG_KEEP6(a, b, c, d, e, f) \
+  asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d), "v"(e), "v"(f))
+#endif
+
 namespace mg {
 
 // Division by a runtime-invariant divisor (dividend < 2^31), multiply-high form.

@@ -269,6 +269,121 @@ __device__ __forceinline__ T nsum(const LevelPtrs<T>& P, int c, int t, const Nbr
   return ret;
 }
 
+// ---------------------------------------------------------------------------
+// latency-oriented forms for the batched kernels: every load of a slot is
+// issued before its FLUID test (neighbour indices are clamped into the padded
+// arrays, so ghost slots stay in bounds; their results are discarded), loads
+// and stores go through the global address space, and the work item with
+// its per-system fields is looked up once per run of virtual blocks.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ Nbr<DIM> neighbours_clamped(const Geo& g, int t, int q) {
+  Nbr<DIM> n = neighbours<DIM>(g, t, q);
+  const int top = g.size - 1;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    n.lo[a] = n.lo[a] < 0 ? 0 : n.lo[a];
+    n.hi[a] = n.hi[a] > top ? top : n.hi[a];
+  }
+  return n;
+}
+
+template <typename T>
+__device__ __forceinline__ Coupling<T> coupling_g(const LevelPtrs<T>& P) {
+  Coupling<T> k{false, (T)0};
+  if (P.meta != nullptr) {
+    const uint32_t mn = ldg(&P.meta->minbits), mx = ldg(&P.meta->maxbits);
+    if (mx == 0u) {
+      k.uniform = true;
+    } else if (mn == mx) {
+      k.uniform = true;
+      k.coef = (T)__uint_as_float(mn);
+    }
+  }
+  return k;
+}
+
+// the lower / upper coupling of every axis at slot t of colour c
+template <typename T, int DIM>
+__device__ __forceinline__ void couplings_g(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
+                                            int t, const Nbr<DIM>& n, T* al, T* ah) {
+  if (k.uniform) {
+    const uint32_t f = ldg(P.nb[c] + t);
+    MG_KEEP(f);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      al[a] = ((f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
+      ah[a] = ((f >> a) & 1u) ? k.coef : (T)0;
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      al[a] = ldg(P.Ax[a][c ^ 1] + n.lo[a]);
+      ah[a] = ldg(P.Ax[a][c] + t);
+    }
+    if constexpr (DIM == 3)
+      MG_KEEP6(al[0], al[1], al[2], ah[0], ah[1], ah[2]);
+    else
+      MG_KEEP4(al[0], al[1], ah[0], ah[1]);
+  }
+}
+
+// neighbour_sum (mgpcg.py:96-102) from gathered couplings and values
+template <typename T, int DIM>
+__device__ __forceinline__ T nsum_vals(const T* al, const T* ah, const T* vl, const T* vh) {
+  T ret = (T)0;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    T lo = al[a] * vl[a];
+    T hi = ah[a] * vh[a];
+    T s = lo + hi;
+    ret = ret + s;
+  }
+  return ret;
+}
+
+// the item of virtual block vb, kept while the workgroup's next virtual
+// blocks fall into it; returns true when it changed
+struct ItemCache {
+  int begin = 0, end = 0, desc = 0, block0 = 0;
+};
+__device__ __forceinline__ bool item_at(const Stage& S, int vb, ItemCache& c) {
+  if (vb >= c.begin && vb < c.end) return false;
+  const int ii = find_item(S.items, S.nitems, vb);
+  const Item it = S.items[ii];
+  c.desc = it.desc;
+  c.block0 = it.block0;
+  c.begin = it.block0;
+  c.end = ii + 1 < S.nitems ? S.items[ii + 1].block0 : S.nblocks;
+  return true;
+}
+
+// smooth(l, c) at slot t in the latency-oriented form; returns the slot's z
+// after the half-sweep (the new value on a FLUID cell)
+template <typename T, int DIM>
+__device__ __forceinline__ T smooth_slot_g(const Geo& g, const LevelPtrs<T>& P,
+                                           const Coupling<T>& k, int c, int t) {
+  const int q = row_q(g, t, c);
+  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
+  const int8_t gt = ldg(P.gt[c] + t);
+  const T rv = ldg(P.r[c] + t);
+  const T ad = ldg(P.Adiag[c] + t);
+  const T* zo = P.z[c ^ 1];
+  T zl[DIM], zh[DIM], al[DIM], ah[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    zl[a] = ldg(zo + n.lo[a]);
+    zh[a] = ldg(zo + n.hi[a]);
+  }
+  couplings_g<T, DIM>(P, k, c, t, n, al, ah);
+  if (gt != MG_FLUID) return (T)0;  // unused: callers mask with the cell type
+  T sm = nsum_vals<T, DIM>(al, ah, zl, zh);
+  T num = rv - sm;
+  const T zn = num / ad;
+  stg(P.z[c] + t, zn);
+  return zn;
+}
+
 // parent slot in the coarser level of fine-level slot `t` of colour `c`
 // (fine original cell (i, j, k) -> (i/2, j/2, k/2)); returns colour in pc.
 __device__ __forceinline__ int parent_slot(const Geo& gf, const Geo& gc, int t, int c, int& pc) {
@@ -467,27 +582,48 @@ enum StageOp {
 template <typename T, int DIM, int OP, int L0>
 __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict__ descs,
                                                  Stage S) {
+  ItemCache ic;
+  bool done = true;
+  Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const LevelDesc<T>& D = descs[item.desc];
-    if (D.st->done) continue;
+    if (item_at(S, it.vb, ic)) {
+      const LevelDesc<T>& D = descs[ic.desc];
+      done = ldg(&D.st->done) != 0;
+      kc = coupling_g(D.P);
+    }
+    if (done) continue;
+    const LevelDesc<T>& D = descs[ic.desc];
     if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
-      if (S.swept != nullptr && it.vb == item.block0 && threadIdx.x == 0)
+      if (S.swept != nullptr && it.vb == ic.block0 && threadIdx.x == 0)
         atomicAdd(S.swept, (unsigned long long)D.g.ncells);
     }
-    const int t = (it.vb - item.block0) * BLOCK + threadIdx.x;
+    const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
-    if (OP == ST_SMOOTH0) smooth_slot<T, DIM>(D.g, D.P, 0, t);
-    else if (OP == ST_SMOOTH1) smooth_slot<T, DIM>(D.g, D.P, 1, t);
-    else if (OP == ST_PAIR_FROM_ZERO) smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
-    else if (OP == ST_PROLONG_SMOOTH1)
+    if (OP == ST_SMOOTH0) {
+      (void)smooth_slot_g<T, DIM>(D.g, D.P, kc, 0, t);
+    } else if (OP == ST_SMOOTH1) {
+      (void)smooth_slot_g<T, DIM>(D.g, D.P, kc, 1, t);
+    } else if (OP == ST_FIRST) {
+      // z.fill(0); smooth(l, 0) on colour 0: z0 = r0 / Adiag0 on FLUID, else 0
+      const int8_t gt = ldg(D.P.gt[0] + t);
+      const T rv = ldg(D.P.r[0] + t);
+      const T ad = ldg(D.P.Adiag[0] + t);
+      stg(D.P.z[0] + t, gt == MG_FLUID ? rv / ad : (T)0);
+    } else if (OP == ST_PROLONGATE0) {
+      Slot sl = decode(D.g, t, 0);
+      if (!sl.interior) continue;
+      const int pi = sl.i >> 1, pj = sl.j >> 1, pk = sl.k >> 1;
+      const int ps = slot_of(D.gc, pi, pj, pk);
+      const T zc = ldg((colour_of(pi, pj, pk) ? D.C.z[1] : D.C.z[0]) + ps);
+      const T zv = ldg(D.P.z[0] + t);
+      stg(D.P.z[0] + t, zv + zc);
+    } else if (OP == ST_PAIR_FROM_ZERO) {
+      smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
+    } else if (OP == ST_PROLONG_SMOOTH1) {
       prolong_smooth1_slot<T, DIM>(D.g, D.gc, D.P, D.C.z[0], D.C.z[1], t);
-    else if (OP == ST_ZERO) zero_slot<T>(D.P, t);
-    else if (OP == ST_FIRST) first_half_sweep_slot<T>(D.P, t);
-    else if (OP == ST_PROLONGATE0)
-      prolongate0_slot<T>(D.g, D.gc, D.P.z[0], D.C.z[0], D.C.z[1], t);
-    else if (OP == ST_PROLONGATE) {
+    } else if (OP == ST_ZERO) {
+      zero_slot<T>(D.P, t);
+    } else if (OP == ST_PROLONGATE) {
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 0, t);
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 1, t);
     }
@@ -507,20 +643,25 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __r
                                                          Stage S) {
   constexpr int NR = (DIM == 3) ? 4 : 2;  // fine rows per coarse row
   __shared__ T res[NR * RESTRICT_ROW_MAX];
+  ItemCache ic;
+  bool done = true;
+  Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const LevelDesc<T>& D = descs[item.desc];
-    if (D.st->done) continue;
+    if (item_at(S, it.vb, ic)) {
+      const LevelDesc<T>& D = descs[ic.desc];
+      done = ldg(&D.st->done) != 0;
+      kc = coupling_g(D.P);
+    }
+    if (done) continue;
+    const LevelDesc<T>& D = descs[ic.desc];
     const Geo& gf = D.g;
     const Geo& gc = D.gc;
-    const int row = it.vb - item.block0;  // coarse row index I * nc1 + J
+    const int row = it.vb - ic.block0;  // coarse row index I * nc1 + J
     if (row >= gc.n0 * gc.n1) continue;
     const int I = (DIM == 3) ? row / gc.n1 : 0;
     const int J = (DIM == 3) ? row - I * gc.n1 : row;
     const int n2 = gf.n2;
     if (n2 > RESTRICT_ROW_MAX) continue;  // the host never selects this kernel then
-    const T* zp[2] = {D.P.z[0], D.P.z[1]};
     for (int idx = threadIdx.x; idx < NR * n2; idx += BLOCK) {
       const int rr = idx / n2;
       const int k = idx - rr * n2;
@@ -528,15 +669,26 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __r
       const int j = 2 * J + (rr & 1);
       const int c = colour_of(i, j, k);
       const int t = slot_of(gf, i, j, k);
+      const int q = (k + 1) & 1;  // padded k' = k + 1 = 2m + q
+      const Nbr<DIM> n = neighbours<DIM>(gf, t, q);  // an interior cell: in bounds
+      const int8_t gt = ldg(D.P.gt[c] + t);
+      const T zv = ldg(D.P.z[c] + t);
+      const T ad = ldg(D.P.Adiag[c] + t);
+      const T rv = ldg(D.P.r[c] + t);
+      const T* zo = D.P.z[c ^ 1];
+      T zl[DIM], zh[DIM], al[DIM], ah[DIM];
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        zl[a] = ldg(zo + n.lo[a]);
+        zh[a] = ldg(zo + n.hi[a]);
+      }
+      couplings_g<T, DIM>(D.P, kc, c, t, n, al, ah);
       T v = (T)0;
-      if (D.P.gt[c][t] == MG_FLUID) {
-        const int q = (k + 1) & 1;  // padded k' = k + 1 = 2m + q
-        Nbr<DIM> n = neighbours<DIM>(gf, t, q);
-        const T* zo = zp[c ^ 1];
-        T Az = D.P.Adiag[c][t] * zp[c][t];
-        T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return zo[u]; });
+      if (gt == MG_FLUID) {
+        T Az = ad * zv;
+        T ns = nsum_vals<T, DIM>(al, ah, zl, zh);
         Az = Az + ns;
-        v = D.P.r[c][t] - Az;
+        v = rv - Az;
       }
       res[idx] = v;
     }
@@ -549,7 +701,7 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __r
         sum = sum + res[o * n2 + 2 * K + 1];
       }
       const int cc = colour_of(I, J, K);
-      (cc ? D.C.r[1] : D.C.r[0])[slot_of(gc, I, J, K)] = sum;
+      stg((cc ? D.C.r[1] : D.C.r[0]) + slot_of(gc, I, J, K), sum);
     }
     __syncthreads();  // res is reused by the next virtual block
   }
@@ -847,27 +999,38 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap(const SysDesc<T, TX>* __restric
 template <typename T, typename TX>
 __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __restrict__ sys,
                                                        Stage S) {
+  ItemCache ic;
+  bool done = true, plain = false;
+  int par = 0;
+  float b32 = 0.0f;
+  double beta = 0.0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const SysDesc<T, TX>& D = sys[item.desc];
-    const CGState* st = D.st;
-    if (st->done) continue;
-    const int t = (it.vb - item.block0) * BLOCK + threadIdx.x;
+    if (item_at(S, it.vb, ic)) {
+      const CGState* st = sys[ic.desc].st;
+      done = ldg(&st->done) != 0;
+      par = ldg(&st->iter) & 1;
+      plain = ldg(&st->precision) == PLAIN;
+      b32 = ldg(&st->beta32);
+      beta = ldg(&st->beta);
+    }
+    if (done) continue;
+    const SysDesc<T, TX>& D = sys[ic.desc];
+    const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
-    const int par = st->iter & 1;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      if (D.P.gt[c][t] != MG_FLUID) continue;
-      const T pold = D.p[par][c][t];
+      const int8_t gt = ldg(D.P.gt[c] + t);
+      const T pold = ldg(D.p[par][c] + t);
+      const T zv = ldg(D.P.z[c] + t);
+      if (gt != MG_FLUID) continue;
       T pv;
-      if (st->precision == PLAIN) {
-        T tt = (T)st->beta32 * pold;
-        pv = D.P.z[c][t] + tt;
+      if (plain) {
+        T tt = (T)b32 * pold;
+        pv = zv + tt;
       } else {
-        pv = (T)((double)D.P.z[c][t] + st->beta * (double)pold);
+        pv = (T)((double)zv + beta * (double)pold);
       }
-      D.p[par ^ 1][c][t] = pv;
+      stg(D.p[par ^ 1][c] + t, pv);
     }
   }
 }
@@ -876,28 +1039,43 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
 template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __restrict__ sys,
                                                       Stage S) {
+  ItemCache ic;
+  bool done = true;
+  int par = 0;
+  Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const SysDesc<T, TX>& D = sys[item.desc];
-    const CGState* st = D.st;
-    if (st->done) continue;
-    const int lb = it.vb - item.block0;
+    if (item_at(S, it.vb, ic)) {
+      const SysDesc<T, TX>& D = sys[ic.desc];
+      done = ldg(&D.st->done) != 0;
+      par = (ldg(&D.st->iter) & 1) ^ 1;
+      kc = coupling_g(D.P);
+    }
+    if (done) continue;
+    const SysDesc<T, TX>& D = sys[ic.desc];
+    const int lb = it.vb - ic.block0;
     const int t = lb * BLOCK + threadIdx.x;
-    const int par = (st->iter & 1) ^ 1;
     DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        if (D.P.gt[c][t] != MG_FLUID) continue;
         const int q = row_q(D.g, t, c);
-        Nbr<DIM> n = neighbours<DIM>(D.g, t, q);
+        const Nbr<DIM> n = neighbours_clamped<DIM>(D.g, t, q);
+        const int8_t gt = ldg(D.P.gt[c] + t);
+        const T pv = ldg(D.p[par][c] + t);
+        const T ad = ldg(D.P.Adiag[c] + t);
         const T* po = D.p[par][c ^ 1];
-        const T pv = D.p[par][c][t];
-        T rr = D.P.Adiag[c][t] * pv;
-        T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return po[u]; });
+        T vl[DIM], vh[DIM], al[DIM], ah[DIM];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          vl[a] = ldg(po + n.lo[a]);
+          vh[a] = ldg(po + n.hi[a]);
+        }
+        couplings_g<T, DIM>(D.P, kc, c, t, n, al, ah);
+        if (gt != MG_FLUID) continue;
+        T rr = ad * pv;
+        T ns = nsum_vals<T, DIM>(al, ah, vl, vh);
         T ap = rr + ns;
-        D.Ap[c][t] = ap;
+        stg(D.Ap[c] + t, ap);
         acc = dd_add(acc, dd_prod(pv, ap));
       }
     }
@@ -911,47 +1089,55 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
 template <typename T, typename TX>
 __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restrict__ sys,
                                                  Stage S) {
+  ItemCache ic;
+  bool done = true, refine = false;
+  int par = 0, prec = PLAIN;
+  float a32 = 0.0f;
+  double alpha = 0.0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const SysDesc<T, TX>& D = sys[item.desc];
-    const CGState* st = D.st;
-    if (st->done) continue;
-    const int lb = it.vb - item.block0;
+    if (item_at(S, it.vb, ic)) {
+      const CGState* st = sys[ic.desc].st;
+      done = ldg(&st->done) != 0;
+      par = (ldg(&st->iter) & 1) ^ 1;  // the p written by k_cg_update_p / k_cg_ap
+      prec = ldg(&st->precision);
+      const int re = ldg(&st->refine_every);
+      refine = prec == MIXED && re > 0 && ldg(&st->since_refine) + 1 >= re;
+      a32 = ldg(&st->alpha32);
+      alpha = ldg(&st->alpha);
+    }
+    if (done) continue;
+    const SysDesc<T, TX>& D = sys[ic.desc];
+    const int lb = it.vb - ic.block0;
     const int t = lb * BLOCK + threadIdx.x;
-    const int par = (st->iter & 1) ^ 1;  // the p written by k_cg_ap
-    const int prec = st->precision;
-    const bool refine = prec == MIXED && st->refine_every > 0 &&
-                        st->since_refine + 1 >= st->refine_every;
     DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        if (D.P.gt[c][t] != MG_FLUID) continue;
-        const T pv = D.p[par][c][t];
-        const T apv = D.Ap[c][t];
+        const int8_t gt = ldg(D.P.gt[c] + t);
+        const T pv = ldg(D.p[par][c] + t);
+        const T apv = ldg(D.Ap[c] + t);
+        const TX xv = ldg(D.x[c] + t);
+        const T r0 = ldg(D.P.r[c] + t);
+        if (gt != MG_FLUID) continue;
         if (prec == PLAIN) {
-          const float a32 = st->alpha32;
           T tp = (T)a32 * pv;
-          D.x[c][t] = (TX)((T)D.x[c][t] + tp);
+          stg(D.x[c] + t, (TX)((T)xv + tp));
           T ta = (T)a32 * apv;
-          T rv = D.P.r[c][t] - ta;
-          D.P.r[c][t] = rv;
+          T rv = r0 - ta;
+          stg(D.P.r[c] + t, rv);
           acc = dd_add(acc, dd_prod(rv, rv));
         } else if (prec == F64) {
-          const double alpha = st->alpha;
           T tp = (T)(alpha * (double)pv);
-          D.x[c][t] = (TX)((double)D.x[c][t] + (double)tp);
+          stg(D.x[c] + t, (TX)((double)xv + (double)tp));
           T ta = (T)(alpha * (double)apv);
-          T rv = D.P.r[c][t] - ta;
-          D.P.r[c][t] = rv;
+          T rv = r0 - ta;
+          stg(D.P.r[c] + t, rv);
           acc = dd_add(acc, dd_prod(rv, rv));
         } else {
-          const double alpha = st->alpha;
-          D.x[c][t] = (TX)((double)D.x[c][t] + alpha * (double)pv);
+          stg(D.x[c] + t, (TX)((double)xv + alpha * (double)pv));
           if (!refine) {
-            T rv = (T)((double)D.P.r[c][t] - alpha * (double)apv);
-            D.P.r[c][t] = rv;
+            T rv = (T)((double)r0 - alpha * (double)apv);
+            stg(D.P.r[c] + t, rv);
             acc = dd_add(acc, dd_prod(rv, rv));
           }
         }
@@ -966,35 +1152,52 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
 template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* __restrict__ sys,
                                                          Stage S) {
+  ItemCache ic;
+  bool skip = true;
+  Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const SysDesc<T, TX>& D = sys[item.desc];
-    const CGState* st = D.st;
-    if (st->done || !st->pending_true) continue;
-    const int lb = it.vb - item.block0;
+    if (item_at(S, it.vb, ic)) {
+      const SysDesc<T, TX>& D = sys[ic.desc];
+      skip = ldg(&D.st->done) != 0 || ldg(&D.st->pending_true) == 0;
+      kc = coupling_g(D.P);
+    }
+    if (skip) continue;
+    const SysDesc<T, TX>& D = sys[ic.desc];
+    const int lb = it.vb - ic.block0;
     const int t = lb * BLOCK + threadIdx.x;
     DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        if (D.P.gt[c][t] != MG_FLUID) continue;
         const int q = row_q(D.g, t, c);
-        Nbr<DIM> n = neighbours<DIM>(D.g, t, q);
+        const Nbr<DIM> n = neighbours_clamped<DIM>(D.g, t, q);
+        const int8_t gt = ldg(D.P.gt[c] + t);
         const TX* xo = D.x[c ^ 1];
+        TX xl[DIM], xh[DIM];
+        T al[DIM], ah[DIM];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          xl[a] = ldg(xo + n.lo[a]);
+          xh[a] = ldg(xo + n.hi[a]);
+        }
+        couplings_g<T, DIM>(D.P, kc, c, t, n, al, ah);
+        const T ad = ldg(D.P.Adiag[c] + t);
+        const TX xv = ldg(D.x[c] + t);
+        const T bv = ldg(D.b[c] + t);
+        if (gt != MG_FLUID) continue;
         double ret = 0.0;
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
-          double lo = (double)D.P.Ax[a][c ^ 1][n.lo[a]] * (double)xo[n.lo[a]];
-          double hi = (double)D.P.Ax[a][c][t] * (double)xo[n.hi[a]];
+          double lo = (double)al[a] * (double)xl[a];
+          double hi = (double)ah[a] * (double)xh[a];
           double s2 = lo + hi;
           ret = ret + s2;
         }
-        double Axv = (double)D.P.Adiag[c][t] * (double)D.x[c][t];
+        double Axv = (double)ad * (double)xv;
         Axv = Axv + ret;
-        double rv = (double)D.b[c][t] - Axv;
+        double rv = (double)bv - Axv;
         T r32 = (T)rv;
-        D.P.r[c][t] = r32;
+        stg(D.P.r[c] + t, r32);
         acc = dd_add(acc, dd_prod(r32, r32));
       }
     }
@@ -1008,18 +1211,29 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
 template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __restrict__ sys,
                                                        Stage S) {
+  ItemCache ic;
+  bool done = true;
+  Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const SysDesc<T, TX>& D = sys[item.desc];
-    if (D.st->done) continue;
-    const int lb = it.vb - item.block0;
+    if (item_at(S, it.vb, ic)) {
+      const SysDesc<T, TX>& D = sys[ic.desc];
+      done = ldg(&D.st->done) != 0;
+      kc = coupling_g(D.P);
+    }
+    if (done) continue;
+    const SysDesc<T, TX>& D = sys[ic.desc];
+    const int lb = it.vb - ic.block0;
     const int t = lb * BLOCK + threadIdx.x;
     DD acc = dd_zero();
     if (t < D.g.size) {
-      smooth_slot<T, DIM>(D.g, D.P, 0, t);
-      if (D.P.gt[0][t] == MG_FLUID) acc = dd_add(acc, dd_prod(D.P.z[0][t], D.P.r[0][t]));
-      if (D.P.gt[1][t] == MG_FLUID) acc = dd_add(acc, dd_prod(D.P.z[1][t], D.P.r[1][t]));
+      const int8_t g0 = ldg(D.P.gt[0] + t);
+      const int8_t g1 = ldg(D.P.gt[1] + t);
+      const T r0 = ldg(D.P.r[0] + t);
+      const T r1 = ldg(D.P.r[1] + t);
+      const T z1 = ldg(D.P.z[1] + t);
+      const T z0 = smooth_slot_g<T, DIM>(D.g, D.P, kc, 0, t);
+      if (g0 == MG_FLUID) acc = dd_add(acc, dd_prod(z0, r0));
+      if (g1 == MG_FLUID) acc = dd_add(acc, dd_prod(z1, r1));
     }
     DD s = block_sum_dd<BLOCK>(acc);
     if (threadIdx.x == 0) D.partials[lb] = s;

@@ -23,6 +23,10 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define MG_GLOBAL
+#define MG_KEEP(x) ((void)(x))
+#define MG_KEEP4(a, b, c, d) ((void)(a), (void)(b), (void)(c), (void)(d))
+#define MG_KEEP6(a, b, c, d, e, f) ((void)(a), (void)(b), (void)(c), (void)(d), (void)(e), (void)(f))
 
 typedef enum hipError_t {
   hipSuccess = 0,
