@@ -69,11 +69,11 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
 
 /* Process-wide tuning knobs (no reference counterpart; `solver` is unused). */
 #define MGPCG_OPT_COARSE_MAX_SLOTS 3 /* levels with at most this many slots per colour are
-                                        solved by one workgroup per system (default 4096);
+                                        solved by one workgroup per system (default 1024);
                                         applies to handles created afterwards */
 #define MGPCG_OPT_MAX_GRID 4         /* workgroups of a persistent stage launch (default 2048) */
 #define MGPCG_OPT_FUSE_FIRST 5    /* 1: fill(0)+smooth(0)+smooth(1) in one kernel (default 0) */
-#define MGPCG_OPT_FUSE_PROLONG 6  /* 1: prolongate inside the next smooth(1) (default 0) */
+#define MGPCG_OPT_FUSE_PROLONG 6  /* 1: prolongate inside the next smooth(1) (default 1) */
 #define MGPCG_OPT_FUSE_P 7        /* 1: update_p inside compute_Ap (default 0) */
 #define MGPCG_OPT_COMPACT 8       /* 1: levels whose nonzero off-diagonals are all one value
                                      read per-cell coupling flags instead of Ax (default 1);
@@ -81,8 +81,8 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
 #define MGPCG_OPT_COARSE_LDS 9    /* 1: the coarse-level kernel keeps z in LDS when a system's
                                      coarse levels fit (FP32 hierarchies; default 1) */
 #define MGPCG_OPT_RESTRICT_ROWS 10 /* 1: restriction tiled by coarse rows with the fine
-                                      residuals in LDS (fine rows up to 512 cells; default 1),
-                                      0: one lane per child */
+                                      residuals in LDS (fine rows up to 512 cells),
+                                      0: one lane per child (default 0) */
 #define MGPCG_OPT_PROBE_EVERY 11  /* the launch probe times one eligible launch in this
                                      many (default 4) */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);

@@ -384,6 +384,58 @@ __device__ __forceinline__ T smooth_slot_g(const Geo& g, const LevelPtrs<T>& P,
   return zn;
 }
 
+// prolongate(l); smooth(l, 1) at colour-1 slot t in the latency-oriented form
+// (same arithmetic as prolong_smooth1_slot below)
+template <typename T, int DIM>
+__device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& gc,
+                                                       const LevelPtrs<T>& P,
+                                                       const Coupling<T>& k, const T* zc0,
+                                                       const T* zc1, int t) {
+  const Slot c = decode(g, t, 1);
+  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, c.q);
+  const int8_t gt = ldg(P.gt[1] + t);
+  const T rv = ldg(P.r[1] + t);
+  const T ad = ldg(P.Adiag[1] + t);
+  const T* z0 = P.z[0];
+  T zl[DIM], zh[DIM], cl[DIM], ch[DIM], al[DIM], ah[DIM];
+  bool il[DIM], ih[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    const int va = a + 3 - DIM;  // view axis of original axis a
+#pragma unroll
+    for (int dir = -1; dir <= 1; dir += 2) {
+      int i = c.i, j = c.j, kk = c.k;
+      if (va == 0) i += dir;
+      else if (va == 1) j += dir;
+      else kk += dir;
+      // a neighbour outside the level has no parent and keeps its z
+      const bool in = !(i < 0 || i >= g.n0 || j < 0 || j >= g.n1 || kk < 0 || kk >= g.n2);
+      const int pi = in ? i >> 1 : 0, pj = in ? j >> 1 : 0, pk = in ? kk >> 1 : 0;
+      const int ps = slot_of(gc, pi, pj, pk);
+      const T zcv = ldg((colour_of(pi, pj, pk) ? zc1 : zc0) + ps);
+      if (dir < 0) {
+        zl[a] = ldg(z0 + n.lo[a]);
+        cl[a] = zcv;
+        il[a] = in;
+      } else {
+        zh[a] = ldg(z0 + n.hi[a]);
+        ch[a] = zcv;
+        ih[a] = in;
+      }
+    }
+  }
+  couplings_g<T, DIM>(P, k, 1, t, n, al, ah);
+  if (gt != MG_FLUID) return;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    if (il[a]) zl[a] = zl[a] + cl[a];
+    if (ih[a]) zh[a] = zh[a] + ch[a];
+  }
+  T ret = nsum_vals<T, DIM>(al, ah, zl, zh);
+  T num = rv - ret;
+  stg(P.z[1] + t, num / ad);
+}
+
 // parent slot in the coarser level of fine-level slot `t` of colour `c`
 // (fine original cell (i, j, k) -> (i/2, j/2, k/2)); returns colour in pc.
 __device__ __forceinline__ int parent_slot(const Geo& gf, const Geo& gc, int t, int c, int& pc) {
@@ -620,7 +672,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     } else if (OP == ST_PAIR_FROM_ZERO) {
       smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
     } else if (OP == ST_PROLONG_SMOOTH1) {
-      prolong_smooth1_slot<T, DIM>(D.g, D.gc, D.P, D.C.z[0], D.C.z[1], t);
+      prolong_smooth1_slot_g<T, DIM>(D.g, D.gc, D.P, kc, D.C.z[0], D.C.z[1], t);
     } else if (OP == ST_ZERO) {
       zero_slot<T>(D.P, t);
     } else if (OP == ST_PROLONGATE) {
@@ -715,42 +767,58 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __r
 constexpr int RESTRICT_SLOTS_PER_BLOCK = BLOCK / 16;
 
 template <typename T, int DIM>
-__device__ __forceinline__ T child_res(const Geo& gf, const LevelPtrs<T>& F, const Slot& s,
-                                       int ch) {
+__device__ __forceinline__ T child_res(const Geo& gf, const LevelPtrs<T>& F, const Coupling<T>& k,
+                                       const Slot& s, int ch) {
   const int o0 = (ch >> 2) & 1, o1 = (ch >> 1) & 1, o2 = ch & 1;
   if (DIM == 2 && o0) return (T)0;
   const int i = (DIM == 3) ? 2 * s.i + o0 : 0;
-  const int j = 2 * s.j + o1, k = 2 * s.k + o2;
-  const int c = colour_of(i, j, k);
-  const int t = slot_of(gf, i, j, k);
-  if (F.gt[c][t] != MG_FLUID) return (T)0;
-  const int q = (k + 1) & 1;  // padded k' = k + 1 = 2m + q
-  Nbr<DIM> n = neighbours<DIM>(gf, t, q);
+  const int j = 2 * s.j + o1, kk = 2 * s.k + o2;
+  const int c = colour_of(i, j, kk);
+  const int t = slot_of(gf, i, j, kk);
+  const int q = (kk + 1) & 1;  // padded k' = k + 1 = 2m + q
+  const Nbr<DIM> n = neighbours<DIM>(gf, t, q);  // a real cell: in bounds
+  const int8_t gt = ldg(F.gt[c] + t);
+  const T zv = ldg(F.z[c] + t);
+  const T ad = ldg(F.Adiag[c] + t);
+  const T rv = ldg(F.r[c] + t);
   const T* zo = F.z[c ^ 1];
-  T Az = F.Adiag[c][t] * F.z[c][t];
-  T ns = nsum<T, DIM>(F, c, t, n, [&](int u) { return zo[u]; });
+  T zl[DIM], zh[DIM], al[DIM], ah[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    zl[a] = ldg(zo + n.lo[a]);
+    zh[a] = ldg(zo + n.hi[a]);
+  }
+  couplings_g<T, DIM>(F, k, c, t, n, al, ah);
+  if (gt != MG_FLUID) return (T)0;
+  T Az = ad * zv;
+  T ns = nsum_vals<T, DIM>(al, ah, zl, zh);
   Az = Az + ns;
-  return F.r[c][t] - Az;
+  return rv - Az;
 }
 
 template <typename T, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_restrict(const LevelDesc<T>* __restrict__ descs,
                                                     Stage S) {
+  ItemCache ic;
+  bool done = true;
+  Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const LevelDesc<T>& D = descs[item.desc];
-    if (D.st->done) continue;
-    const int tc = (it.vb - item.block0) * RESTRICT_SLOTS_PER_BLOCK + (threadIdx.x >> 4);
+    if (item_at(S, it.vb, ic)) {
+      const LevelDesc<T>& D = descs[ic.desc];
+      done = ldg(&D.st->done) != 0;
+      kc = coupling_g(D.P);
+    }
+    if (done) continue;
+    const LevelDesc<T>& D = descs[ic.desc];
+    const int tc = (it.vb - ic.block0) * RESTRICT_SLOTS_PER_BLOCK + (threadIdx.x >> 4);
     const int cc = (threadIdx.x >> 3) & 1;
     const int ch = threadIdx.x & 7;
     T res = (T)0;
     bool interior = false;
-    Slot s;
     if (tc < D.gc.size) {
-      s = decode(D.gc, tc, cc);
+      const Slot s = decode(D.gc, tc, cc);
       interior = s.interior;
-      if (interior) res = child_res<T, DIM>(D.g, D.P, s, ch);
+      if (interior) res = child_res<T, DIM>(D.g, D.P, kc, s, ch);
     }
     // gather the eight children into lane ch == 0 through LDS (all lanes take part)
     __shared__ T xch[BLOCK];
@@ -760,7 +828,7 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(const LevelDesc<T>* __restri
       T sum = (T)0;
 #pragma unroll
       for (int o = 0; o < 8; ++o) sum = sum + xch[threadIdx.x + o];
-      (cc ? D.C.r[1] : D.C.r[0])[tc] = sum;
+      stg((cc ? D.C.r[1] : D.C.r[0]) + tc, sum);
     }
     __syncthreads();  // xch is reused by the next virtual block
   }

@@ -13,6 +13,7 @@
 //               boundaries -- which is also how the server does continuous
 //               batching.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstdarg>
@@ -51,14 +52,14 @@ struct LevelBuf {
   LevelMeta* meta;   // nonzero-coupling summary, written when the level is loaded
 };
 
-int g_coarse_max_slots = 4096;  // levels at most this large go to k_coarse
+int g_coarse_max_slots = 1024;  // levels at most this large go to k_coarse
 int g_max_grid = 2048;          // workgroups of a persistent stage launch
 int g_fuse_first = 0;    // first pre-smoothing pair in one kernel (else two)
-int g_fuse_prolong = 0;  // prolongation inside the first post-smoothing half-sweep
+int g_fuse_prolong = 1;  // prolongation inside the first post-smoothing half-sweep
 int g_fuse_p = 0;        // update_p inside compute_Ap
 int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
 int g_coarse_lds = 1;    // coarse kernel keeps z in LDS when it fits
-int g_restrict_rows = 1;  // row-tiled restriction (else child-parallel)
+int g_restrict_rows = 0;  // row-tiled restriction (else child-parallel)
 int g_probe_every = 4;    // the launch probe times one eligible launch in this many
 
 }  // namespace
@@ -539,9 +540,10 @@ int engine_build_items(Engine* e) {
 const int PROBE_POOL = 8192;
 // Returns whether the launch is recorded; a recorded launch also counts the
 // cells it sweeps into d_swept (on the device, so systems that finished on
-// the device but are not retired yet are not counted).
-// Every g_probe_every-th eligible launch is timed: the event pair costs a few
-// microseconds of queue time, which the others are spared.
+// the device but are not retired yet are not counted).  Every
+// g_probe_every-th eligible launch is timed.  The event pair travels with
+// the kernel's own dispatch (hipExtLaunchKernelGGL), so it times the
+// kernel's execution rather than the queue around it.
 bool probe_begin(Engine* e) {
   if (e->probe_seen++ % g_probe_every != 0) return false;
   if (e->probe_used >= PROBE_POOL || e->d_swept == nullptr) return false;
@@ -555,13 +557,9 @@ bool probe_begin(Engine* e) {
     e->probe_ev.push_back(a);
     e->probe_ev.push_back(b);
   }
-  (void)hipEventRecord(e->probe_ev[2 * e->probe_used], e->stream);
   return true;
 }
-void probe_end(Engine* e) {
-  (void)hipEventRecord(e->probe_ev[2 * e->probe_used + 1], e->stream);
-  e->probe_used++;
-}
+void probe_end(Engine* e) { e->probe_used++; }
 
 inline Stage stage_of(const Engine* e, const Engine::StageRef& r) {
   Stage s;
@@ -592,9 +590,15 @@ struct Sched {
     const bool probe = level0 && OP == ST_SMOOTH0 && e->probe_on && probe_begin(e);
     if (level0) {
       Stage st = stage_of(e, r);
-      if (probe) st.swept = e->d_swept;
-      hipLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
-                         levels(e), st);
+      if (probe) {
+        st.swept = e->d_swept;
+        hipExtLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0,
+                              e->stream, e->probe_ev[2 * e->probe_used],
+                              e->probe_ev[2 * e->probe_used + 1], 0, levels(e), st);
+      } else {
+        hipLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0,
+                           e->stream, levels(e), st);
+      }
     } else
       hipLaunchKernelGGL((k_stage<T, DIM, OP, 0>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
                          levels(e), stage_of(e, r));

@@ -162,7 +162,7 @@ def test_restrict_bit_exact(dev, case, coupling, rows):
             ref = core.restrict(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z)
             assert np.array_equal(got, ref), f"level {l}"
     finally:
-        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_RESTRICT_ROWS, 1))
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_RESTRICT_ROWS, 0))
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
