@@ -85,6 +85,8 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
                                       0: one lane per child (default 0) */
 #define MGPCG_OPT_PROBE_EVERY 11  /* the launch probe times one eligible launch in this
                                      many (default 4) */
+#define MGPCG_OPT_SERVER_DEPTH 12 /* iterations a server keeps queued on the device ahead of
+                                     the host, 1..3 (default 2) */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,

@@ -61,6 +61,7 @@ int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
 int g_coarse_lds = 1;    // coarse kernel keeps z in LDS when it fits
 int g_restrict_rows = 0;  // row-tiled restriction (else child-parallel)
 int g_probe_every = 4;    // the launch probe times one eligible launch in this many
+int g_server_depth = 2;   // server iterations queued on the device ahead of the host
 
 }  // namespace
 
@@ -205,9 +206,11 @@ struct Engine {
 
   // state readback
   CGState* d_states = nullptr;
-  CGState* h_states[2] = {nullptr, nullptr};
-  hipEvent_t ev_states[2] = {nullptr, nullptr};
-  std::vector<std::pair<int, uint32_t>> gathered[2];  // (member slot, generation) per readback
+  // state readbacks in flight (a ring: the server keeps several iterations queued)
+  static const int NSLOT = 4;
+  CGState* h_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<std::pair<int, uint32_t>> gathered[NSLOT];  // (member slot, generation)
   long iteration = 0;
 
   // pinned staging of admissions (per member slot): descriptors, initial state
@@ -236,7 +239,7 @@ struct Engine {
       if (h_items[i]) (void)hipHostFree(h_items[i]);
       if (ev_items[i]) (void)hipEventDestroy(ev_items[i]);
     }
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NSLOT; ++i) {
       if (h_states[i]) (void)hipHostFree(h_states[i]);
       if (ev_states[i]) (void)hipEventDestroy(ev_states[i]);
     }
@@ -263,7 +266,7 @@ int engine_init(Engine* e) {
   MG_HIP(hipEventCreateWithFlags(&e->ev_done, hipEventDisableTiming));
   for (int i = 0; i < Engine::RING; ++i)
     MG_HIP(hipEventCreateWithFlags(&e->ev_items[i], hipEventDisableTiming));
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < Engine::NSLOT; ++i)
     MG_HIP(hipEventCreateWithFlags(&e->ev_states[i], hipEventDisableTiming));
   return 0;
 }
@@ -308,7 +311,7 @@ int engine_reserve(Engine* e, int cap) {
   e->d_sys = ns;
   if (e->d_states) MG_HIP(hipFree(e->d_states));
   MG_HIP(hipMalloc(&e->d_states, sizeof(CGState) * (size_t)ncap));
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < Engine::NSLOT; ++i) {
     if (e->h_states[i]) MG_HIP(hipHostFree(e->h_states[i]));
     MG_HIP(hipHostMalloc((void**)&e->h_states[i], sizeof(CGState) * (size_t)ncap, 0));
   }
@@ -1113,6 +1116,11 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
       MG_REQUIRE(value >= 1, "probe period must be >= 1");
       g_probe_every = value;
       return 0;
+    case MGPCG_OPT_SERVER_DEPTH:
+      MG_REQUIRE(value >= 1 && value < Engine::NSLOT, "server depth must be in [1, %d]",
+                 Engine::NSLOT - 1);
+      g_server_depth = value;
+      return 0;
     default:
       (void)s;
       mg_set_error("unknown option %d", option);
@@ -1664,8 +1672,8 @@ extern "C" int mgpcg_batch_solve(mgpcg_batch_t* b, mgpcg_solver_t* const* system
 // the ones whose state (read one iteration behind) says they are done.
 struct mgpcg_server {
   Engine* e = nullptr;
-  int slot = 0;
-  bool inflight = false;
+  std::deque<int> inflight;  // readback slots of the queued iterations, oldest first
+  int next_slot = 0;
   std::deque<std::pair<mgpcg_solver*, int64_t>> pending;
   std::vector<std::pair<int64_t, mgpcg_result_t>> finished;
   struct Params {
@@ -1732,15 +1740,24 @@ extern "C" int mgpcg_server_step(mgpcg_server_t* sv, int* n_active) {
                           sv->prm.refine_every};
     if (engine_add(e, pr.first, pr.second, prm) < 0) return -2;
   }
+  // Up to g_server_depth iterations stay queued on the device while the host
+  // returns to its caller (admissions, assembly); a member's completion is
+  // seen that many iterations late, and its later iterations skip it.
   std::vector<Finished> done;
   if (engine_active(e) > 0) {
-    if (engine_enqueue(e, sv->slot, false)) return -2;
-    if (sv->inflight && engine_collect(e, sv->slot ^ 1, &done)) return -2;
-    sv->inflight = true;
-    sv->slot ^= 1;
-  } else if (sv->inflight) {
-    if (engine_collect(e, sv->slot ^ 1, &done)) return -2;
-    sv->inflight = false;
+    const int slot = sv->next_slot;
+    sv->next_slot = (sv->next_slot + 1) % Engine::NSLOT;
+    if (engine_enqueue(e, slot, false)) return -2;
+    sv->inflight.push_back(slot);
+    while ((int)sv->inflight.size() > g_server_depth) {
+      if (engine_collect(e, sv->inflight.front(), &done)) return -2;
+      sv->inflight.pop_front();
+    }
+  } else {
+    while (!sv->inflight.empty()) {
+      if (engine_collect(e, sv->inflight.front(), &done)) return -2;
+      sv->inflight.pop_front();
+    }
   }
   for (const Finished& f : done) {
     mgpcg_result_t r;
@@ -1773,7 +1790,7 @@ extern "C" int mgpcg_server_pending(const mgpcg_server_t* sv, int* n_pending, in
   MG_REQUIRE(sv != nullptr, "server is NULL");
   if (n_pending) *n_pending = (int)sv->pending.size();
   if (n_active) *n_active = engine_active(sv->e);
-  if (inflight) *inflight = sv->inflight ? 1 : 0;
+  if (inflight) *inflight = sv->inflight.empty() ? 0 : 1;
   return 0;
 }
 

@@ -81,7 +81,8 @@ def test_constants_match_header():
              "OPT_FUSE_PROLONG": "MGPCG_OPT_FUSE_PROLONG", "OPT_FUSE_P": "MGPCG_OPT_FUSE_P",
              "OPT_COMPACT": "MGPCG_OPT_COMPACT", "OPT_COARSE_LDS": "MGPCG_OPT_COARSE_LDS",
              "OPT_RESTRICT_ROWS": "MGPCG_OPT_RESTRICT_ROWS",
-             "OPT_PROBE_EVERY": "MGPCG_OPT_PROBE_EVERY"}
+             "OPT_PROBE_EVERY": "MGPCG_OPT_PROBE_EVERY",
+             "OPT_SERVER_DEPTH": "MGPCG_OPT_SERVER_DEPTH"}
     for py, c in pairs.items():
         assert getattr(_lib, py) == int(defs[c]), (py, c)
     # every #define of the header is mirrored
