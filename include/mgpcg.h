@@ -80,9 +80,10 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
                                      applies to systems solved or submitted afterwards */
 #define MGPCG_OPT_COARSE_LDS 9    /* 1: the coarse-level kernel keeps z in LDS when a system's
                                      coarse levels fit (FP32 hierarchies; default 1) */
-#define MGPCG_OPT_RESTRICT_ROWS 10 /* 1: restriction tiled by coarse rows with the fine
-                                      residuals in LDS (fine rows up to 512 cells),
-                                      0: one lane per child (default 0) */
+#define MGPCG_OPT_RESTRICT_ROWS 10 /* restriction kernels: 2 a residual pass into scratch
+                                      then a child gather (default), 1 tiled by coarse
+                                      rows with the residuals in LDS (fine rows up to 512
+                                      cells), 0 one lane per child */
 #define MGPCG_OPT_PROBE_EVERY 11  /* the launch probe times one eligible launch in this
                                      many (default 4) */
 #define MGPCG_OPT_SERVER_DEPTH 12 /* iterations a server keeps queued on the device ahead of

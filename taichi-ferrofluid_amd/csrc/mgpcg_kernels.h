@@ -55,6 +55,7 @@ struct LevelPtrs {
   T* z[2];
   const uint8_t* nb[2];    // coupling flags per slot
   const LevelMeta* meta;   // NULL: always read Ax
+  T* res[2];               // restriction scratch: the level's residual (not the coarsest)
 };
 
 // The level's single coupling value when its couplings are uniform.
@@ -756,6 +757,91 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __r
       stg((cc ? D.C.r[1] : D.C.r[0]) + slot_of(gc, I, J, K), sum);
     }
     __syncthreads();  // res is reused by the next virtual block
+  }
+}
+
+// restrict(l) (mgpcg.py:111-118) in two coalesced passes.  Pass 1: the
+// residual r - (Adiag z + neighbour_sum) of every FLUID cell, 0 elsewhere,
+// into the level's scratch arrays, both colours of a slot per thread.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restrict__ descs,
+                                                    Stage S) {
+  ItemCache ic;
+  bool done = true;
+  Coupling<T> kc{false, (T)0};
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    if (item_at(S, it.vb, ic)) {
+      const LevelDesc<T>& D = descs[ic.desc];
+      done = ldg(&D.st->done) != 0;
+      kc = coupling_g(D.P);
+    }
+    if (done) continue;
+    const LevelDesc<T>& D = descs[ic.desc];
+    const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
+    if (t >= D.g.size) continue;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int q = row_q(D.g, t, c);
+      const Nbr<DIM> n = neighbours_clamped<DIM>(D.g, t, q);
+      const int8_t gt = ldg(D.P.gt[c] + t);
+      const T zv = ldg(D.P.z[c] + t);
+      const T ad = ldg(D.P.Adiag[c] + t);
+      const T rv = ldg(D.P.r[c] + t);
+      const T* zo = D.P.z[c ^ 1];
+      T zl[DIM], zh[DIM], al[DIM], ah[DIM];
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        zl[a] = ldg(zo + n.lo[a]);
+        zh[a] = ldg(zo + n.hi[a]);
+      }
+      couplings_g<T, DIM>(D.P, kc, c, t, n, al, ah);
+      T v = (T)0;
+      if (gt == MG_FLUID) {
+        T Az = ad * zv;
+        T ns = nsum_vals<T, DIM>(al, ah, zl, zh);
+        Az = Az + ns;
+        v = rv - Az;
+      }
+      stg(D.P.res[c] + t, v);
+    }
+  }
+}
+
+// Pass 2: every interior coarse cell sums its children's residuals in
+// lexicographic order (o0, o1, o2) into the coarse r (a 0 residual leaves the
+// sum unchanged, as skipping a non-FLUID child does).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_restrict_gather(const LevelDesc<T>* __restrict__ descs,
+                                                           Stage S) {
+  ItemCache ic;
+  bool done = true;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+    if (item_at(S, it.vb, ic)) done = ldg(&descs[ic.desc].st->done) != 0;
+    if (done) continue;
+    const LevelDesc<T>& D = descs[ic.desc];
+    const int tc = (it.vb - ic.block0) * BLOCK + threadIdx.x;
+    if (tc >= D.gc.size) continue;
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const Slot s = decode(D.gc, tc, cc);
+      if (!s.interior) continue;
+      T v[8];
+      int nch = 0;
+#pragma unroll
+      for (int o0 = 0; o0 < (DIM == 3 ? 2 : 1); ++o0)
+#pragma unroll
+        for (int o1 = 0; o1 < 2; ++o1)
+#pragma unroll
+          for (int o2 = 0; o2 < 2; ++o2) {
+            const int i = (DIM == 3) ? 2 * s.i + o0 : 0;
+            const int j = 2 * s.j + o1, k = 2 * s.k + o2;
+            v[nch++] = ldg(D.P.res[colour_of(i, j, k)] + slot_of(D.g, i, j, k));
+          }
+      T sum = (T)0;
+#pragma unroll
+      for (int o = 0; o < (DIM == 3 ? 8 : 4); ++o) sum = sum + v[o];
+      stg((cc ? D.C.r[1] : D.C.r[0]) + tc, sum);
+    }
   }
 }
 

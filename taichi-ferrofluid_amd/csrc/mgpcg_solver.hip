@@ -50,6 +50,7 @@ struct LevelBuf {
   void* z[2];
   uint8_t* nb[2];    // coupling flags
   LevelMeta* meta;   // nonzero-coupling summary, written when the level is loaded
+  void* res[2];      // restriction scratch (NULL on the coarsest level)
 };
 
 int g_coarse_max_slots = 1024;  // levels at most this large go to k_coarse
@@ -59,7 +60,9 @@ int g_fuse_prolong = 1;  // prolongation inside the first post-smoothing half-sw
 int g_fuse_p = 0;        // update_p inside compute_Ap
 int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
 int g_coarse_lds = 1;    // coarse kernel keeps z in LDS when it fits
-int g_restrict_rows = 0;  // row-tiled restriction (else child-parallel)
+// restriction kernels: 0 one lane per child, 1 row-tiled with LDS, 2 residual
+// pass + child gather (both coalesced)
+int g_restrict_mode = 2;
 int g_probe_every = 4;    // the launch probe times one eligible launch in this many
 int g_server_depth = 2;   // server iterations queued on the device ahead of the host
 
@@ -108,6 +111,7 @@ LevelPtrs<T> level_ptrs(const mgpcg_solver* s, int l) {
     P.z[c] = (T*)B.z[c];
     for (int a = 0; a < 3; ++a) P.Ax[a][c] = (const T*)B.Ax[a][c];
     P.nb[c] = B.nb[c];
+    P.res[c] = (T*)B.res[c];
   }
   P.meta = (g_compact && !s->full_coupling) ? B.meta : nullptr;
   return P;
@@ -190,11 +194,12 @@ struct Engine {
   std::vector<Item> items;
   struct StageRef {
     int off = 0, n = 0, nblocks = 0;
+    int fine_nblocks = 0;  // one-item restriction stages: the fine level's slot blocks
   };
   StageRef st_init, st_cg, fin_init, fin_cg, fin_all, st_dot_fused, st_dot_plain, st_coarse,
       st_gather;
   bool coarse_lds = false;  // every coarse item's z fits the coarse kernel's LDS
-  bool restrict_rows = false;  // restriction stages are laid out for k_restrict_rows
+  int restrict_mode = 0;  // restriction stages are laid out for this g_restrict_mode
   std::vector<StageRef> st_slot, st_restrict, st_bottom;  // per level
   int n_items_cap = 0;
   Item* d_items = nullptr;
@@ -436,6 +441,14 @@ int engine_add(Engine* e, mgpcg_solver* s, int64_t tag, const SolveParams& prm) 
   return slot;
 }
 
+// virtual blocks of one system's restriction into the coarse level gc, for the
+// kernel the engine's current layout selects
+inline int restrict_blocks(const Engine* e, const Geo& gc) {
+  if (e->restrict_mode == 1) return gc.n0 * gc.n1;
+  if (e->restrict_mode == 2) return nblk(gc.size);
+  return rblk(gc.size);
+}
+
 // Rebuild the item lists of every stage from the member set.
 int engine_build_items(Engine* e) {
   std::vector<Item>& it = e->items;
@@ -456,11 +469,11 @@ int engine_build_items(Engine* e) {
     ref.nblocks = b0;
   };
   const int L = e->L;
-  // row-tiled restriction when every member's fine rows fit its LDS tile
-  e->restrict_rows = g_restrict_rows != 0;
-  for (const Member& M : e->mem)
-    if (M.active && M.s->geo[0].n2 > RESTRICT_ROW_MAX) e->restrict_rows = false;
-  const bool rows = e->restrict_rows;
+  // row-tiled restriction only when every member's fine rows fit its LDS tile
+  e->restrict_mode = g_restrict_mode;
+  if (e->restrict_mode == 1)
+    for (const Member& M : e->mem)
+      if (M.active && M.s->geo[0].n2 > RESTRICT_ROW_MAX) e->restrict_mode = 0;
   auto sysidx = [](int m) { return m; };
   auto lvl0 = [](const Member& M) { return nblk(M.s->geo[0].size); };
   auto one = [](const Member&) { return 1; };
@@ -476,10 +489,9 @@ int engine_build_items(Engine* e) {
               [l, L](int m) { return m * L + l; });
     add_stage(e->st_restrict[l],
               [l, L](const Member& M) { return l < M.s->coarse_from && l + 1 < L; },
-              [l, L, rows](const Member& M) {
+              [l, L, e](const Member& M) {
                 if (l + 1 >= L) return 0;
-                const Geo& gc = M.s->geo[l + 1];
-                return rows ? gc.n0 * gc.n1 : rblk(gc.size);
+                return restrict_blocks(e, M.s->geo[l + 1]);
               },
               [l, L](int m) { return m * L + l; });
   }
@@ -573,11 +585,6 @@ inline Stage stage_of(const Engine* e, const Engine::StageRef& r) {
   return s;
 }
 inline int grid_of(const Engine::StageRef& r) { return std::min(r.nblocks, g_max_grid); }
-// virtual blocks of one system's restriction into the coarse level gc, for the
-// kernel the engine's current layout selects
-inline int restrict_blocks(const Engine* e, const Geo& gc) {
-  return e->restrict_rows ? gc.n0 * gc.n1 : rblk(gc.size);
-}
 
 // ---------------------------------------------------------------------------
 // iteration schedule, templated on hierarchy type T, x type TX and dimension
@@ -607,14 +614,21 @@ struct Sched {
                          levels(e), stage_of(e, r));
     if (probe) probe_end(e);
   }
-  static void restrict_(Engine* e, const Engine::StageRef& r) {
+  // fine: the level's slot stage (the residual pass), r: the restriction stage
+  static void restrict_(Engine* e, const Engine::StageRef& fine, const Engine::StageRef& r) {
     if (r.nblocks == 0) return;
-    if (e->restrict_rows)
+    if (e->restrict_mode == 2) {
+      hipLaunchKernelGGL((k_residual<T, DIM>), dim3(grid_of(fine)), dim3(BLOCK), 0, e->stream,
+                         levels(e), stage_of(e, fine));
+      hipLaunchKernelGGL((k_restrict_gather<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0,
+                         e->stream, levels(e), stage_of(e, r));
+    } else if (e->restrict_mode == 1) {
       hipLaunchKernelGGL((k_restrict_rows<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
                          levels(e), stage_of(e, r));
-    else
+    } else {
       hipLaunchKernelGGL((k_restrict<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
                          levels(e), stage_of(e, r));
+    }
   }
   template <int OP>
   static void fin(Engine* e, const Engine::StageRef& r) {
@@ -642,7 +656,7 @@ struct Sched {
       } else {
         stage<ST_ZERO>(e, S, l == 0);
       }
-      restrict_(e, e->st_restrict[l]);
+      restrict_(e, e->st_slot[l], e->st_restrict[l]);
     }
     // bottom level on the batched kernels (only when it is too big for k_coarse)
     const Engine::StageRef& B = e->st_bottom[0];
@@ -988,7 +1002,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     return o;
   };
   struct Offs {
-    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], meta;
+    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta;
   };
   std::vector<Offs> lo;
   s->coarse_from = n_levels;
@@ -1006,6 +1020,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       o.r[c] = take(g.size * s->esz);
       o.z[c] = take(g.size * s->esz);
       o.nb[c] = take(g.size);
+      o.res[c] = (l + 1 < n_levels) ? take(g.size * s->esz) : (size_t)-1;
     }
     o.meta = take(sizeof(LevelMeta));
     lo.push_back(o);
@@ -1041,6 +1056,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       B.r[c] = s->pool + lo[l].r[c];
       B.z[c] = s->pool + lo[l].z[c];
       B.nb[c] = (uint8_t*)(s->pool + lo[l].nb[c]);
+      B.res[c] = (l + 1 < n_levels) ? (void*)(s->pool + lo[l].res[c]) : nullptr;
     }
     B.meta = (LevelMeta*)(s->pool + lo[l].meta);
     s->lv.push_back(B);
@@ -1110,7 +1126,8 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
       g_coarse_lds = value != 0;
       return 0;
     case MGPCG_OPT_RESTRICT_ROWS:
-      g_restrict_rows = value != 0;
+      MG_REQUIRE(value >= 0 && value <= 2, "restriction mode must be 0, 1 or 2");
+      g_restrict_mode = value;
       return 0;
     case MGPCG_OPT_PROBE_EVERY:
       MG_REQUIRE(value >= 1, "probe period must be >= 1");
@@ -1298,7 +1315,7 @@ struct RefPath {
           Sd::template stage<ST_SMOOTH1>(e, lvl(l, false));
         }
         fill(l + 1, 3);  // r[l+1].fill(0); z[l+1].fill(0)
-        Sd::restrict_(e, lvl(l, true));
+        Sd::restrict_(e, lvl(l, false), lvl(l, true));
       }
       const int half = s->bottom / 2;
       for (int i = 0; i < half; ++i) {
@@ -1473,8 +1490,10 @@ int prepare_op(Engine* e, const mgpcg_solver* s, int op, int level, Engine::Stag
   it.block0 = 0;
   int nb = nblk(s->geo[level].size);
   if (op == MGPCG_OP_RESTRICT) {
-    e->restrict_rows = g_restrict_rows && s->geo[0].n2 <= RESTRICT_ROW_MAX;
+    e->restrict_mode = g_restrict_mode;
+    if (e->restrict_mode == 1 && s->geo[0].n2 > RESTRICT_ROW_MAX) e->restrict_mode = 0;
     nb = restrict_blocks(e, s->geo[level + 1]);
+    r->fine_nblocks = nblk(s->geo[level].size);
   }
   if (op == MGPCG_OP_COMPUTE_AP) it.desc = 0;  // SysDesc of member 0
   MG_HIP(hipStreamSynchronize(e->stream));
@@ -1503,9 +1522,12 @@ struct RunOp {
       case MGPCG_OP_PROLONG_SMOOTH1:
         Sd::template stage<ST_PROLONG_SMOOTH1>(e, r);
         return 0;
-      case MGPCG_OP_RESTRICT:
-        Sd::restrict_(e, r);
+      case MGPCG_OP_RESTRICT: {
+        Engine::StageRef fine = r;
+        fine.nblocks = r.fine_nblocks;
+        Sd::restrict_(e, fine, r);
         return 0;
+      }
       case MGPCG_OP_PROLONGATE:
         Sd::template stage<ST_PROLONGATE>(e, r);
         return 0;

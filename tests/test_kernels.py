@@ -133,11 +133,11 @@ def test_prolong_smooth1_bit_exact(dev, case):
         assert np.array_equal(got[upd], zr[upd]), f"level {l}"
 
 
-@pytest.mark.parametrize("rows", [1, 0], ids=["rows", "children"])
+@pytest.mark.parametrize("rows", [2, 1, 0], ids=["gather", "rows", "children"])
 @pytest.mark.parametrize("coupling", [None, -0.75], ids=["random", "uniform"])
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 def test_restrict_bit_exact(dev, case, coupling, rows):
-    """Both restriction kernels (row-tiled, child-parallel), with stored and
+    """Every restriction kernel (residual + gather, row-tiled, child-parallel), with stored and
     with uniform (flag-byte) couplings, against the oracle bit for bit."""
     import _lib
 
@@ -162,7 +162,7 @@ def test_restrict_bit_exact(dev, case, coupling, rows):
             ref = core.restrict(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z)
             assert np.array_equal(got, ref), f"level {l}"
     finally:
-        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_RESTRICT_ROWS, 0))
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_RESTRICT_ROWS, 2))
 
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)

@@ -101,7 +101,7 @@ def test_reference_structured_equals_device_loop(dev):
 
 
 KNOB_DEFAULTS = {"coarse_max": 1024, "max_grid": 2048, "fuse_first": 0, "fuse_prolong": 1,
-                 "fuse_p": 0, "compact": 1, "coarse_lds": 1, "restrict_rows": 0}
+                 "fuse_p": 0, "compact": 1, "coarse_lds": 1, "restrict_rows": 2}
 
 
 def set_knobs(**kw):
@@ -124,6 +124,7 @@ def set_knobs(**kw):
     {"compact": 0}, {"compact": 0, "fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1},
     {"coarse_lds": 0}, {"coarse_lds": 0, "compact": 0, "fuse_prolong": 1},
     {"restrict_rows": 1}, {"restrict_rows": 1, "compact": 0, "max_grid": 8},
+    {"restrict_rows": 0}, {"restrict_rows": 0, "compact": 0, "fuse_prolong": 0},
 ], ids=lambda k: "-".join(f"{a}{b}" for a, b in k.items()))
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
 def test_schedule_knobs_agree(dev, knobs, precision):
