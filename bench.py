@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--seeds", type=int, default=None, help="distinct scenes cycled through")
     ap.add_argument("--concurrency", type=int, default=None, help="requests in flight")
     ap.add_argument("--precision", default="mixed", choices=["plain", "mixed", "f64"])
-    ap.add_argument("--admit-per-step", type=int, default=2,
+    ap.add_argument("--admit-per-step", type=int, default=0,
                     help="admissions per server step (0: no limit)")
     ap.add_argument("--server-depth", type=int, default=2,
                     help="iterations the server keeps queued on the device ahead of the host")
