@@ -34,6 +34,9 @@ namespace mg {
 
 constexpr int BLOCK = 256;
 constexpr int COARSE_THREADS = 1024;
+// slots per lane of the half-sweep stages: a virtual block of a half-sweep
+// covers BLOCK * SLOT_LANES slots
+constexpr int SLOT_LANES = 2;
 
 // Nonzero-coupling summary of a level, accumulated on the device while the
 // level is packed: the smallest and largest FP32 bit pattern of its nonzero
@@ -385,6 +388,168 @@ __device__ __forceinline__ T smooth_slot_g(const Geo& g, const LevelPtrs<T>& P,
   return zn;
 }
 
+// Two-phase form of a half-sweep slot: every value a slot needs is loaded
+// first (smooth_load), the update is computed and stored afterwards
+// (smooth_store).  A lane that loads several slots before storing any keeps
+// all their loads in flight at once (a store could alias a later load, so
+// the compiler would not reorder them by itself).
+template <typename T, int DIM, bool COMPACT>
+struct SmoothVals {
+  int gt;
+  uint32_t f;  // coupling flags (COMPACT)
+  T rv, ad;
+  T zl[DIM], zh[DIM], al[DIM], ah[DIM];  // al / ah: stored couplings (not COMPACT)
+};
+
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void smooth_load(const Geo& g, const LevelPtrs<T>& P, int c, int t,
+                                            SmoothVals<T, DIM, COMPACT>& v) {
+  const int q = row_q(g, t, c);
+  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
+  v.gt = ldg(P.gt[c] + t);
+  v.rv = ldg(P.r[c] + t);
+  v.ad = ldg(P.Adiag[c] + t);
+  const T* zo = P.z[c ^ 1];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    v.zl[a] = ldg(zo + n.lo[a]);
+    v.zh[a] = ldg(zo + n.hi[a]);
+  }
+  if constexpr (COMPACT) {
+    v.f = ldg(P.nb[c] + t);
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      v.al[a] = ldg(P.Ax[a][c ^ 1] + n.lo[a]);
+      v.ah[a] = ldg(P.Ax[a][c] + t);
+    }
+  }
+}
+
+// keeps a slot's loaded values live here (see MG_KEEP)
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void smooth_keep(const SmoothVals<T, DIM, COMPACT>& v) {
+  MG_KEEP(v.gt);
+  if constexpr (COMPACT) {
+    MG_KEEP(v.f);
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      MG_KEEP(v.al[a]);
+      MG_KEEP(v.ah[a]);
+    }
+  }
+}
+
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void smooth_store(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
+                                             int t, const SmoothVals<T, DIM, COMPACT>& v) {
+  if (v.gt != MG_FLUID) return;
+  T al[DIM], ah[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    if constexpr (COMPACT) {
+      al[a] = ((v.f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
+      ah[a] = ((v.f >> a) & 1u) ? k.coef : (T)0;
+    } else {
+      al[a] = v.al[a];
+      ah[a] = v.ah[a];
+    }
+  }
+  T sm = nsum_vals<T, DIM>(al, ah, v.zl, v.zh);
+  T num = v.rv - sm;
+  stg(P.z[c] + t, num / v.ad);
+}
+
+// SLOT_LANES slots per lane from t0 (stride BLOCK), all loaded before any is stored
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void smooth_slots(const Geo& g, const LevelPtrs<T>& P,
+                                             const Coupling<T>& k, int c, int t0) {
+  SmoothVals<T, DIM, COMPACT> v[SLOT_LANES];
+#pragma unroll
+  for (int u = 0; u < SLOT_LANES; ++u) {
+    const int t = t0 + u * BLOCK;
+    smooth_load<T, DIM, COMPACT>(g, P, c, t < g.size ? t : g.size - 1, v[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < SLOT_LANES; ++u) smooth_keep<T, DIM, COMPACT>(v[u]);
+#pragma unroll
+  for (int u = 0; u < SLOT_LANES; ++u) {
+    const int t = t0 + u * BLOCK;
+    if (t < g.size) smooth_store<T, DIM, COMPACT>(P, k, c, t, v[u]);
+  }
+}
+
+// Two-phase form of A v at one slot of colour c (both colours of a slot are
+// loaded before either result is stored): own value, diagonal, neighbours
+// and couplings.
+template <typename T, int DIM, bool COMPACT>
+struct StencilVals {
+  int gt;
+  uint32_t f;
+  T own, ad;
+  T vl[DIM], vh[DIM], al[DIM], ah[DIM];
+};
+
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void stencil_load(const Geo& g, const LevelPtrs<T>& P, const T* const* v,
+                                             int c, int t, StencilVals<T, DIM, COMPACT>& s) {
+  const int q = row_q(g, t, c);
+  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
+  s.gt = ldg(P.gt[c] + t);
+  s.own = ldg(v[c] + t);
+  s.ad = ldg(P.Adiag[c] + t);
+  const T* vo = v[c ^ 1];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    s.vl[a] = ldg(vo + n.lo[a]);
+    s.vh[a] = ldg(vo + n.hi[a]);
+  }
+  if constexpr (COMPACT) {
+    s.f = ldg(P.nb[c] + t);
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      s.al[a] = ldg(P.Ax[a][c ^ 1] + n.lo[a]);
+      s.ah[a] = ldg(P.Ax[a][c] + t);
+    }
+  }
+}
+
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void stencil_keep(const StencilVals<T, DIM, COMPACT>& s) {
+  MG_KEEP(s.gt);
+  if constexpr (COMPACT) {
+    MG_KEEP(s.f);
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      MG_KEEP(s.al[a]);
+      MG_KEEP(s.ah[a]);
+    }
+  }
+}
+
+// Adiag * own + neighbour_sum, in the reference order (compute_Ap, mgpcg.py:218-224)
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ T stencil_apply(const Coupling<T>& k,
+                                           const StencilVals<T, DIM, COMPACT>& s) {
+  T al[DIM], ah[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    if constexpr (COMPACT) {
+      al[a] = ((s.f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
+      ah[a] = ((s.f >> a) & 1u) ? k.coef : (T)0;
+    } else {
+      al[a] = s.al[a];
+      ah[a] = s.ah[a];
+    }
+  }
+  T Av = s.ad * s.own;
+  T ns = nsum_vals<T, DIM>(al, ah, s.vl, s.vh);
+  return Av + ns;
+}
+
 // prolongate(l); smooth(l, 1) at colour-1 slot t in the latency-oriented form
 // (same arithmetic as prolong_smooth1_slot below)
 template <typename T, int DIM>
@@ -650,13 +815,16 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
       if (S.swept != nullptr && it.vb == ic.block0 && threadIdx.x == 0)
         atomicAdd(S.swept, (unsigned long long)D.g.ncells);
     }
+    if constexpr (OP == ST_SMOOTH0 || OP == ST_SMOOTH1) {
+      constexpr int c = OP == ST_SMOOTH0 ? 0 : 1;
+      const int t0 = (it.vb - ic.block0) * (BLOCK * SLOT_LANES) + threadIdx.x;
+      if (kc.uniform) smooth_slots<T, DIM, true>(D.g, D.P, kc, c, t0);
+      else smooth_slots<T, DIM, false>(D.g, D.P, kc, c, t0);
+      continue;
+    }
     const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
-    if (OP == ST_SMOOTH0) {
-      (void)smooth_slot_g<T, DIM>(D.g, D.P, kc, 0, t);
-    } else if (OP == ST_SMOOTH1) {
-      (void)smooth_slot_g<T, DIM>(D.g, D.P, kc, 1, t);
-    } else if (OP == ST_FIRST) {
+    if (OP == ST_FIRST) {
       // z.fill(0); smooth(l, 0) on colour 0: z0 = r0 / Adiag0 on FLUID, else 0
       const int8_t gt = ldg(D.P.gt[0] + t);
       const T rv = ldg(D.P.r[0] + t);
@@ -760,6 +928,26 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __r
   }
 }
 
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void residual_slot(const LevelDesc<T>& D, const Coupling<T>& k,
+                                              int t) {
+  StencilVals<T, DIM, COMPACT> s[2];
+  T rv[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.P.z, c, t, s[c]);
+    rv[c] = ldg(D.P.r[c] + t);
+  }
+  stencil_keep<T, DIM, COMPACT>(s[0]);
+  stencil_keep<T, DIM, COMPACT>(s[1]);
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    T v = (T)0;
+    if (s[c].gt == MG_FLUID) v = rv[c] - stencil_apply<T, DIM, COMPACT>(k, s[c]);
+    stg(D.P.res[c] + t, v);
+  }
+}
+
 // restrict(l) (mgpcg.py:111-118) in two coalesced passes.  Pass 1: the
 // residual r - (Adiag z + neighbour_sum) of every FLUID cell, 0 elsewhere,
 // into the level's scratch arrays, both colours of a slot per thread.
@@ -779,31 +967,8 @@ __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restri
     const LevelDesc<T>& D = descs[ic.desc];
     const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int q = row_q(D.g, t, c);
-      const Nbr<DIM> n = neighbours_clamped<DIM>(D.g, t, q);
-      const int8_t gt = ldg(D.P.gt[c] + t);
-      const T zv = ldg(D.P.z[c] + t);
-      const T ad = ldg(D.P.Adiag[c] + t);
-      const T rv = ldg(D.P.r[c] + t);
-      const T* zo = D.P.z[c ^ 1];
-      T zl[DIM], zh[DIM], al[DIM], ah[DIM];
-#pragma unroll
-      for (int a = 0; a < DIM; ++a) {
-        zl[a] = ldg(zo + n.lo[a]);
-        zh[a] = ldg(zo + n.hi[a]);
-      }
-      couplings_g<T, DIM>(D.P, kc, c, t, n, al, ah);
-      T v = (T)0;
-      if (gt == MG_FLUID) {
-        T Az = ad * zv;
-        T ns = nsum_vals<T, DIM>(al, ah, zl, zh);
-        Az = Az + ns;
-        v = rv - Az;
-      }
-      stg(D.P.res[c] + t, v);
-    }
+    if (kc.uniform) residual_slot<T, DIM, true>(D, kc, t);
+    else residual_slot<T, DIM, false>(D, kc, t);
   }
 }
 
@@ -1171,11 +1336,20 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
+    int gts[2];
+    T polds[2], zvs[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {  // both colours loaded before either is stored
+      gts[c] = ldg(D.P.gt[c] + t);
+      polds[c] = ldg(D.p[par][c] + t);
+      zvs[c] = ldg(D.P.z[c] + t);
+    }
+    MG_KEEP(gts[0]);
+    MG_KEEP(gts[1]);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      const int8_t gt = ldg(D.P.gt[c] + t);
-      const T pold = ldg(D.p[par][c] + t);
-      const T zv = ldg(D.P.z[c] + t);
+      const int gt = gts[c];
+      const T pold = polds[c], zv = zvs[c];
       if (gt != MG_FLUID) continue;
       T pv;
       if (plain) {
@@ -1187,6 +1361,26 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
       stg(D.p[par ^ 1][c] + t, pv);
     }
   }
+}
+
+// compute_Ap at both colours of slot t: Ap stored, p.Ap accumulated
+template <typename T, typename TX, int DIM, bool COMPACT>
+__device__ __forceinline__ DD ap_slot(const SysDesc<T, TX>& D, const Coupling<T>& k, int par,
+                                      int t) {
+  StencilVals<T, DIM, COMPACT> s[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) stencil_load<T, DIM, COMPACT>(D.g, D.P, D.p[par], c, t, s[c]);
+  stencil_keep<T, DIM, COMPACT>(s[0]);
+  stencil_keep<T, DIM, COMPACT>(s[1]);
+  DD acc = dd_zero();
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (s[c].gt != MG_FLUID) continue;
+    const T ap = stencil_apply<T, DIM, COMPACT>(k, s[c]);
+    stg(D.Ap[c] + t, ap);
+    acc = dd_add(acc, dd_prod(s[c].own, ap));
+  }
+  return acc;
 }
 
 // compute_Ap (mgpcg.py:218-224) on the p written by k_cg_update_p, partial p.Ap.
@@ -1210,28 +1404,8 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
     const int t = lb * BLOCK + threadIdx.x;
     DD acc = dd_zero();
     if (t < D.g.size) {
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int q = row_q(D.g, t, c);
-        const Nbr<DIM> n = neighbours_clamped<DIM>(D.g, t, q);
-        const int8_t gt = ldg(D.P.gt[c] + t);
-        const T pv = ldg(D.p[par][c] + t);
-        const T ad = ldg(D.P.Adiag[c] + t);
-        const T* po = D.p[par][c ^ 1];
-        T vl[DIM], vh[DIM], al[DIM], ah[DIM];
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) {
-          vl[a] = ldg(po + n.lo[a]);
-          vh[a] = ldg(po + n.hi[a]);
-        }
-        couplings_g<T, DIM>(D.P, kc, c, t, n, al, ah);
-        if (gt != MG_FLUID) continue;
-        T rr = ad * pv;
-        T ns = nsum_vals<T, DIM>(al, ah, vl, vh);
-        T ap = rr + ns;
-        stg(D.Ap[c] + t, ap);
-        acc = dd_add(acc, dd_prod(pv, ap));
-      }
+      if (kc.uniform) acc = ap_slot<T, TX, DIM, true>(D, kc, par, t);
+      else acc = ap_slot<T, TX, DIM, false>(D, kc, par, t);
     }
     DD s = block_sum_dd<BLOCK>(acc);
     if (threadIdx.x == 0) D.partials[lb] = s;
@@ -1265,13 +1439,24 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     const int t = lb * BLOCK + threadIdx.x;
     DD acc = dd_zero();
     if (t < D.g.size) {
+      int gts[2];
+      T pvs[2], apvs[2], rs[2];
+      TX xvs[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {  // both colours loaded before either is stored
+        gts[c] = ldg(D.P.gt[c] + t);
+        pvs[c] = ldg(D.p[par][c] + t);
+        apvs[c] = ldg(D.Ap[c] + t);
+        xvs[c] = ldg(D.x[c] + t);
+        rs[c] = ldg(D.P.r[c] + t);
+      }
+      MG_KEEP(gts[0]);
+      MG_KEEP(gts[1]);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        const int8_t gt = ldg(D.P.gt[c] + t);
-        const T pv = ldg(D.p[par][c] + t);
-        const T apv = ldg(D.Ap[c] + t);
-        const TX xv = ldg(D.x[c] + t);
-        const T r0 = ldg(D.P.r[c] + t);
+        const int gt = gts[c];
+        const T pv = pvs[c], apv = apvs[c], r0 = rs[c];
+        const TX xv = xvs[c];
         if (gt != MG_FLUID) continue;
         if (prec == PLAIN) {
           T tp = (T)a32 * pv;

@@ -95,6 +95,8 @@ struct mgpcg_solver {
 namespace {
 
 inline int nblk(int n) { return (n + BLOCK - 1) / BLOCK; }
+// virtual blocks of a half-sweep stage over n slots
+inline int nsweep(int n) { return (n + BLOCK * SLOT_LANES - 1) / (BLOCK * SLOT_LANES); }
 inline int rblk(int coarse_slots) {
   return (coarse_slots + RESTRICT_SLOTS_PER_BLOCK - 1) / RESTRICT_SLOTS_PER_BLOCK;
 }
@@ -201,6 +203,8 @@ struct Engine {
   bool coarse_lds = false;  // every coarse item's z fits the coarse kernel's LDS
   int restrict_mode = 0;  // restriction stages are laid out for this g_restrict_mode
   std::vector<StageRef> st_slot, st_restrict, st_bottom;  // per level
+  // the half-sweep stages: virtual blocks of BLOCK * SLOT_LANES slots
+  std::vector<StageRef> st_sweep, st_bottom_sweep;
   int n_items_cap = 0;
   Item* d_items = nullptr;
   static const int RING = 4;
@@ -386,6 +390,8 @@ int engine_configure(Engine* e, const mgpcg_solver* s) {
   e->level_sz = level_desc_size(e->precision);
   e->sys_sz = sys_desc_size(e->precision);
   e->st_slot.assign(e->L, {});
+  e->st_sweep.assign(e->L, {});
+  e->st_bottom_sweep.assign(1, {});
   e->st_restrict.assign(e->L, {});
   e->st_bottom.assign(1, {});
   return 0;
@@ -487,6 +493,9 @@ int engine_build_items(Engine* e) {
     add_stage(e->st_slot[l], [l](const Member& M) { return l < M.s->coarse_from; },
               [l](const Member& M) { return nblk(M.s->geo[l].size); },
               [l, L](int m) { return m * L + l; });
+    add_stage(e->st_sweep[l], [l](const Member& M) { return l < M.s->coarse_from; },
+              [l](const Member& M) { return nsweep(M.s->geo[l].size); },
+              [l, L](int m) { return m * L + l; });
     add_stage(e->st_restrict[l],
               [l, L](const Member& M) { return l < M.s->coarse_from && l + 1 < L; },
               [l, L, e](const Member& M) {
@@ -498,6 +507,9 @@ int engine_build_items(Engine* e) {
   // members whose bottom level is too large for the coarse kernel
   add_stage(e->st_bottom[0], [L](const Member& M) { return M.s->coarse_from >= L; },
             [L](const Member& M) { return nblk(M.s->geo[L - 1].size); },
+            [L](int m) { return m * L + L - 1; });
+  add_stage(e->st_bottom_sweep[0], [L](const Member& M) { return M.s->coarse_from >= L; },
+            [L](const Member& M) { return nsweep(M.s->geo[L - 1].size); },
             [L](int m) { return m * L + L - 1; });
   // coarse kernel: desc = first coarse level, block0 = number of coarse levels
   e->st_coarse.off = (int)it.size();
@@ -642,16 +654,17 @@ struct Sched {
     const int L = e->L, nu = e->nu;
     for (int l = 0; l + 1 < L; ++l) {
       const Engine::StageRef& S = e->st_slot[l];
+      const Engine::StageRef& W = e->st_sweep[l];
       if (nu > 0) {
         if (g_fuse_first) {
           stage<ST_PAIR_FROM_ZERO>(e, S, l == 0);
         } else {
           stage<ST_FIRST>(e, S, l == 0);
-          stage<ST_SMOOTH1>(e, S, l == 0);
+          stage<ST_SMOOTH1>(e, W, l == 0);
         }
         for (int i = 1; i < nu; ++i) {
-          stage<ST_SMOOTH0>(e, S, l == 0);
-          stage<ST_SMOOTH1>(e, S, l == 0);
+          stage<ST_SMOOTH0>(e, W, l == 0);
+          stage<ST_SMOOTH1>(e, W, l == 0);
         }
       } else {
         stage<ST_ZERO>(e, S, l == 0);
@@ -660,17 +673,18 @@ struct Sched {
     }
     // bottom level on the batched kernels (only when it is too big for k_coarse)
     const Engine::StageRef& B = e->st_bottom[0];
+    const Engine::StageRef& BW = e->st_bottom_sweep[0];
     if (B.nblocks) {
       const int half = e->bottom / 2;
       if (half > 0) {
         stage<ST_PAIR_FROM_ZERO>(e, B);
         for (int i = 1; i < half; ++i) {
-          stage<ST_SMOOTH0>(e, B);
-          stage<ST_SMOOTH1>(e, B);
+          stage<ST_SMOOTH0>(e, BW);
+          stage<ST_SMOOTH1>(e, BW);
         }
         for (int i = 0; i < half; ++i) {
-          stage<ST_SMOOTH1>(e, B);
-          stage<ST_SMOOTH0>(e, B);
+          stage<ST_SMOOTH1>(e, BW);
+          stage<ST_SMOOTH0>(e, BW);
         }
       } else {
         stage<ST_ZERO>(e, B);
@@ -693,17 +707,18 @@ struct Sched {
     }
     for (int l = L - 2; l >= 0; --l) {
       const Engine::StageRef& S = e->st_slot[l];
+      const Engine::StageRef& W = e->st_sweep[l];
       if (nu > 0) {
         if (g_fuse_prolong) {
           stage<ST_PROLONG_SMOOTH1>(e, S, l == 0);
         } else {
           stage<ST_PROLONGATE0>(e, S, l == 0);
-          stage<ST_SMOOTH1>(e, S, l == 0);
+          stage<ST_SMOOTH1>(e, W, l == 0);
         }
         for (int i = 0; i < nu; ++i) {
-          if (i > 0) stage<ST_SMOOTH1>(e, S, l == 0);
+          if (i > 0) stage<ST_SMOOTH1>(e, W, l == 0);
           const bool last = (i == nu - 1);
-          if (!(last && l == 0)) stage<ST_SMOOTH0>(e, S, l == 0);
+          if (!(last && l == 0)) stage<ST_SMOOTH0>(e, W, l == 0);
         }
       } else {
         stage<ST_PROLONGATE>(e, S, l == 0);
@@ -1290,6 +1305,13 @@ struct RefPath {
       r.nblocks = coarse ? restrict_blocks(e, s->geo[l + 1]) : nblk(s->geo[l].size);
       return r;
     };
+    auto sweep = [&](int l) {  // a half-sweep stage of level l
+      Engine::StageRef r;
+      r.off = l;
+      r.n = 1;
+      r.nblocks = nsweep(s->geo[l].size);
+      return r;
+    };
     auto fill = [&](int l, int which) {
       hipLaunchKernelGGL((k_ref_fill<T>), dim3(nblk(s->geo[l].size)), dim3(BLOCK), 0, e->stream,
                          lv, l, which);
@@ -1311,26 +1333,26 @@ struct RefPath {
       fill(0, 1);  // z[0].fill(0)
       for (int l = 0; l + 1 < L; ++l) {
         for (int i = 0; i < nu; ++i) {
-          Sd::template stage<ST_SMOOTH0>(e, lvl(l, false));
-          Sd::template stage<ST_SMOOTH1>(e, lvl(l, false));
+          Sd::template stage<ST_SMOOTH0>(e, sweep(l));
+          Sd::template stage<ST_SMOOTH1>(e, sweep(l));
         }
         fill(l + 1, 3);  // r[l+1].fill(0); z[l+1].fill(0)
         Sd::restrict_(e, lvl(l, false), lvl(l, true));
       }
       const int half = s->bottom / 2;
       for (int i = 0; i < half; ++i) {
-        Sd::template stage<ST_SMOOTH0>(e, lvl(L - 1, false));
-        Sd::template stage<ST_SMOOTH1>(e, lvl(L - 1, false));
+        Sd::template stage<ST_SMOOTH0>(e, sweep(L - 1));
+        Sd::template stage<ST_SMOOTH1>(e, sweep(L - 1));
       }
       for (int i = 0; i < half; ++i) {
-        Sd::template stage<ST_SMOOTH1>(e, lvl(L - 1, false));
-        Sd::template stage<ST_SMOOTH0>(e, lvl(L - 1, false));
+        Sd::template stage<ST_SMOOTH1>(e, sweep(L - 1));
+        Sd::template stage<ST_SMOOTH0>(e, sweep(L - 1));
       }
       for (int l = L - 2; l >= 0; --l) {
         Sd::template stage<ST_PROLONGATE>(e, lvl(l, false));
         for (int i = 0; i < nu; ++i) {
-          Sd::template stage<ST_SMOOTH1>(e, lvl(l, false));
-          Sd::template stage<ST_SMOOTH0>(e, lvl(l, false));
+          Sd::template stage<ST_SMOOTH1>(e, sweep(l));
+          Sd::template stage<ST_SMOOTH0>(e, sweep(l));
         }
       }
     };
@@ -1489,6 +1511,7 @@ int prepare_op(Engine* e, const mgpcg_solver* s, int op, int level, Engine::Stag
   it.desc = level;  // member 0: LevelDesc index = level
   it.block0 = 0;
   int nb = nblk(s->geo[level].size);
+  if (op == MGPCG_OP_SMOOTH) nb = nsweep(s->geo[level].size);
   if (op == MGPCG_OP_RESTRICT) {
     e->restrict_mode = g_restrict_mode;
     if (e->restrict_mode == 1 && s->geo[0].n2 > RESTRICT_ROW_MAX) e->restrict_mode = 0;
