@@ -362,6 +362,32 @@ __device__ __forceinline__ bool item_at(const Stage& S, int vb, ItemCache& c) {
   return true;
 }
 
+// Dot-product partials of one workgroup over a run of its virtual blocks in
+// one system: summed in registers and reduced over the workgroup once per run
+// (instead of once per virtual block) into the partial of the run's first
+// virtual block; the run's other virtual blocks get a zero partial, so every
+// partial of a system is written exactly once per launch.  run_mark and
+// run_flush are called by the whole workgroup (block-uniform control flow).
+struct RunSum {
+  DD acc = dd_zero();
+  DD* dst = nullptr;  // receives the run's total
+};
+__device__ __forceinline__ void stg_dd(DD* p, DD v) {
+  stg(&p->hi, v.hi);
+  stg(&p->lo, v.lo);
+}
+__device__ __forceinline__ void run_mark(RunSum& r, DD* partials, int lb) {
+  if (r.dst == nullptr) r.dst = partials + lb;
+  else if (threadIdx.x == 0) stg_dd(partials + lb, dd_zero());
+}
+__device__ __forceinline__ void run_flush(RunSum& r) {
+  if (r.dst == nullptr) return;
+  const DD t = block_sum_dd<BLOCK>(r.acc);
+  if (threadIdx.x == 0) stg_dd(r.dst, t);
+  r.acc = dd_zero();
+  r.dst = nullptr;
+}
+
 // smooth(l, c) at slot t in the latency-oriented form; returns the slot's z
 // after the half-sweep (the new value on a FLUID cell)
 template <typename T, int DIM>
@@ -931,19 +957,14 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __r
 template <typename T, int DIM, bool COMPACT>
 __device__ __forceinline__ void residual_slot(const LevelDesc<T>& D, const Coupling<T>& k,
                                               int t) {
-  StencilVals<T, DIM, COMPACT> s[2];
-  T rv[2];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.P.z, c, t, s[c]);
-    rv[c] = ldg(D.P.r[c] + t);
-  }
-  stencil_keep<T, DIM, COMPACT>(s[0]);
-  stencil_keep<T, DIM, COMPACT>(s[1]);
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < 2; ++c) {  // one colour after the other
+    StencilVals<T, DIM, COMPACT> s;
+    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.P.z, c, t, s);
+    const T rv = ldg(D.P.r[c] + t);
+    stencil_keep<T, DIM, COMPACT>(s);
     T v = (T)0;
-    if (s[c].gt == MG_FLUID) v = rv[c] - stencil_apply<T, DIM, COMPACT>(k, s[c]);
+    if (s.gt == MG_FLUID) v = rv - stencil_apply<T, DIM, COMPACT>(k, s);
     stg(D.P.res[c] + t, v);
   }
 }
@@ -1241,13 +1262,14 @@ __device__ __forceinline__ double rnd_plain(const CGState* st, double v) {
 template <typename T, typename TX>
 __global__ void __launch_bounds__(BLOCK) k_cg_init(const SysDesc<T, TX>* __restrict__ sys,
                                                    Stage S) {
+  ItemCache ic;
+  RunSum rs;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const SysDesc<T, TX>& D = sys[item.desc];
-    const int lb = it.vb - item.block0;
+    if (item_at(S, it.vb, ic)) run_flush(rs);
+    const SysDesc<T, TX>& D = sys[ic.desc];
+    const int lb = it.vb - ic.block0;
+    run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
-    DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -1257,12 +1279,11 @@ __global__ void __launch_bounds__(BLOCK) k_cg_init(const SysDesc<T, TX>* __restr
         D.p[0][c][t] = (T)0;
         D.p[1][c][t] = (T)0;
         D.Ap[c][t] = (T)0;
-        if (D.P.gt[c][t] == MG_FLUID) acc = dd_add(acc, dd_prod(bv, bv));
+        if (D.P.gt[c][t] == MG_FLUID) rs.acc = dd_add(rs.acc, dd_prod(bv, bv));
       }
     }
-    DD s = block_sum_dd<BLOCK>(acc);
-    if (threadIdx.x == 0) D.partials[lb] = s;
   }
+  run_flush(rs);
 }
 
 // p = z + beta p (update_p, mgpcg.py:234-238) into the other parity buffer,
@@ -1270,13 +1291,15 @@ __global__ void __launch_bounds__(BLOCK) k_cg_init(const SysDesc<T, TX>* __restr
 template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_cg_ap(const SysDesc<T, TX>* __restrict__ sys,
                                                  Stage S) {
+  ItemCache ic;
+  RunSum rs;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const SysDesc<T, TX>& D = sys[item.desc];
+    if (item_at(S, it.vb, ic)) run_flush(rs);
+    const SysDesc<T, TX>& D = sys[ic.desc];
     const CGState* st = D.st;
     if (st->done) continue;
-    const int lb = it.vb - item.block0;
+    const int lb = it.vb - ic.block0;
+    run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
     const int par = st->iter & 1;
     const T* pold[2] = {D.p[par][0], D.p[par][1]};
@@ -1293,7 +1316,6 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap(const SysDesc<T, TX>* __restric
       }
       return (T)((double)D.P.z[c][u] + beta * (double)pold[c][u]);
     };
-    DD acc = dd_zero();
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -1306,12 +1328,11 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap(const SysDesc<T, TX>* __restric
         T ns = nsum<T, DIM>(D.P, c, t, n, [&](int u) { return upd(c ^ 1, u); });
         T ap = rr + ns;
         D.Ap[c][t] = ap;
-        acc = dd_add(acc, dd_prod(pv, ap));
+        rs.acc = dd_add(rs.acc, dd_prod(pv, ap));
       }
     }
-    DD s = block_sum_dd<BLOCK>(acc);
-    if (threadIdx.x == 0) D.partials[lb] = s;
   }
+  run_flush(rs);
 }
 
 // update_p alone (mgpcg.py:234-238): p_next = z + beta p on FLUID cells.
@@ -1363,24 +1384,21 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
   }
 }
 
-// compute_Ap at both colours of slot t: Ap stored, p.Ap accumulated
+// compute_Ap at both colours of slot t, one colour after the other: Ap
+// stored, p.Ap added to acc
 template <typename T, typename TX, int DIM, bool COMPACT>
-__device__ __forceinline__ DD ap_slot(const SysDesc<T, TX>& D, const Coupling<T>& k, int par,
-                                      int t) {
-  StencilVals<T, DIM, COMPACT> s[2];
-#pragma unroll
-  for (int c = 0; c < 2; ++c) stencil_load<T, DIM, COMPACT>(D.g, D.P, D.p[par], c, t, s[c]);
-  stencil_keep<T, DIM, COMPACT>(s[0]);
-  stencil_keep<T, DIM, COMPACT>(s[1]);
-  DD acc = dd_zero();
+__device__ __forceinline__ void ap_slot(const SysDesc<T, TX>& D, const Coupling<T>& k, int par,
+                                        int t, DD& acc) {
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    if (s[c].gt != MG_FLUID) continue;
-    const T ap = stencil_apply<T, DIM, COMPACT>(k, s[c]);
+    StencilVals<T, DIM, COMPACT> s;
+    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.p[par], c, t, s);
+    stencil_keep<T, DIM, COMPACT>(s);
+    if (s.gt != MG_FLUID) continue;
+    const T ap = stencil_apply<T, DIM, COMPACT>(k, s);
     stg(D.Ap[c] + t, ap);
-    acc = dd_add(acc, dd_prod(s[c].own, ap));
+    acc = dd_add(acc, dd_prod(s.own, ap));
   }
-  return acc;
 }
 
 // compute_Ap (mgpcg.py:218-224) on the p written by k_cg_update_p, partial p.Ap.
@@ -1388,11 +1406,13 @@ template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __restrict__ sys,
                                                       Stage S) {
   ItemCache ic;
+  RunSum rs;
   bool done = true;
   int par = 0;
   Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
     if (item_at(S, it.vb, ic)) {
+      run_flush(rs);
       const SysDesc<T, TX>& D = sys[ic.desc];
       done = ldg(&D.st->done) != 0;
       par = (ldg(&D.st->iter) & 1) ^ 1;
@@ -1401,15 +1421,14 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
+    run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
-    DD acc = dd_zero();
     if (t < D.g.size) {
-      if (kc.uniform) acc = ap_slot<T, TX, DIM, true>(D, kc, par, t);
-      else acc = ap_slot<T, TX, DIM, false>(D, kc, par, t);
+      if (kc.uniform) ap_slot<T, TX, DIM, true>(D, kc, par, t, rs.acc);
+      else ap_slot<T, TX, DIM, false>(D, kc, par, t, rs.acc);
     }
-    DD s = block_sum_dd<BLOCK>(acc);
-    if (threadIdx.x == 0) D.partials[lb] = s;
   }
+  run_flush(rs);
 }
 
 // update_xr (mgpcg.py:226-232) + partial r.r.  MIXED: x is FP64 and on a
@@ -1418,12 +1437,14 @@ template <typename T, typename TX>
 __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restrict__ sys,
                                                  Stage S) {
   ItemCache ic;
+  RunSum rs;
   bool done = true, refine = false;
   int par = 0, prec = PLAIN;
   float a32 = 0.0f;
   double alpha = 0.0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
     if (item_at(S, it.vb, ic)) {
+      run_flush(rs);
       const CGState* st = sys[ic.desc].st;
       done = ldg(&st->done) != 0;
       par = (ldg(&st->iter) & 1) ^ 1;  // the p written by k_cg_update_p / k_cg_ap
@@ -1436,27 +1457,17 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
+    run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
-    DD acc = dd_zero();
+    DD& acc = rs.acc;
     if (t < D.g.size) {
-      int gts[2];
-      T pvs[2], apvs[2], rs[2];
-      TX xvs[2];
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {  // both colours loaded before either is stored
-        gts[c] = ldg(D.P.gt[c] + t);
-        pvs[c] = ldg(D.p[par][c] + t);
-        apvs[c] = ldg(D.Ap[c] + t);
-        xvs[c] = ldg(D.x[c] + t);
-        rs[c] = ldg(D.P.r[c] + t);
-      }
-      MG_KEEP(gts[0]);
-      MG_KEEP(gts[1]);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        const int gt = gts[c];
-        const T pv = pvs[c], apv = apvs[c], r0 = rs[c];
-        const TX xv = xvs[c];
+        const int8_t gt = ldg(D.P.gt[c] + t);
+        const T pv = ldg(D.p[par][c] + t);
+        const T apv = ldg(D.Ap[c] + t);
+        const TX xv = ldg(D.x[c] + t);
+        const T r0 = ldg(D.P.r[c] + t);
         if (gt != MG_FLUID) continue;
         if (prec == PLAIN) {
           T tp = (T)a32 * pv;
@@ -1482,9 +1493,8 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
         }
       }
     }
-    DD s = block_sum_dd<BLOCK>(acc);
-    if (threadIdx.x == 0) D.partials[lb] = s;
   }
+  run_flush(rs);
 }
 
 // MIXED: r = b - A x with x in FP64, accumulated in FP64, stored as T; partial r.r.
@@ -1492,10 +1502,12 @@ template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* __restrict__ sys,
                                                          Stage S) {
   ItemCache ic;
+  RunSum rs;
   bool skip = true;
   Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
     if (item_at(S, it.vb, ic)) {
+      run_flush(rs);
       const SysDesc<T, TX>& D = sys[ic.desc];
       skip = ldg(&D.st->done) != 0 || ldg(&D.st->pending_true) == 0;
       kc = coupling_g(D.P);
@@ -1503,8 +1515,9 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
     if (skip) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
+    run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
-    DD acc = dd_zero();
+    DD& acc = rs.acc;
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -1540,9 +1553,8 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
         acc = dd_add(acc, dd_prod(r32, r32));
       }
     }
-    DD s = block_sum_dd<BLOCK>(acc);
-    if (threadIdx.x == 0) D.partials[lb] = s;
   }
+  run_flush(rs);
 }
 
 // Last post-smoothing half-sweep of level 0, smooth(0, 0) (mgpcg.py:148),
@@ -1551,10 +1563,12 @@ template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __restrict__ sys,
                                                        Stage S) {
   ItemCache ic;
+  RunSum rs;
   bool done = true;
   Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
     if (item_at(S, it.vb, ic)) {
+      run_flush(rs);
       const SysDesc<T, TX>& D = sys[ic.desc];
       done = ldg(&D.st->done) != 0;
       kc = coupling_g(D.P);
@@ -1562,8 +1576,9 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
+    run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
-    DD acc = dd_zero();
+    DD& acc = rs.acc;
     if (t < D.g.size) {
       const int8_t g0 = ldg(D.P.gt[0] + t);
       const int8_t g1 = ldg(D.P.gt[1] + t);
@@ -1574,9 +1589,8 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
       if (g0 == MG_FLUID) acc = dd_add(acc, dd_prod(z0, r0));
       if (g1 == MG_FLUID) acc = dd_add(acc, dd_prod(z1, r1));
     }
-    DD s = block_sum_dd<BLOCK>(acc);
-    if (threadIdx.x == 0) D.partials[lb] = s;
   }
+  run_flush(rs);
 }
 
 // reduce(z, r) alone (used when level 0 is inside the coarse kernel or gets
@@ -1584,21 +1598,22 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
 template <typename T, typename TX>
 __global__ void __launch_bounds__(BLOCK) k_dot_zr(const SysDesc<T, TX>* __restrict__ sys,
                                                   Stage S) {
+  ItemCache ic;
+  RunSum rs;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    const int ii = find_item(S.items, S.nitems, it.vb);
-    const Item item = S.items[ii];
-    const SysDesc<T, TX>& D = sys[item.desc];
+    if (item_at(S, it.vb, ic)) run_flush(rs);
+    const SysDesc<T, TX>& D = sys[ic.desc];
     if (D.st->done) continue;
-    const int lb = it.vb - item.block0;
+    const int lb = it.vb - ic.block0;
+    run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
-    DD acc = dd_zero();
+    DD& acc = rs.acc;
     if (t < D.g.size) {
       if (D.P.gt[0][t] == MG_FLUID) acc = dd_add(acc, dd_prod(D.P.z[0][t], D.P.r[0][t]));
       if (D.P.gt[1][t] == MG_FLUID) acc = dd_add(acc, dd_prod(D.P.z[1][t], D.P.r[1][t]));
     }
-    DD s = block_sum_dd<BLOCK>(acc);
-    if (threadIdx.x == 0) D.partials[lb] = s;
   }
+  run_flush(rs);
 }
 
 // ---------------------------------------------------------------------------
