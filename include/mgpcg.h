@@ -69,7 +69,7 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
 
 /* Process-wide tuning knobs (no reference counterpart; `solver` is unused). */
 #define MGPCG_OPT_COARSE_MAX_SLOTS 3 /* levels with at most this many slots per colour are
-                                        solved by one workgroup per system (default 1024);
+                                        solved by one workgroup per system (default 4096);
                                         applies to handles created afterwards */
 #define MGPCG_OPT_MAX_GRID 4         /* workgroups of a persistent stage launch (default 2048) */
 #define MGPCG_OPT_FUSE_FIRST 5    /* 1: fill(0)+smooth(0)+smooth(1) in one kernel (default 0) */
@@ -78,8 +78,9 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
 #define MGPCG_OPT_COMPACT 8       /* 1: levels whose nonzero off-diagonals are all one value
                                      read per-cell coupling flags instead of Ax (default 1);
                                      applies to systems solved or submitted afterwards */
-#define MGPCG_OPT_COARSE_LDS 9    /* 1: the coarse-level kernel keeps z in LDS when a system's
-                                     coarse levels fit (FP32 hierarchies; default 1) */
+#define MGPCG_OPT_COARSE_LDS 9    /* 1: the coarse-level kernel runs on a copy of its levels'
+                                     cell types, flags, Adiag, r and z in LDS (160 KB) when
+                                     they fit (FP32 hierarchies; default 1) */
 #define MGPCG_OPT_RESTRICT_ROWS 10 /* restriction kernels: 2 a residual pass into scratch
                                       then a child gather (default), 1 tiled by coarse
                                       rows with the residuals in LDS (fine rows up to 512

@@ -1111,45 +1111,46 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(const LevelDesc<T>* __restri
 // (mgpcg.py:127-148 restricted to those levels), barriers between stages.
 // ---------------------------------------------------------------------------
 template <typename T, int DIM>
-__device__ void coarse_pre(const LevelDesc<T>& D, T* const* z, int nu) {
-  const int n = D.g.size;
+__device__ void coarse_pre(const Geo& g, const LevelPtrs<T>& P, int nu) {
+  const int n = g.size;
   if (nu <= 0) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(z, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(P, t);
     __syncthreads();
     return;
   }
   for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
-    smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, z, t);
+    smooth_pair_from_zero_slot<T, DIM>(g, P, t);
   __syncthreads();
   for (int i = 1; i < nu; ++i) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 0, t);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 1, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 1, t);
     __syncthreads();
   }
 }
 
-// z, zc: this level's and the next coarser level's z colour arrays
+// zc: the next coarser level's z colour arrays
 template <typename T, int DIM>
-__device__ void coarse_post(const LevelDesc<T>& D, T* const* z, T* const* zc, int nu) {
-  const int n = D.g.size;
+__device__ void coarse_post(const Geo& g, const Geo& gc, const LevelPtrs<T>& P,
+                            T* const* zc, int nu) {
+  const int n = g.size;
   if (nu <= 0) {
     for (int t = threadIdx.x; t < n; t += COARSE_THREADS) {
-      prolongate_slot<T>(D.g, D.gc, z, zc[0], zc[1], 0, t);
-      prolongate_slot<T>(D.g, D.gc, z, zc[0], zc[1], 1, t);
+      prolongate_slot<T>(g, gc, P.z, zc[0], zc[1], 0, t);
+      prolongate_slot<T>(g, gc, P.z, zc[0], zc[1], 1, t);
     }
     __syncthreads();
     return;
   }
   for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
-    prolong_smooth1_slot<T, DIM>(D.g, D.gc, D.P, z, zc[0], zc[1], t);
+    prolong_smooth1_slot<T, DIM>(g, gc, P, zc[0], zc[1], t);
   __syncthreads();
-  for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
+  for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 0, t);
   __syncthreads();
   for (int i = 1; i < nu; ++i) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 1, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 1, t);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 0, t);
     __syncthreads();
   }
 }
@@ -1157,40 +1158,49 @@ __device__ void coarse_post(const LevelDesc<T>& D, T* const* z, T* const* zc, in
 // bottom solve (mgpcg.py:136-142): half x (smooth 0, smooth 1) from z = 0,
 // then half x (smooth 1, smooth 0).
 template <typename T, int DIM>
-__device__ void coarse_bottom(const LevelDesc<T>& D, T* const* z, int bottom) {
-  const int n = D.g.size;
+__device__ void coarse_bottom(const Geo& g, const LevelPtrs<T>& P, int bottom) {
+  const int n = g.size;
   const int half = bottom / 2;
   if (half <= 0) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(z, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) zero_slot<T>(P, t);
     __syncthreads();
     return;
   }
   for (int t = threadIdx.x; t < n; t += COARSE_THREADS)
-    smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, z, t);
+    smooth_pair_from_zero_slot<T, DIM>(g, P, t);
   __syncthreads();
   for (int i = 1; i < half; ++i) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 0, t);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 1, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 1, t);
     __syncthreads();
   }
   for (int i = 0; i < half; ++i) {
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 1, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 1, t);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(D.g, D.P, z, 0, t);
+    for (int t = threadIdx.x; t < n; t += COARSE_THREADS) smooth_slot<T, DIM>(g, P, 0, t);
     __syncthreads();
   }
 }
 
-// LDS elements of the z copies of one system's coarse levels (48 KB of FP32)
-constexpr int COARSE_LDS_ELEMS = 12288;
+// LDS image of one system's coarse levels: per level the z, r and Adiag
+// colour arrays, the cell types, the coupling flags and the coupling range.
+// The stored couplings (Ax) of a level without uniform couplings stay in
+// global memory.
+constexpr int COARSE_LDS_BYTES = 160 * 1024;
+__host__ __device__ constexpr int lds_align(int b) { return (b + 15) & ~15; }
+__host__ __device__ constexpr int coarse_image_bytes(int size, int esz) {
+  return 6 * lds_align(size * esz) + 4 * lds_align(size) + lds_align((int)sizeof(LevelMeta));
+}
 
 // One workgroup per item: item.desc = the system's LevelDesc of its first
 // coarse level, item.block0 = number of levels from there to the bottom
-// (a system's LevelDescs are contiguous).  With USE_LDS the z arrays of
-// those levels live in LDS for the whole kernel (the host checks that
-// 2 * sum(level slots) <= COARSE_LDS_ELEMS) and are copied out at the end;
-// r, the operator and the cell types are read from global memory.
+// (a system's LevelDescs are contiguous).  With USE_LDS every per-slot array
+// the V-cycle reads is copied into LDS once (the host checks that the
+// levels' coarse_image_bytes fit COARSE_LDS_BYTES), the V-cycle runs on the
+// copy, and z of every level and the restricted r of the levels below the
+// first are copied back at the end.  The slot functions are the ones the
+// global-memory form uses, on pointers into LDS.
 template <typename T, int DIM, bool USE_LDS>
 __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* __restrict__ descs,
                                                            const Item* __restrict__ items,
@@ -1199,52 +1209,78 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
   const int d0 = items[blockIdx.x].desc;
   const int nlev = items[blockIdx.x].block0;
   if (descs[d0].st->done) return;
-  __shared__ T zs[USE_LDS ? COARSE_LDS_ELEMS : 1];
-  // z colour arrays of coarse level k (offsets are prefix sums of 2 * size)
-  auto zof = [&](int k, T** z) {
+  __shared__ alignas(16) unsigned char img[USE_LDS ? COARSE_LDS_BYTES : 16];
+  // the operator and vectors of coarse level k as the V-cycle sees them
+  auto lv = [&](int k) -> LevelPtrs<T> {
+    LevelPtrs<T> P = descs[d0 + k].P;
     if constexpr (USE_LDS) {
       int off = 0;
-      for (int j = 0; j < k; ++j) off += 2 * descs[d0 + j].g.size;
-      z[0] = zs + off;
-      z[1] = zs + off + descs[d0 + k].g.size;
-    } else {
-      z[0] = descs[d0 + k].P.z[0];
-      z[1] = descs[d0 + k].P.z[1];
+      for (int j = 0; j < k; ++j) off += coarse_image_bytes(descs[d0 + j].g.size, (int)sizeof(T));
+      const int n = descs[d0 + k].g.size;
+      const int vb = lds_align(n * (int)sizeof(T)), bb = lds_align(n);
+      unsigned char* b = img + off;
+      P.z[0] = (T*)(b + 0 * vb);
+      P.z[1] = (T*)(b + 1 * vb);
+      P.r[0] = (T*)(b + 2 * vb);
+      P.r[1] = (T*)(b + 3 * vb);
+      P.Adiag[0] = (const T*)(b + 4 * vb);
+      P.Adiag[1] = (const T*)(b + 5 * vb);
+      P.gt[0] = (const int8_t*)(b + 6 * vb);
+      P.gt[1] = (const int8_t*)(b + 6 * vb + bb);
+      P.nb[0] = (const uint8_t*)(b + 6 * vb + 2 * bb);
+      P.nb[1] = (const uint8_t*)(b + 6 * vb + 3 * bb);
+      if (P.meta != nullptr) P.meta = (const LevelMeta*)(b + 6 * vb + 4 * bb);
     }
+    return P;
   };
-  for (int k = 0; k < nlev - 1; ++k) {
-    const LevelDesc<T>& D = descs[d0 + k];
-    T* z[2];
-    zof(k, z);
-    coarse_pre<T, DIM>(D, z, nu);
-    for (int tc = threadIdx.x; tc < D.gc.size; tc += COARSE_THREADS) {
-      restrict_slot<T, DIM>(D.g, D.P, z, D.gc, D.C.r[0], D.C.r[1], 0, tc);
-      restrict_slot<T, DIM>(D.g, D.P, z, D.gc, D.C.r[0], D.C.r[1], 1, tc);
+  if constexpr (USE_LDS) {
+    for (int k = 0; k < nlev; ++k) {
+      const LevelPtrs<T>& G = descs[d0 + k].P;
+      const LevelPtrs<T> P = lv(k);
+      const int n = descs[d0 + k].g.size;
+      for (int t = threadIdx.x; t < n; t += COARSE_THREADS) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          P.r[c][t] = ldg(G.r[c] + t);
+          const_cast<T*>(P.Adiag[c])[t] = ldg(G.Adiag[c] + t);
+          const_cast<int8_t*>(P.gt[c])[t] = ldg(G.gt[c] + t);
+          if (G.meta != nullptr) const_cast<uint8_t*>(P.nb[c])[t] = ldg(G.nb[c] + t);
+        }
+      }
+      if (G.meta != nullptr && threadIdx.x == 0) *const_cast<LevelMeta*>(P.meta) = *G.meta;
     }
     __syncthreads();
   }
-  {
-    T* z[2];
-    zof(nlev - 1, z);
-    coarse_bottom<T, DIM>(descs[d0 + nlev - 1], z, bottom);
+  for (int k = 0; k < nlev - 1; ++k) {
+    const LevelDesc<T>& D = descs[d0 + k];
+    const LevelPtrs<T> P = lv(k);
+    const LevelPtrs<T> C = lv(k + 1);
+    coarse_pre<T, DIM>(D.g, P, nu);
+    for (int tc = threadIdx.x; tc < D.gc.size; tc += COARSE_THREADS) {
+      restrict_slot<T, DIM>(D.g, P, P.z, D.gc, C.r[0], C.r[1], 0, tc);
+      restrict_slot<T, DIM>(D.g, P, P.z, D.gc, C.r[0], C.r[1], 1, tc);
+    }
+    __syncthreads();
   }
+  coarse_bottom<T, DIM>(descs[d0 + nlev - 1].g, lv(nlev - 1), bottom);
   for (int k = nlev - 2; k >= 0; --k) {
-    T* z[2];
-    T* zc[2];
-    zof(k, z);
-    zof(k + 1, zc);
-    coarse_post<T, DIM>(descs[d0 + k], z, zc, nu);
+    const LevelDesc<T>& D = descs[d0 + k];
+    const LevelPtrs<T> C = lv(k + 1);
+    coarse_post<T, DIM>(D.g, D.gc, lv(k), C.z, nu);
   }
   if constexpr (USE_LDS) {
     // the finer level's prolongation reads the first coarse level's z; the
-    // others are copied too so every level's z holds its V-cycle value
+    // other levels get z and r too, as the global-memory form leaves them
     for (int k = 0; k < nlev; ++k) {
-      const LevelDesc<T>& D = descs[d0 + k];
-      T* z[2];
-      zof(k, z);
-      for (int t = threadIdx.x; t < D.g.size; t += COARSE_THREADS) {
-        D.P.z[0][t] = z[0][t];
-        D.P.z[1][t] = z[1][t];
+      const LevelPtrs<T>& G = descs[d0 + k].P;
+      const LevelPtrs<T> P = lv(k);
+      const int n = descs[d0 + k].g.size;
+      for (int t = threadIdx.x; t < n; t += COARSE_THREADS) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          stg(G.z[c] + t, P.z[c][t]);
+          if (k > 0) stg(G.r[c] + t, P.r[c][t]);
+        }
       }
     }
   }

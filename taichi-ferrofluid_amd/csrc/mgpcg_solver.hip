@@ -53,13 +53,13 @@ struct LevelBuf {
   void* res[2];      // restriction scratch (NULL on the coarsest level)
 };
 
-int g_coarse_max_slots = 1024;  // levels at most this large go to k_coarse
+int g_coarse_max_slots = 4096;  // levels at most this large go to k_coarse
 int g_max_grid = 2048;          // workgroups of a persistent stage launch
 int g_fuse_first = 0;    // first pre-smoothing pair in one kernel (else two)
 int g_fuse_prolong = 1;  // prolongation inside the first post-smoothing half-sweep
 int g_fuse_p = 0;        // update_p inside compute_Ap
 int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
-int g_coarse_lds = 1;    // coarse kernel keeps z in LDS when it fits
+int g_coarse_lds = 1;    // coarse kernel works on an LDS image of its levels when they fit
 // restriction kernels: 0 one lane per child, 1 row-tiled with LDS, 2 residual
 // pass + child gather (both coalesced)
 int g_restrict_mode = 2;
@@ -200,7 +200,7 @@ struct Engine {
   };
   StageRef st_init, st_cg, fin_init, fin_cg, fin_all, st_dot_fused, st_dot_plain, st_coarse,
       st_gather;
-  bool coarse_lds = false;  // every coarse item's z fits the coarse kernel's LDS
+  bool coarse_lds = false;  // every coarse item's levels fit the coarse kernel's LDS image
   int restrict_mode = 0;  // restriction stages are laid out for this g_restrict_mode
   std::vector<StageRef> st_slot, st_restrict, st_bottom;  // per level
   // the half-sweep stages: virtual blocks of BLOCK * SLOT_LANES slots
@@ -521,9 +521,10 @@ int engine_build_items(Engine* e) {
     x.desc = m * L + M.s->coarse_from;
     x.block0 = L - M.s->coarse_from;
     it.push_back(x);
-    long elems = 0;
-    for (int l = M.s->coarse_from; l < L; ++l) elems += 2L * M.s->geo[l].size;
-    if (elems > COARSE_LDS_ELEMS) e->coarse_lds = false;
+    long bytes = 0;
+    for (int l = M.s->coarse_from; l < L; ++l)
+      bytes += coarse_image_bytes(M.s->geo[l].size, (int)M.s->esz);
+    if (bytes > COARSE_LDS_BYTES) e->coarse_lds = false;
   }
   e->st_coarse.n = (int)it.size() - e->st_coarse.off;
   e->st_coarse.nblocks = e->st_coarse.n;

@@ -100,7 +100,7 @@ def test_reference_structured_equals_device_loop(dev):
     assert np.array_equal(_x(s), xa)
 
 
-KNOB_DEFAULTS = {"coarse_max": 1024, "max_grid": 2048, "fuse_first": 0, "fuse_prolong": 1,
+KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 0, "fuse_prolong": 1,
                  "fuse_p": 0, "compact": 1, "coarse_lds": 1, "restrict_rows": 2}
 
 
@@ -120,7 +120,7 @@ def set_knobs(**kw):
 @pytest.mark.parametrize("knobs", [
     {"coarse_max": 0}, {"coarse_max": 1 << 30}, {"max_grid": 8}, {"max_grid": 64},
     {"fuse_first": 1}, {"fuse_prolong": 0}, {"fuse_p": 1},
-    {"fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1}, {"coarse_max": 4096},
+    {"fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1}, {"coarse_max": 1024},
     {"compact": 0}, {"compact": 0, "fuse_first": 1, "fuse_prolong": 1, "fuse_p": 1},
     {"coarse_lds": 0}, {"coarse_lds": 0, "compact": 0, "fuse_prolong": 1},
     {"restrict_rows": 1}, {"restrict_rows": 1, "compact": 0, "max_grid": 8},
