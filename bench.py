@@ -64,6 +64,8 @@ def parse():
                     help="admissions per server step (0: no limit)")
     ap.add_argument("--server-depth", type=int, default=2,
                     help="iterations the server keeps queued on the device ahead of the host")
+    ap.add_argument("--server-streams", type=int, default=2,
+                    help="engine streams the server spreads the requests in flight over")
     ap.add_argument("--baseline-steps", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -298,6 +300,7 @@ def main():
         del src, dst
     _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_PROBE_EVERY, args.probe_every))
     _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_SERVER_DEPTH, args.server_depth))
+    _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_SERVER_STREAMS, args.server_streams))
     wl, n, L = args.workload, args.n, args.levels
     what = WORKLOADS[wl]["what"]
 
@@ -323,7 +326,8 @@ def main():
                                f"{args.precision} precision, continuous batching with "
                                f"{args.concurrency} requests in flight",
                    "n": n, "levels": L, "seeds": args.seeds, "concurrency": args.concurrency,
-                   "admit_per_step": args.admit_per_step, "server_depth": args.server_depth},
+                   "admit_per_step": args.admit_per_step, "server_depth": args.server_depth,
+                   "server_streams": args.server_streams},
         "p50_ms": round(head["lat"]["p50_ms"], 3),
         "p99_ms": round(head["lat"]["p99_ms"], 3),
         "iters": head["iters"],

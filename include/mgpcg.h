@@ -89,6 +89,8 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
                                      many (default 4) */
 #define MGPCG_OPT_SERVER_DEPTH 12 /* iterations a server keeps queued on the device ahead of
                                      the host, 1..3 (default 2) */
+#define MGPCG_OPT_SERVER_STREAMS 13 /* engines (each with its own stream) a server created
+                                       afterwards spreads its members over, 1..4 (default 2) */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,

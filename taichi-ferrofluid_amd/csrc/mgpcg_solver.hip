@@ -65,6 +65,7 @@ int g_coarse_lds = 1;    // coarse kernel works on an LDS image of its levels wh
 int g_restrict_mode = 2;
 int g_probe_every = 4;    // the launch probe times one eligible launch in this many
 int g_server_depth = 2;   // server iterations queued on the device ahead of the host
+int g_server_streams = 2; // engines (streams) a server spreads its members over
 
 }  // namespace
 
@@ -1154,6 +1155,10 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
                  Engine::NSLOT - 1);
       g_server_depth = value;
       return 0;
+    case MGPCG_OPT_SERVER_STREAMS:
+      MG_REQUIRE(value >= 1 && value <= 4, "server streams must be in [1, 4]");
+      g_server_streams = value;
+      return 0;
     default:
       (void)s;
       mg_set_error("unknown option %d", option);
@@ -1715,11 +1720,17 @@ extern "C" int mgpcg_batch_solve(mgpcg_batch_t* b, mgpcg_solver_t* const* system
 // The server: continuous batching.  Systems are submitted at any time; each
 // call of mgpcg_server_step admits the submitted ones (they join at the next
 // iteration boundary), enqueues one iteration for every member and retires
-// the ones whose state (read one iteration behind) says they are done.
+// the ones whose state (read back a few iterations behind) says they are done.
+// Members are spread over lanes, each an engine with its own stream, so one
+// lane's latency-bound coarse levels and reductions overlap another lane's
+// bandwidth-bound level-0 kernels on the device.
 struct mgpcg_server {
-  Engine* e = nullptr;
-  std::deque<int> inflight;  // readback slots of the queued iterations, oldest first
-  int next_slot = 0;
+  struct Lane {
+    Engine* e = nullptr;
+    std::deque<int> inflight;  // readback slots of the queued iterations, oldest first
+    int next_slot = 0;
+  };
+  std::vector<Lane> lanes;
   std::deque<std::pair<mgpcg_solver*, int64_t>> pending;
   std::vector<std::pair<int64_t, mgpcg_result_t>> finished;
   struct Params {
@@ -1727,6 +1738,9 @@ struct mgpcg_server {
     double rel_tol = 1e-12, abs_tol = 1e-14, eps = 1e-12;
     int refine_every = 10;
   } prm;
+  ~mgpcg_server() {
+    for (Lane& ln : lanes) delete ln.e;
+  }
 };
 
 extern "C" int mgpcg_server_create(int max_iters, double rel_tol, double abs_tol, double eps,
@@ -1734,10 +1748,13 @@ extern "C" int mgpcg_server_create(int max_iters, double rel_tol, double abs_tol
   MG_REQUIRE(out != nullptr, "out is NULL");
   mgpcg_server* sv = new (std::nothrow) mgpcg_server();
   MG_REQUIRE(sv != nullptr, "out of host memory");
-  sv->e = new_engine();
-  if (!sv->e) {
-    delete sv;
-    return -2;
+  sv->lanes.resize(g_server_streams);
+  for (mgpcg_server::Lane& ln : sv->lanes) {
+    ln.e = new_engine();
+    if (!ln.e) {
+      delete sv;
+      return -2;
+    }
   }
   sv->prm.max_iters = max_iters;
   sv->prm.rel_tol = rel_tol;
@@ -1749,69 +1766,86 @@ extern "C" int mgpcg_server_create(int max_iters, double rel_tol, double abs_tol
 }
 
 extern "C" int mgpcg_server_destroy(mgpcg_server_t* sv) {
-  if (!sv) return 0;
-  delete sv->e;
   delete sv;
   return 0;
 }
 
 extern "C" int mgpcg_server_set_stream(mgpcg_server_t* sv, void* stream) {
   MG_REQUIRE(sv != nullptr, "server is NULL");
-  sv->e->user_stream = (hipStream_t)stream;
+  for (mgpcg_server::Lane& ln : sv->lanes) ln.e->user_stream = (hipStream_t)stream;
   return 0;
 }
 
 extern "C" int mgpcg_server_submit(mgpcg_server_t* sv, mgpcg_solver_t* s, int64_t tag) {
   MG_REQUIRE(sv != nullptr && s != nullptr, "NULL argument");
-  for (const Member& M : sv->e->mem)
-    MG_REQUIRE(!(M.active && M.s == s), "system is already being solved");
+  for (const mgpcg_server::Lane& ln : sv->lanes)
+    for (const Member& M : ln.e->mem)
+      MG_REQUIRE(!(M.active && M.s == s), "system is already being solved");
+  for (const auto& pr : sv->pending)
+    MG_REQUIRE(pr.first != s, "system is already submitted");
   sv->pending.push_back({s, tag});
   return 0;
 }
 
 extern "C" int mgpcg_server_step(mgpcg_server_t* sv, int* n_active) {
   MG_REQUIRE(sv != nullptr, "server is NULL");
-  Engine* e = sv->e;
-  // newly submitted systems were assembled on the caller's stream: order the
-  // engine after it (only then, so assembly and iterations overlap otherwise)
-  if (!sv->pending.empty() && engine_begin(e)) return -2;
-  // admit submitted systems at this iteration boundary
+  const int nl = (int)sv->lanes.size();
+  // admit submitted systems at this iteration boundary, each into the lane
+  // with the fewest members
+  std::vector<int> count(nl);
+  std::vector<char> begun(nl, 0);
+  for (int i = 0; i < nl; ++i) count[i] = engine_active(sv->lanes[i].e);
   while (!sv->pending.empty()) {
     auto pr = sv->pending.front();
     sv->pending.pop_front();
+    int li = 0;
+    for (int i = 1; i < nl; ++i)
+      if (count[i] < count[li]) li = i;
+    Engine* e = sv->lanes[li].e;
+    // newly submitted systems were assembled on the caller's stream: order
+    // the lane after it (only then, so assembly and iterations overlap otherwise)
+    if (!begun[li]) {
+      if (engine_begin(e)) return -2;
+      begun[li] = 1;
+    }
     // free member slots of retired systems are reused
     for (Member& M : e->mem)
       if (!M.active) M.s = nullptr;
     const SolveParams prm{sv->prm.max_iters, sv->prm.rel_tol, sv->prm.abs_tol, sv->prm.eps,
                           sv->prm.refine_every};
     if (engine_add(e, pr.first, pr.second, prm) < 0) return -2;
+    ++count[li];
   }
-  // Up to g_server_depth iterations stay queued on the device while the host
-  // returns to its caller (admissions, assembly); a member's completion is
-  // seen that many iterations late, and its later iterations skip it.
-  std::vector<Finished> done;
-  if (engine_active(e) > 0) {
-    const int slot = sv->next_slot;
-    sv->next_slot = (sv->next_slot + 1) % Engine::NSLOT;
-    if (engine_enqueue(e, slot, false)) return -2;
-    sv->inflight.push_back(slot);
-    while ((int)sv->inflight.size() > g_server_depth) {
-      if (engine_collect(e, sv->inflight.front(), &done)) return -2;
-      sv->inflight.pop_front();
+  // every lane's iteration is enqueued before the host waits for any readback
+  for (mgpcg_server::Lane& ln : sv->lanes) {
+    if (engine_active(ln.e) == 0) continue;
+    const int slot = ln.next_slot;
+    ln.next_slot = (ln.next_slot + 1) % Engine::NSLOT;
+    if (engine_enqueue(ln.e, slot, false)) return -2;
+    ln.inflight.push_back(slot);
+  }
+  // Up to g_server_depth iterations per lane stay queued on the device while
+  // the host returns to its caller (admissions, assembly); a member's
+  // completion is seen that many iterations late, and its later iterations
+  // skip it.
+  int active = 0;
+  for (mgpcg_server::Lane& ln : sv->lanes) {
+    Engine* e = ln.e;
+    std::vector<Finished> done;
+    const size_t keep = engine_active(e) > 0 ? (size_t)g_server_depth : 0;
+    while (ln.inflight.size() > keep) {
+      if (engine_collect(e, ln.inflight.front(), &done)) return -2;
+      ln.inflight.pop_front();
     }
-  } else {
-    while (!sv->inflight.empty()) {
-      if (engine_collect(e, sv->inflight.front(), &done)) return -2;
-      sv->inflight.pop_front();
+    for (const Finished& f : done) {
+      mgpcg_result_t r;
+      fill_result(f.st, &r);
+      sv->finished.push_back({e->mem[f.member].tag, r});
     }
+    active += engine_active(e);
   }
-  for (const Finished& f : done) {
-    mgpcg_result_t r;
-    fill_result(f.st, &r);
-    sv->finished.push_back({e->mem[f.member].tag, r});
-  }
-  if (n_active) *n_active = engine_active(e);
-  // no ordering of the caller's stream after the engine: results are released
+  if (n_active) *n_active = active;
+  // no ordering of the caller's stream after the engines: results are released
   // only after the host waited for the readback recorded behind their last
   // kernels, so the caller may read x at once
   MG_HIP(hipGetLastError());
@@ -1834,9 +1868,14 @@ extern "C" int mgpcg_server_poll(mgpcg_server_t* sv, int64_t* tags, mgpcg_result
 extern "C" int mgpcg_server_pending(const mgpcg_server_t* sv, int* n_pending, int* n_active,
                                     int* inflight) {
   MG_REQUIRE(sv != nullptr, "server is NULL");
+  int act = 0, infl = 0;
+  for (const mgpcg_server::Lane& ln : sv->lanes) {
+    act += engine_active(ln.e);
+    infl |= ln.inflight.empty() ? 0 : 1;
+  }
   if (n_pending) *n_pending = (int)sv->pending.size();
-  if (n_active) *n_active = engine_active(sv->e);
-  if (inflight) *inflight = sv->inflight.empty() ? 0 : 1;
+  if (n_active) *n_active = act;
+  if (inflight) *inflight = infl;
   return 0;
 }
 
@@ -1871,7 +1910,24 @@ int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells
 extern "C" int mgpcg_server_probe(mgpcg_server_t* sv, int enable, int* n_launches,
                                   double* total_ms, int64_t* total_cells) {
   MG_REQUIRE(sv != nullptr, "server is NULL");
-  return engine_probe(sv->e, enable, n_launches, total_ms, total_cells);
+  int n = 0;
+  double ms = 0.0;
+  int64_t cells = 0;
+  for (mgpcg_server::Lane& ln : sv->lanes) {
+    int n1 = 0;
+    double ms1 = 0.0;
+    int64_t c1 = 0;
+    if (engine_probe(ln.e, enable, &n1, &ms1, &c1)) return -2;
+    n += n1;
+    ms += ms1;
+    cells += c1;
+  }
+  if (!enable) {
+    if (n_launches) *n_launches = n;
+    if (total_ms) *total_ms = ms;
+    if (total_cells) *total_cells = cells;
+  }
+  return 0;
 }
 
 extern "C" int mgpcg_batch_probe(mgpcg_batch_t* b, int enable, int* n_launches,

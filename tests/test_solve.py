@@ -112,7 +112,8 @@ def set_knobs(**kw):
            "fuse_first": _lib.OPT_FUSE_FIRST, "fuse_prolong": _lib.OPT_FUSE_PROLONG,
            "fuse_p": _lib.OPT_FUSE_P, "compact": _lib.OPT_COMPACT,
            "coarse_lds": _lib.OPT_COARSE_LDS, "restrict_rows": _lib.OPT_RESTRICT_ROWS,
-           "probe_every": _lib.OPT_PROBE_EVERY, "server_depth": _lib.OPT_SERVER_DEPTH}
+           "probe_every": _lib.OPT_PROBE_EVERY, "server_depth": _lib.OPT_SERVER_DEPTH,
+           "server_streams": _lib.OPT_SERVER_STREAMS}
     for k, v in kw.items():
         _lib.check(lib.mgpcg_set_option(None, opt[k], v))
 
@@ -226,17 +227,19 @@ def test_batch_uneven_equals_alone(dev, precision):
         assert rel_l2(xb, xr) <= 1e-5
 
 
-@pytest.mark.parametrize("depth", [2, 1, 3])
-def test_server_continuous_batching_equals_alone(dev, depth):
+@pytest.mark.parametrize("depth,streams", [(2, 2), (1, 2), (2, 1), (3, 3)],
+                         ids=lambda v: str(v))
+def test_server_continuous_batching_equals_alone(dev, depth, streams):
     """Requests joining at different iterations (continuous batching) give the
     same bits as solving alone, however many iterations the server keeps
-    queued ahead of the host; slots and hierarchies are reused."""
+    queued ahead of the host and however many streams it spreads its members
+    over; slots and hierarchies are reused."""
     import serve
     import systems
 
     if _emu(dev) and depth == 3:
         pytest.skip("the emulator covers depths 1 and 2")
-    set_knobs(server_depth=depth)
+    set_knobs(server_depth=depth, server_streams=streams)
 
     emu = _emu(dev)
     nreq = 4 if emu else 6
@@ -259,7 +262,7 @@ def test_server_continuous_batching_equals_alone(dev, depth):
             srv.retire(rq)
             done.append(rq)
     n_launch, ms, cells = srv.probe(False)
-    set_knobs(probe_every=4, server_depth=2)
+    set_knobs(probe_every=4, server_depth=2, server_streams=2)
     assert srv.max_members >= 2  # they really overlapped
     # with every launch timed, the probe counts exactly the level-0 cells swept
     # by smooth(0, 0): two launches per V-cycle (nu = 2), iters + 1 V-cycles
