@@ -157,10 +157,11 @@ def cpu_baseline(workload, n, levels, budget_s):
                       f"threads={threads}"}
 
 
-def measure_served(args, workload, n, L, seeds, concurrency, steps, warmup, dev):
+def measure_served(args, workload, n, L, seeds, concurrency, steps, warmup, dev, streams=None):
     """Closed-loop continuous batching of `steps` requests with `concurrency`
     in flight; returns the throughput, latencies and the level-0 half-sweep
-    roofline measured over the timed requests."""
+    roofline measured over the timed requests.  `streams` overrides the
+    server's engine streams."""
     import _lib
     import serve
     import systems
@@ -168,8 +169,13 @@ def measure_served(args, workload, n, L, seeds, concurrency, steps, warmup, dev)
 
     scenes_dev = [systems.upload(make_scene(workload, i, n, L), dev) for i in range(seeds)]
     torch.cuda.synchronize()
+    lib = _lib.load()
+    if streams is not None:
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_SERVER_STREAMS, streams))
     srv = serve.Server(64 << 30, precision=args.precision, n_levels=L,
                        admit_per_step=args.admit_per_step or None, device=dev)
+    if streams is not None:
+        _lib.check(lib.mgpcg_set_option(None, _lib.OPT_SERVER_STREAMS, args.server_streams))
     mk = (lambda i: scenes_dev[i % len(scenes_dev)])
     if args.probe_warmup:
         srv.probe(True)
@@ -336,6 +342,14 @@ def main():
     }
 
     if not args.no_extras:
+        if args.server_streams > 1:
+            # the same kernel with no other lane's kernels beside it on the
+            # device: the headline's launches share it with the second lane
+            iso = measure_served(args, wl, n, L, args.seeds, args.concurrency,
+                                 max(16, args.steps // 4), 4, dev, streams=1)
+            iso.pop("scenes")
+            line["roofline_isolated"] = dict(iso["roofline"] or {}, server_streams=1,
+                                             solves_per_s=round(iso["value"], 3))
         # single request in flight: the latency-bound rate
         one = measure_served(args, wl, n, L, args.seeds, 1, max(4, args.steps // 8), 2, dev)
         one.pop("scenes")
