@@ -43,9 +43,11 @@ HBM_PEAK_GBS = 8000.0
 # storage: own r, Adiag, type (9 B / 2), other-colour z read (4 / 2), own z
 # written (4 / 2), every Ax (4 * dim)  ->  20.5 B in 3D
 SMOOTH_BYTES_PER_CELL_3D = 20.5
-# the same half-sweep on a level with uniform couplings: own r, Adiag, type and
-# flag byte (10 / 2), other-colour z (4 / 2), own z written (4 / 2)  ->  9 B
-SMOOTH_BYTES_PER_CELL_3D_COMPACT = 9.0
+# the same half-sweep on a level with uniform couplings: own r, type and 16-bit
+# flag word carrying the couplings and the diagonal (7 / 2), other-colour z
+# (4 / 2), own z written (4 / 2)  ->  7.5 B (plus Adiag where a cell's diagonal
+# is not a plain sum of couplings)
+SMOOTH_BYTES_PER_CELL_3D_COMPACT = 7.5
 
 
 def parse():
@@ -204,8 +206,8 @@ def measure_served(args, workload, n, L, seeds, concurrency, steps, warmup, dev,
             "alg_bytes_per_launch": round(alg_bytes), "launch_us": round(launch_s * 1e6, 3),
             "launches": nl, "cells_per_launch": round(cells_tot / nl),
             "traffic_bytes_per_cell": tpc,
-            # pressure systems have uniform couplings: the kernel reads flag bytes
-            # instead of Ax and must move 9 B per cell, not 20.5
+            # pressure systems have uniform couplings: the kernel reads flag words
+            # instead of Ax and Adiag and must move 7.5 B per cell, not 20.5
             "compact_bytes_per_cell": SMOOTH_BYTES_PER_CELL_3D_COMPACT,
             "achieved_compact": round(SMOOTH_BYTES_PER_CELL_3D_COMPACT * cells_tot / nl
                                       / launch_s / 1e9, 1)}

@@ -45,9 +45,14 @@ struct LevelMeta {
   uint32_t minbits, maxbits;
 };
 
-// flag byte of a slot: bit a = coupling to the upper neighbour on axis a is
-// nonzero (Ax[I][a]), bit NB_DN + a = coupling to the lower one (Ax[I - e_a][a])
+// flag word of a slot: bit a = coupling to the upper neighbour on axis a is
+// nonzero (Ax[I][a]), bit NB_DN + a = coupling to the lower one (Ax[I - e_a][a]),
+// bits NB_AD.. = the diagonal's code: n < NB_AD_STORED when Adiag equals n
+// couplings of the slot's magnitude added one at a time from 0 (diag_of_code),
+// NB_AD_STORED when it is read from the Adiag array
 constexpr int NB_DN = 3;
+constexpr int NB_AD = 6;
+constexpr uint32_t NB_AD_STORED = 7u;
 
 template <typename T>
 struct LevelPtrs {
@@ -56,7 +61,7 @@ struct LevelPtrs {
   const T* Ax[3][2];  // [axis][colour]
   T* r[2];
   T* z[2];
-  const uint8_t* nb[2];    // coupling flags per slot
+  const uint16_t* nb[2];   // flag word per slot
   const LevelMeta* meta;   // NULL: always read Ax
   T* res[2];               // restriction scratch: the level's residual (not the coarsest)
 };
@@ -67,6 +72,22 @@ struct Coupling {
   bool uniform;
   T coef;
 };
+// The diagonal encoded by code n < NB_AD_STORED on a level whose couplings
+// are all `coef`: -coef added n times to 0, in order, as the operator
+// assembly accumulates it.  The packing kernel assigns a code only where
+// this reproduces the stored Adiag bit for bit.
+template <typename T>
+__host__ __device__ __forceinline__ T diag_of_code(uint32_t n, T coef) {
+  const T m = -coef;
+  T a = (T)0;
+#pragma unroll
+  for (uint32_t i = 1; i < NB_AD_STORED; ++i) {
+    const T b = a + m;
+    a = i <= n ? b : a;
+  }
+  return a;
+}
+
 template <typename T>
 __device__ __forceinline__ Coupling<T> coupling(const LevelPtrs<T>& P) {
   Coupling<T> k{false, (T)0};
@@ -388,32 +409,6 @@ __device__ __forceinline__ void run_flush(RunSum& r) {
   r.dst = nullptr;
 }
 
-// smooth(l, c) at slot t in the latency-oriented form; returns the slot's z
-// after the half-sweep (the new value on a FLUID cell)
-template <typename T, int DIM>
-__device__ __forceinline__ T smooth_slot_g(const Geo& g, const LevelPtrs<T>& P,
-                                           const Coupling<T>& k, int c, int t) {
-  const int q = row_q(g, t, c);
-  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
-  const int8_t gt = ldg(P.gt[c] + t);
-  const T rv = ldg(P.r[c] + t);
-  const T ad = ldg(P.Adiag[c] + t);
-  const T* zo = P.z[c ^ 1];
-  T zl[DIM], zh[DIM], al[DIM], ah[DIM];
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    zl[a] = ldg(zo + n.lo[a]);
-    zh[a] = ldg(zo + n.hi[a]);
-  }
-  couplings_g<T, DIM>(P, k, c, t, n, al, ah);
-  if (gt != MG_FLUID) return (T)0;  // unused: callers mask with the cell type
-  T sm = nsum_vals<T, DIM>(al, ah, zl, zh);
-  T num = rv - sm;
-  const T zn = num / ad;
-  stg(P.z[c] + t, zn);
-  return zn;
-}
-
 // Two-phase form of a half-sweep slot: every value a slot needs is loaded
 // first (smooth_load), the update is computed and stored afterwards
 // (smooth_store).  A lane that loads several slots before storing any keeps
@@ -422,10 +417,24 @@ __device__ __forceinline__ T smooth_slot_g(const Geo& g, const LevelPtrs<T>& P,
 template <typename T, int DIM, bool COMPACT>
 struct SmoothVals {
   int gt;
-  uint32_t f;  // coupling flags (COMPACT)
-  T rv, ad;
+  uint32_t f;  // flag word (COMPACT)
+  T rv, ad;    // ad: the stored diagonal (not COMPACT)
   T zl[DIM], zh[DIM], al[DIM], ah[DIM];  // al / ah: stored couplings (not COMPACT)
 };
+
+// The diagonal of slot t of colour c: decoded from its flag word f on a level
+// with uniform couplings (read from Adiag only for the code NB_AD_STORED),
+// else the value `stored` loaded with the slot's other operands.
+template <typename T, bool COMPACT>
+__device__ __forceinline__ T slot_diag(const LevelPtrs<T>& P, const Coupling<T>& k, int c, int t,
+                                       uint32_t f, T stored) {
+  if constexpr (COMPACT) {
+    const uint32_t n = (f >> NB_AD) & 7u;
+    return n == NB_AD_STORED ? ldg(P.Adiag[c] + t) : diag_of_code<T>(n, k.coef);
+  } else {
+    return stored;
+  }
+}
 
 template <typename T, int DIM, bool COMPACT>
 __device__ __forceinline__ void smooth_load(const Geo& g, const LevelPtrs<T>& P, int c, int t,
@@ -434,7 +443,7 @@ __device__ __forceinline__ void smooth_load(const Geo& g, const LevelPtrs<T>& P,
   const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
   v.gt = ldg(P.gt[c] + t);
   v.rv = ldg(P.r[c] + t);
-  v.ad = ldg(P.Adiag[c] + t);
+  if constexpr (!COMPACT) v.ad = ldg(P.Adiag[c] + t);
   const T* zo = P.z[c ^ 1];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
@@ -467,10 +476,10 @@ __device__ __forceinline__ void smooth_keep(const SmoothVals<T, DIM, COMPACT>& v
   }
 }
 
+// the new z of a FLUID slot: (r - neighbour_sum) / Adiag (mgpcg.py:104-109)
 template <typename T, int DIM, bool COMPACT>
-__device__ __forceinline__ void smooth_store(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
-                                             int t, const SmoothVals<T, DIM, COMPACT>& v) {
-  if (v.gt != MG_FLUID) return;
+__device__ __forceinline__ T smooth_value(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
+                                          int t, const SmoothVals<T, DIM, COMPACT>& v) {
   T al[DIM], ah[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
@@ -484,7 +493,28 @@ __device__ __forceinline__ void smooth_store(const LevelPtrs<T>& P, const Coupli
   }
   T sm = nsum_vals<T, DIM>(al, ah, v.zl, v.zh);
   T num = v.rv - sm;
-  stg(P.z[c] + t, num / v.ad);
+  return num / slot_diag<T, COMPACT>(P, k, c, t, v.f, v.ad);
+}
+
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void smooth_store(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
+                                             int t, const SmoothVals<T, DIM, COMPACT>& v) {
+  if (v.gt != MG_FLUID) return;
+  stg(P.z[c] + t, smooth_value<T, DIM, COMPACT>(P, k, c, t, v));
+}
+
+// smooth(l, c) at slot t in the latency-oriented form; returns the slot's z
+// after the half-sweep (the new value on a FLUID cell)
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ T smooth_slot_g(const Geo& g, const LevelPtrs<T>& P,
+                                           const Coupling<T>& k, int c, int t) {
+  SmoothVals<T, DIM, COMPACT> v;
+  smooth_load<T, DIM, COMPACT>(g, P, c, t, v);
+  smooth_keep<T, DIM, COMPACT>(v);
+  if (v.gt != MG_FLUID) return (T)0;  // unused: callers mask with the cell type
+  const T zn = smooth_value<T, DIM, COMPACT>(P, k, c, t, v);
+  stg(P.z[c] + t, zn);
+  return zn;
 }
 
 // SLOT_LANES slots per lane from t0 (stride BLOCK), all loaded before any is stored
@@ -512,8 +542,8 @@ __device__ __forceinline__ void smooth_slots(const Geo& g, const LevelPtrs<T>& P
 template <typename T, int DIM, bool COMPACT>
 struct StencilVals {
   int gt;
-  uint32_t f;
-  T own, ad;
+  uint32_t f;  // flag word (COMPACT)
+  T own, ad;   // ad: the stored diagonal (not COMPACT)
   T vl[DIM], vh[DIM], al[DIM], ah[DIM];
 };
 
@@ -524,7 +554,7 @@ __device__ __forceinline__ void stencil_load(const Geo& g, const LevelPtrs<T>& P
   const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
   s.gt = ldg(P.gt[c] + t);
   s.own = ldg(v[c] + t);
-  s.ad = ldg(P.Adiag[c] + t);
+  if constexpr (!COMPACT) s.ad = ldg(P.Adiag[c] + t);
   const T* vo = v[c ^ 1];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
@@ -556,10 +586,11 @@ __device__ __forceinline__ void stencil_keep(const StencilVals<T, DIM, COMPACT>&
   }
 }
 
-// Adiag * own + neighbour_sum, in the reference order (compute_Ap, mgpcg.py:218-224)
+// Adiag * own + neighbour_sum at slot t of colour c, in the reference order
+// (compute_Ap, mgpcg.py:218-224)
 template <typename T, int DIM, bool COMPACT>
-__device__ __forceinline__ T stencil_apply(const Coupling<T>& k,
-                                           const StencilVals<T, DIM, COMPACT>& s) {
+__device__ __forceinline__ T stencil_apply(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
+                                           int t, const StencilVals<T, DIM, COMPACT>& s) {
   T al[DIM], ah[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
@@ -571,14 +602,14 @@ __device__ __forceinline__ T stencil_apply(const Coupling<T>& k,
       ah[a] = s.ah[a];
     }
   }
-  T Av = s.ad * s.own;
+  T Av = slot_diag<T, COMPACT>(P, k, c, t, s.f, s.ad) * s.own;
   T ns = nsum_vals<T, DIM>(al, ah, s.vl, s.vh);
   return Av + ns;
 }
 
 // prolongate(l); smooth(l, 1) at colour-1 slot t in the latency-oriented form
 // (same arithmetic as prolong_smooth1_slot below)
-template <typename T, int DIM>
+template <typename T, int DIM, bool COMPACT>
 __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& gc,
                                                        const LevelPtrs<T>& P,
                                                        const Coupling<T>& k, const T* zc0,
@@ -587,7 +618,8 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
   const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, c.q);
   const int8_t gt = ldg(P.gt[1] + t);
   const T rv = ldg(P.r[1] + t);
-  const T ad = ldg(P.Adiag[1] + t);
+  T ad = (T)0;
+  if constexpr (!COMPACT) ad = ldg(P.Adiag[1] + t);
   const T* z0 = P.z[0];
   T zl[DIM], zh[DIM], cl[DIM], ch[DIM], al[DIM], ah[DIM];
   bool il[DIM], ih[DIM];
@@ -616,7 +648,18 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
       }
     }
   }
-  couplings_g<T, DIM>(P, k, 1, t, n, al, ah);
+  uint32_t f = 0u;
+  if constexpr (COMPACT) {
+    f = ldg(P.nb[1] + t);
+    MG_KEEP(f);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      al[a] = ((f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
+      ah[a] = ((f >> a) & 1u) ? k.coef : (T)0;
+    }
+  } else {
+    couplings_g<T, DIM>(P, k, 1, t, n, al, ah);
+  }
   if (gt != MG_FLUID) return;
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
@@ -625,7 +668,22 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
   }
   T ret = nsum_vals<T, DIM>(al, ah, zl, zh);
   T num = rv - ret;
-  stg(P.z[1] + t, num / ad);
+  stg(P.z[1] + t, num / slot_diag<T, COMPACT>(P, k, 1, t, f, ad));
+}
+
+// z.fill(0); smooth(l, 0) on colour 0 at slot t in the latency-oriented form:
+// z0 = r0 / Adiag0 on FLUID, else 0
+template <typename T, bool COMPACT>
+__device__ __forceinline__ void first_slot_g(const LevelPtrs<T>& P, const Coupling<T>& k, int t) {
+  const int8_t gt = ldg(P.gt[0] + t);
+  const T rv = ldg(P.r[0] + t);
+  uint32_t f = 0u;
+  T ad = (T)0;
+  if constexpr (COMPACT) f = ldg(P.nb[0] + t);
+  else ad = ldg(P.Adiag[0] + t);
+  T z = (T)0;
+  if (gt == MG_FLUID) z = rv / slot_diag<T, COMPACT>(P, k, 0, t, f, ad);
+  stg(P.z[0] + t, z);
 }
 
 // parent slot in the coarser level of fine-level slot `t` of colour `c`
@@ -851,11 +909,8 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
     if (OP == ST_FIRST) {
-      // z.fill(0); smooth(l, 0) on colour 0: z0 = r0 / Adiag0 on FLUID, else 0
-      const int8_t gt = ldg(D.P.gt[0] + t);
-      const T rv = ldg(D.P.r[0] + t);
-      const T ad = ldg(D.P.Adiag[0] + t);
-      stg(D.P.z[0] + t, gt == MG_FLUID ? rv / ad : (T)0);
+      if (kc.uniform) first_slot_g<T, true>(D.P, kc, t);
+      else first_slot_g<T, false>(D.P, kc, t);
     } else if (OP == ST_PROLONGATE0) {
       Slot sl = decode(D.g, t, 0);
       if (!sl.interior) continue;
@@ -867,7 +922,10 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     } else if (OP == ST_PAIR_FROM_ZERO) {
       smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
     } else if (OP == ST_PROLONG_SMOOTH1) {
-      prolong_smooth1_slot_g<T, DIM>(D.g, D.gc, D.P, kc, D.C.z[0], D.C.z[1], t);
+      if (kc.uniform)
+        prolong_smooth1_slot_g<T, DIM, true>(D.g, D.gc, D.P, kc, D.C.z[0], D.C.z[1], t);
+      else
+        prolong_smooth1_slot_g<T, DIM, false>(D.g, D.gc, D.P, kc, D.C.z[0], D.C.z[1], t);
     } else if (OP == ST_ZERO) {
       zero_slot<T>(D.P, t);
     } else if (OP == ST_PROLONGATE) {
@@ -964,7 +1022,7 @@ __device__ __forceinline__ void residual_slot(const LevelDesc<T>& D, const Coupl
     const T rv = ldg(D.P.r[c] + t);
     stencil_keep<T, DIM, COMPACT>(s);
     T v = (T)0;
-    if (s.gt == MG_FLUID) v = rv - stencil_apply<T, DIM, COMPACT>(k, s);
+    if (s.gt == MG_FLUID) v = rv - stencil_apply<T, DIM, COMPACT>(D.P, k, c, t, s);
     stg(D.P.res[c] + t, v);
   }
 }
@@ -1190,7 +1248,8 @@ __device__ void coarse_bottom(const Geo& g, const LevelPtrs<T>& P, int bottom) {
 constexpr int COARSE_LDS_BYTES = 160 * 1024;
 __host__ __device__ constexpr int lds_align(int b) { return (b + 15) & ~15; }
 __host__ __device__ constexpr int coarse_image_bytes(int size, int esz) {
-  return 6 * lds_align(size * esz) + 4 * lds_align(size) + lds_align((int)sizeof(LevelMeta));
+  return 6 * lds_align(size * esz) + 2 * lds_align(size) + 2 * lds_align(2 * size) +
+         lds_align((int)sizeof(LevelMeta));
 }
 
 // One workgroup per item: item.desc = the system's LevelDesc of its first
@@ -1217,7 +1276,7 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
       int off = 0;
       for (int j = 0; j < k; ++j) off += coarse_image_bytes(descs[d0 + j].g.size, (int)sizeof(T));
       const int n = descs[d0 + k].g.size;
-      const int vb = lds_align(n * (int)sizeof(T)), bb = lds_align(n);
+      const int vb = lds_align(n * (int)sizeof(T)), bb = lds_align(n), wb = lds_align(2 * n);
       unsigned char* b = img + off;
       P.z[0] = (T*)(b + 0 * vb);
       P.z[1] = (T*)(b + 1 * vb);
@@ -1227,9 +1286,9 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
       P.Adiag[1] = (const T*)(b + 5 * vb);
       P.gt[0] = (const int8_t*)(b + 6 * vb);
       P.gt[1] = (const int8_t*)(b + 6 * vb + bb);
-      P.nb[0] = (const uint8_t*)(b + 6 * vb + 2 * bb);
-      P.nb[1] = (const uint8_t*)(b + 6 * vb + 3 * bb);
-      if (P.meta != nullptr) P.meta = (const LevelMeta*)(b + 6 * vb + 4 * bb);
+      P.nb[0] = (const uint16_t*)(b + 6 * vb + 2 * bb);
+      P.nb[1] = (const uint16_t*)(b + 6 * vb + 2 * bb + wb);
+      if (P.meta != nullptr) P.meta = (const LevelMeta*)(b + 6 * vb + 2 * bb + 2 * wb);
     }
     return P;
   };
@@ -1244,7 +1303,7 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
           P.r[c][t] = ldg(G.r[c] + t);
           const_cast<T*>(P.Adiag[c])[t] = ldg(G.Adiag[c] + t);
           const_cast<int8_t*>(P.gt[c])[t] = ldg(G.gt[c] + t);
-          if (G.meta != nullptr) const_cast<uint8_t*>(P.nb[c])[t] = ldg(G.nb[c] + t);
+          if (G.meta != nullptr) const_cast<uint16_t*>(P.nb[c])[t] = ldg(G.nb[c] + t);
         }
       }
       if (G.meta != nullptr && threadIdx.x == 0) *const_cast<LevelMeta*>(P.meta) = *G.meta;
@@ -1431,7 +1490,7 @@ __device__ __forceinline__ void ap_slot(const SysDesc<T, TX>& D, const Coupling<
     stencil_load<T, DIM, COMPACT>(D.g, D.P, D.p[par], c, t, s);
     stencil_keep<T, DIM, COMPACT>(s);
     if (s.gt != MG_FLUID) continue;
-    const T ap = stencil_apply<T, DIM, COMPACT>(k, s);
+    const T ap = stencil_apply<T, DIM, COMPACT>(D.P, k, c, t, s);
     stg(D.Ap[c] + t, ap);
     acc = dd_add(acc, dd_prod(s.own, ap));
   }
@@ -1593,6 +1652,19 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
   run_flush(rs);
 }
 
+template <typename T, typename TX, int DIM, bool COMPACT>
+__device__ __forceinline__ void smooth0_dot_slot(const SysDesc<T, TX>& D, const Coupling<T>& kc,
+                                                 int t, DD& acc) {
+  const int8_t g0 = ldg(D.P.gt[0] + t);
+  const int8_t g1 = ldg(D.P.gt[1] + t);
+  const T r0 = ldg(D.P.r[0] + t);
+  const T r1 = ldg(D.P.r[1] + t);
+  const T z1 = ldg(D.P.z[1] + t);
+  const T z0 = smooth_slot_g<T, DIM, COMPACT>(D.g, D.P, kc, 0, t);
+  if (g0 == MG_FLUID) acc = dd_add(acc, dd_prod(z0, r0));
+  if (g1 == MG_FLUID) acc = dd_add(acc, dd_prod(z1, r1));
+}
+
 // Last post-smoothing half-sweep of level 0, smooth(0, 0) (mgpcg.py:148),
 // fused with reduce(z, r) (mgpcg.py:171 / 201) over both colours.
 template <typename T, typename TX, int DIM>
@@ -1616,14 +1688,8 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
     const int t = lb * BLOCK + threadIdx.x;
     DD& acc = rs.acc;
     if (t < D.g.size) {
-      const int8_t g0 = ldg(D.P.gt[0] + t);
-      const int8_t g1 = ldg(D.P.gt[1] + t);
-      const T r0 = ldg(D.P.r[0] + t);
-      const T r1 = ldg(D.P.r[1] + t);
-      const T z1 = ldg(D.P.z[1] + t);
-      const T z0 = smooth_slot_g<T, DIM>(D.g, D.P, kc, 0, t);
-      if (g0 == MG_FLUID) acc = dd_add(acc, dd_prod(z0, r0));
-      if (g1 == MG_FLUID) acc = dd_add(acc, dd_prod(z1, r1));
+      if (kc.uniform) smooth0_dot_slot<T, TX, DIM, true>(D, kc, t, acc);
+      else smooth0_dot_slot<T, TX, DIM, false>(D, kc, t, acc);
     }
   }
   run_flush(rs);
@@ -1903,8 +1969,22 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
-// Packs one level from the canonical arrays, computes every slot's coupling
-// flags, and folds the level's nonzero coupling bit patterns into *meta.
+template <typename T>
+__device__ __forceinline__ bool same_bits(T a, T b) {
+  unsigned char x[sizeof(T)], y[sizeof(T)];
+  __builtin_memcpy(x, &a, sizeof(T));
+  __builtin_memcpy(y, &b, sizeof(T));
+  for (unsigned i = 0; i < sizeof(T); ++i)
+    if (x[i] != y[i]) return false;
+  return true;
+}
+
+// Packs one level from the canonical arrays, computes every slot's flag word
+// (coupling flags and diagonal code), and folds the level's nonzero coupling
+// bit patterns into *meta.  A slot's diagonal code is checked against the
+// first nonzero coupling of the slot itself: on a level whose couplings turn
+// out uniform that is the level's value, and the kernels read codes only on
+// such levels.
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __restrict__ gt,
                                                       const float* __restrict__ Adiag,
@@ -1918,6 +1998,8 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
       Slot s = decode(g, t, c);
       int8_t gv = MG_SOLID;
       T ad = (T)0, a[3] = {(T)0, (T)0, (T)0};
+      T cl = (T)0;  // the slot's first nonzero coupling
+      bool have = false;
       uint32_t f = 0u;
       if (s.interior) {
         const long ci = canon_index(g, s.i, s.j, s.k);
@@ -1932,6 +2014,8 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
             f |= 1u << k;
             mn = ub < mn ? ub : mn;
             mx = ub > mx ? ub : mx;
+            if (!have) cl = (T)up;
+            have = true;
           }
           // lower neighbour on original axis k (view axis k + 3 - dim)
           const int va = k + 3 - g.dim;
@@ -1939,14 +2023,26 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
             int w[3] = {s.i, s.j, s.k};
             w[va] -= 1;
             const float dn = Ax[canon_index(g, w[0], w[1], w[2]) * g.dim + k];
-            if (__float_as_uint(dn) != 0u) f |= 1u << (NB_DN + k);
+            if (__float_as_uint(dn) != 0u) {
+              f |= 1u << (NB_DN + k);
+              if (!have) cl = (T)dn;
+              have = true;
+            }
           }
         }
       }
       ((int8_t*)P.gt[c])[t] = gv;
       ((T*)P.Adiag[c])[t] = ad;
       for (int k = 0; k < g.dim; ++k) ((T*)P.Ax[k][c])[t] = a[k];
-      ((uint8_t*)P.nb[c])[t] = (uint8_t)f;
+      uint32_t code = NB_AD_STORED;
+      for (uint32_t n = 0; n < NB_AD_STORED; ++n) {
+        if (n > 0 && !have) break;
+        if (same_bits(diag_of_code<T>(n, cl), ad)) {
+          code = n;
+          break;
+        }
+      }
+      ((uint16_t*)P.nb[c])[t] = (uint16_t)(f | (code << NB_AD));
       P.r[c][t] = (T)0;
       P.z[c][t] = (T)0;
     }

@@ -48,7 +48,7 @@ struct LevelBuf {
   void* Ax[3][2];
   void* r[2];
   void* z[2];
-  uint8_t* nb[2];    // coupling flags
+  uint16_t* nb[2];   // flag words (coupling flags, diagonal code)
   LevelMeta* meta;   // nonzero-coupling summary, written when the level is loaded
   void* res[2];      // restriction scratch (NULL on the coarsest level)
 };
@@ -1036,7 +1036,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       for (int a = 0; a < 3; ++a) o.Ax[a][c] = (a < dim) ? take(g.size * s->esz) : (size_t)-1;
       o.r[c] = take(g.size * s->esz);
       o.z[c] = take(g.size * s->esz);
-      o.nb[c] = take(g.size);
+      o.nb[c] = take(g.size * sizeof(uint16_t));
       o.res[c] = (l + 1 < n_levels) ? take(g.size * s->esz) : (size_t)-1;
     }
     o.meta = take(sizeof(LevelMeta));
@@ -1072,7 +1072,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
         B.Ax[a][c] = (a < dim) ? (void*)(s->pool + lo[l].Ax[a][c]) : nullptr;
       B.r[c] = s->pool + lo[l].r[c];
       B.z[c] = s->pool + lo[l].z[c];
-      B.nb[c] = (uint8_t*)(s->pool + lo[l].nb[c]);
+      B.nb[c] = (uint16_t*)(s->pool + lo[l].nb[c]);
       B.res[c] = (l + 1 < n_levels) ? (void*)(s->pool + lo[l].res[c]) : nullptr;
     }
     B.meta = (LevelMeta*)(s->pool + lo[l].meta);
