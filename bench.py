@@ -21,8 +21,10 @@ Also reported, on the same requests:
                                 an FP64 MGPCG solve to 1e-12 relative residual;
   roofline                      the level-0 red-black half-sweep, its
                                 algorithmic bytes over its launch durations
-                                measured with HIP events on the engine stream
-                                during the timed steps;
+                                measured on the device (first workgroup start
+                                to last workgroup end) during the timed steps;
+  roofline_isolated             the same with one engine stream, where no
+                                other lane's kernels share the device;
   cpu_baseline                  the oracle's C restatement of mgpcg.py on the
                                 host cores (OpenMP), a bounded sample.
 """

@@ -178,12 +178,14 @@ int mgpcg_server_poll(mgpcg_server_t* server, int64_t* tags, mgpcg_result_t* res
 int mgpcg_server_pending(const mgpcg_server_t* server, int* n_pending, int* n_active,
                          int* inflight);
 
-/* Measurement (no reference counterpart): enable = 1 starts recording HIP
- * events on the engine's stream around one in MGPCG_OPT_PROBE_EVERY launches of the level-0
- * red-black half-sweep (the dominant kernel, mg::k_stage<..., 0, 1>);
- * enable = 0 stops and returns the number of timed launches, their summed
- * device time and the level-0 cells those launches swept (counted on the
- * device: systems already finished are not swept). */
+/* Measurement (no reference counterpart): enable = 1 starts timing one in
+ * MGPCG_OPT_PROBE_EVERY launches of the level-0 red-black half-sweep (the
+ * dominant kernel, mg::k_stage<..., 0, 1>) on the device: a timed launch
+ * records the device clock when its first workgroup starts and when its last
+ * one ends.  enable = 0 stops and returns the number of timed launches, their
+ * summed device time and the level-0 cells those launches swept (counted on
+ * the device: systems already finished are not swept).  A server sums over
+ * its engine streams. */
 int mgpcg_server_probe(mgpcg_server_t* server, int enable, int* n_launches, double* total_ms,
                        int64_t* total_cells);
 int mgpcg_batch_probe(mgpcg_batch_t* batch, int enable, int* n_launches, double* total_ms,

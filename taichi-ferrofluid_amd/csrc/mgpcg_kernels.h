@@ -138,8 +138,13 @@ struct Stage {
   int nitems;
   int nblocks;  // total virtual blocks
   // measurement only: when non-NULL, a level-0 smooth(0) launch adds the
-  // level-0 cells of every system it sweeps (done systems are skipped)
+  // level-0 cells of every system it sweeps (done systems are skipped) to
+  // *swept, and its span on the device's constant-rate clock to *t0 (the
+  // earliest workgroup start, atomic min) and *t1 (the latest workgroup
+  // end, atomic max)
   unsigned long long* swept;
+  unsigned long long* t0;
+  unsigned long long* t1;
 };
 
 // ---------------------------------------------------------------------------
@@ -887,6 +892,9 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
   ItemCache ic;
   bool done = true;
   Coupling<T> kc{false, (T)0};
+  if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
+    if (S.t0 != nullptr && threadIdx.x == 0) atomicMin(S.t0, (unsigned long long)wall_clock64());
+  }
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
     if (item_at(S, it.vb, ic)) {
       const LevelDesc<T>& D = descs[ic.desc];
@@ -931,6 +939,12 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     } else if (OP == ST_PROLONGATE) {
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 0, t);
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 1, t);
+    }
+  }
+  if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
+    if (S.t1 != nullptr) {  // uniform: a kernel argument
+      __syncthreads();      // every wave of the workgroup has finished
+      if (threadIdx.x == 0) atomicMax(S.t1, (unsigned long long)wall_clock64());
     }
   }
 }

@@ -13,7 +13,6 @@
 //               boundaries -- which is also how the server does continuous
 //               batching.
 #include <hip/hip_runtime.h>
-#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstdarg>
@@ -227,12 +226,13 @@ struct Engine {
   char* h_desc = nullptr;
   CGState* h_stinit = nullptr;
 
-  // launch-duration probe of the level-0 half-sweeps (HIP events on `stream`)
+  // launch-duration probe of the level-0 half-sweeps, timed on the device
   bool probe_on = false;
-  std::vector<hipEvent_t> probe_ev;  // begin/end pairs
-  int probe_used = 0;  // pairs recorded
+  int probe_used = 0;  // launches recorded
   int probe_seen = 0;  // eligible launches since the probe started
   unsigned long long* d_swept = nullptr;  // level-0 cells swept by probed launches
+  // per recorded launch: earliest workgroup start [i], latest end [PROBE_POOL + i]
+  unsigned long long* d_span = nullptr;
 
   ~Engine() { release(); }
   void release() {
@@ -253,8 +253,7 @@ struct Engine {
       if (h_states[i]) (void)hipHostFree(h_states[i]);
       if (ev_states[i]) (void)hipEventDestroy(ev_states[i]);
     }
-    for (hipEvent_t ev : probe_ev) (void)hipEventDestroy(ev);
-    probe_ev.clear();
+    if (d_span) (void)hipFree(d_span);
     if (d_swept) (void)hipFree(d_swept);
     d_swept = nullptr;
     if (ev_user) (void)hipEventDestroy(ev_user);
@@ -565,28 +564,17 @@ int engine_build_items(Engine* e) {
   return 0;
 }
 
-// Probe: an event pair around a launch; pairs beyond the pool are not recorded.
+// Probe: launches beyond the pool are not recorded.
 const int PROBE_POOL = 8192;
-// Returns whether the launch is recorded; a recorded launch also counts the
-// cells it sweeps into d_swept (on the device, so systems that finished on
-// the device but are not retired yet are not counted).  Every
-// g_probe_every-th eligible launch is timed.  The event pair travels with
-// the kernel's own dispatch (hipExtLaunchKernelGGL), so it times the
-// kernel's execution rather than the queue around it.
+// Returns whether the launch is recorded.  Every g_probe_every-th eligible
+// launch is; it counts the cells it sweeps into d_swept (on the device, so
+// systems that finished on the device but are not retired yet are not
+// counted) and records its span from the first workgroup's start to the last
+// workgroup's end on the device clock, i.e. the kernel's execution and not
+// the queue around it.
 bool probe_begin(Engine* e) {
   if (e->probe_seen++ % g_probe_every != 0) return false;
-  if (e->probe_used >= PROBE_POOL || e->d_swept == nullptr) return false;
-  if ((int)e->probe_ev.size() < 2 * (e->probe_used + 1)) {
-    hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess) return false;
-    if (hipEventCreate(&b) != hipSuccess) {
-      (void)hipEventDestroy(a);
-      return false;
-    }
-    e->probe_ev.push_back(a);
-    e->probe_ev.push_back(b);
-  }
-  return true;
+  return e->probe_used < PROBE_POOL && e->d_swept != nullptr && e->d_span != nullptr;
 }
 void probe_end(Engine* e) { e->probe_used++; }
 
@@ -596,6 +584,8 @@ inline Stage stage_of(const Engine* e, const Engine::StageRef& r) {
   s.nitems = r.n;
   s.nblocks = r.nblocks;
   s.swept = nullptr;
+  s.t0 = nullptr;
+  s.t1 = nullptr;
   return s;
 }
 inline int grid_of(const Engine::StageRef& r) { return std::min(r.nblocks, g_max_grid); }
@@ -616,13 +606,11 @@ struct Sched {
       Stage st = stage_of(e, r);
       if (probe) {
         st.swept = e->d_swept;
-        hipExtLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0,
-                              e->stream, e->probe_ev[2 * e->probe_used],
-                              e->probe_ev[2 * e->probe_used + 1], 0, levels(e), st);
-      } else {
-        hipLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0,
-                           e->stream, levels(e), st);
+        st.t0 = e->d_span + e->probe_used;
+        st.t1 = e->d_span + PROBE_POOL + e->probe_used;
       }
+      hipLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
+                         levels(e), st);
     } else
       hipLaunchKernelGGL((k_stage<T, DIM, OP, 0>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
                          levels(e), stage_of(e, r));
@@ -1884,7 +1872,13 @@ namespace {
 int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells) {
   if (enable) {
     if (e->d_swept == nullptr) MG_HIP(hipMalloc(&e->d_swept, sizeof(unsigned long long)));
+    if (e->d_span == nullptr)
+      MG_HIP(hipMalloc(&e->d_span, 2 * PROBE_POOL * sizeof(unsigned long long)));
     MG_HIP(hipMemsetAsync(e->d_swept, 0, sizeof(unsigned long long), e->stream));
+    // starts to the largest value (atomic min), ends to 0 (atomic max)
+    MG_HIP(hipMemsetAsync(e->d_span, 0xff, PROBE_POOL * sizeof(unsigned long long), e->stream));
+    MG_HIP(hipMemsetAsync(e->d_span + PROBE_POOL, 0, PROBE_POOL * sizeof(unsigned long long),
+                          e->stream));
     e->probe_on = true;
     e->probe_used = 0;
     e->probe_seen = 0;
@@ -1893,10 +1887,18 @@ int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells
   e->probe_on = false;
   MG_HIP(hipStreamSynchronize(e->stream));
   double tot = 0.0;
-  for (int i = 0; i < e->probe_used; ++i) {
-    float ms = 0.0f;
-    MG_HIP(hipEventElapsedTime(&ms, e->probe_ev[2 * i], e->probe_ev[2 * i + 1]));
-    tot += ms;
+  if (e->probe_used > 0) {
+    std::vector<unsigned long long> t0(e->probe_used), t1(e->probe_used);
+    MG_HIP(hipMemcpy(t0.data(), e->d_span, e->probe_used * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+    MG_HIP(hipMemcpy(t1.data(), e->d_span + PROBE_POOL,
+                     e->probe_used * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    int dev = 0, khz = 0;
+    MG_HIP(hipGetDevice(&dev));
+    MG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    MG_REQUIRE(khz > 0, "device clock rate unknown");
+    for (int i = 0; i < e->probe_used; ++i)
+      if (t1[i] > t0[i]) tot += (double)(t1[i] - t0[i]) / (double)khz;  // ms
   }
   unsigned long long c = 0;
   if (e->d_swept) MG_HIP(hipMemcpy(&c, e->d_swept, sizeof(c), hipMemcpyDeviceToHost));

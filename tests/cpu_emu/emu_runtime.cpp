@@ -209,6 +209,18 @@ const char* hipGetErrorString(hipError_t e) {
   }
 }
 hipError_t hipGetLastError(void) { return hipSuccess; }
+unsigned long long wall_clock64() {
+  return (unsigned long long)(std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                  std::chrono::steady_clock::now().time_since_epoch())
+                                  .count() /
+                              10);
+}
+hipError_t hipDeviceGetAttribute(int* value, hipDeviceAttribute_t attr, int device) {
+  (void)device;
+  if (attr != hipDeviceAttributeWallClockRate) return hipErrorInvalidValue;
+  *value = 100000;  // kHz
+  return hipSuccess;
+}
 hipError_t hipGetDevice(int* d) {
   *d = 0;
   return hipSuccess;

@@ -109,6 +109,18 @@ inline unsigned atomicMax(unsigned* p, unsigned v) {
   *p = v > old ? v : old;
   return old;
 }
+inline unsigned long long atomicMin(unsigned long long* p, unsigned long long v) {
+  const unsigned long long old = *p;
+  *p = v < old ? v : old;
+  return old;
+}
+inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
+  const unsigned long long old = *p;
+  *p = v > old ? v : old;
+  return old;
+}
+// the device's constant-rate clock: 100 MHz ticks of the host's steady clock
+unsigned long long wall_clock64();
 inline unsigned __float_as_uint(float f) {
   unsigned u;
   memcpy(&u, &f, sizeof u);
@@ -129,6 +141,8 @@ inline unsigned __umulhi(unsigned a, unsigned b) {
 const char* hipGetErrorString(hipError_t e);
 hipError_t hipGetLastError(void);
 hipError_t hipGetDevice(int* d);
+enum hipDeviceAttribute_t { hipDeviceAttributeWallClockRate = 1 };
+hipError_t hipDeviceGetAttribute(int* value, hipDeviceAttribute_t attr, int device);
 hipError_t hipSetDevice(int d);
 hipError_t hipDeviceSynchronize(void);
 hipError_t hipMalloc(void** p, size_t bytes);
