@@ -133,15 +133,17 @@ struct Item {
   int block0;  // first virtual block of this item
 };
 
+constexpr int SPAN_SLOTS = 256;
+
 struct Stage {
   const Item* items;
   int nitems;
   int nblocks;  // total virtual blocks
   // measurement only: when non-NULL, a level-0 smooth(0) launch adds the
   // level-0 cells of every system it sweeps (done systems are skipped) to
-  // *swept, and its span on the device's constant-rate clock to *t0 (the
-  // earliest workgroup start, atomic min) and *t1 (the latest workgroup
-  // end, atomic max)
+  // *swept, and its span on the device's constant-rate clock: workgroup b
+  // folds its start into t0[b % SPAN_SLOTS] (atomic min) and its end into
+  // t1[b % SPAN_SLOTS] (atomic max), so few workgroups share an address
   unsigned long long* swept;
   unsigned long long* t0;
   unsigned long long* t1;
@@ -893,7 +895,8 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
   bool done = true;
   Coupling<T> kc{false, (T)0};
   if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
-    if (S.t0 != nullptr && threadIdx.x == 0) atomicMin(S.t0, (unsigned long long)wall_clock64());
+    if (S.t0 != nullptr && threadIdx.x == 0)
+      atomicMin(S.t0 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
   }
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
     if (item_at(S, it.vb, ic)) {
@@ -944,7 +947,8 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
   if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
     if (S.t1 != nullptr) {  // uniform: a kernel argument
       __syncthreads();      // every wave of the workgroup has finished
-      if (threadIdx.x == 0) atomicMax(S.t1, (unsigned long long)wall_clock64());
+      if (threadIdx.x == 0)
+        atomicMax(S.t1 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
     }
   }
 }

@@ -231,7 +231,8 @@ struct Engine {
   int probe_used = 0;  // launches recorded
   int probe_seen = 0;  // eligible launches since the probe started
   unsigned long long* d_swept = nullptr;  // level-0 cells swept by probed launches
-  // per recorded launch: earliest workgroup start [i], latest end [PROBE_POOL + i]
+  // per recorded launch i: workgroup starts folded into SPAN_SLOTS minima at
+  // [i * SPAN_SLOTS], ends into maxima at [(PROBE_POOL + i) * SPAN_SLOTS]
   unsigned long long* d_span = nullptr;
 
   ~Engine() { release(); }
@@ -565,7 +566,7 @@ int engine_build_items(Engine* e) {
 }
 
 // Probe: launches beyond the pool are not recorded.
-const int PROBE_POOL = 8192;
+const int PROBE_POOL = 4096;
 // Returns whether the launch is recorded.  Every g_probe_every-th eligible
 // launch is; it counts the cells it sweeps into d_swept (on the device, so
 // systems that finished on the device but are not retired yet are not
@@ -606,8 +607,8 @@ struct Sched {
       Stage st = stage_of(e, r);
       if (probe) {
         st.swept = e->d_swept;
-        st.t0 = e->d_span + e->probe_used;
-        st.t1 = e->d_span + PROBE_POOL + e->probe_used;
+        st.t0 = e->d_span + (size_t)e->probe_used * SPAN_SLOTS;
+        st.t1 = e->d_span + (size_t)(PROBE_POOL + e->probe_used) * SPAN_SLOTS;
       }
       hipLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
                          levels(e), st);
@@ -1872,13 +1873,13 @@ namespace {
 int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells) {
   if (enable) {
     if (e->d_swept == nullptr) MG_HIP(hipMalloc(&e->d_swept, sizeof(unsigned long long)));
+    const size_t half = (size_t)PROBE_POOL * SPAN_SLOTS;
     if (e->d_span == nullptr)
-      MG_HIP(hipMalloc(&e->d_span, 2 * PROBE_POOL * sizeof(unsigned long long)));
+      MG_HIP(hipMalloc(&e->d_span, 2 * half * sizeof(unsigned long long)));
     MG_HIP(hipMemsetAsync(e->d_swept, 0, sizeof(unsigned long long), e->stream));
     // starts to the largest value (atomic min), ends to 0 (atomic max)
-    MG_HIP(hipMemsetAsync(e->d_span, 0xff, PROBE_POOL * sizeof(unsigned long long), e->stream));
-    MG_HIP(hipMemsetAsync(e->d_span + PROBE_POOL, 0, PROBE_POOL * sizeof(unsigned long long),
-                          e->stream));
+    MG_HIP(hipMemsetAsync(e->d_span, 0xff, half * sizeof(unsigned long long), e->stream));
+    MG_HIP(hipMemsetAsync(e->d_span + half, 0, half * sizeof(unsigned long long), e->stream));
     e->probe_on = true;
     e->probe_used = 0;
     e->probe_seen = 0;
@@ -1888,17 +1889,24 @@ int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells
   MG_HIP(hipStreamSynchronize(e->stream));
   double tot = 0.0;
   if (e->probe_used > 0) {
-    std::vector<unsigned long long> t0(e->probe_used), t1(e->probe_used);
-    MG_HIP(hipMemcpy(t0.data(), e->d_span, e->probe_used * sizeof(unsigned long long),
+    const size_t n = (size_t)e->probe_used * SPAN_SLOTS;
+    std::vector<unsigned long long> t0(n), t1(n);
+    MG_HIP(hipMemcpy(t0.data(), e->d_span, n * sizeof(unsigned long long),
                      hipMemcpyDeviceToHost));
-    MG_HIP(hipMemcpy(t1.data(), e->d_span + PROBE_POOL,
-                     e->probe_used * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    MG_HIP(hipMemcpy(t1.data(), e->d_span + (size_t)PROBE_POOL * SPAN_SLOTS,
+                     n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     int dev = 0, khz = 0;
     MG_HIP(hipGetDevice(&dev));
     MG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     MG_REQUIRE(khz > 0, "device clock rate unknown");
-    for (int i = 0; i < e->probe_used; ++i)
-      if (t1[i] > t0[i]) tot += (double)(t1[i] - t0[i]) / (double)khz;  // ms
+    for (int i = 0; i < e->probe_used; ++i) {
+      unsigned long long a = ~0ull, b = 0ull;
+      for (int k = 0; k < SPAN_SLOTS; ++k) {
+        a = std::min(a, t0[(size_t)i * SPAN_SLOTS + k]);
+        b = std::max(b, t1[(size_t)i * SPAN_SLOTS + k]);
+      }
+      if (b > a) tot += (double)(b - a) / (double)khz;  // ms
+    }
   }
   unsigned long long c = 0;
   if (e->d_swept) MG_HIP(hipMemcpy(&c, e->d_swept, sizeof(c), hipMemcpyDeviceToHost));
