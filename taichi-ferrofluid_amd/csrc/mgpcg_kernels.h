@@ -2078,9 +2078,13 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
       mn = smn[w] < mn ? smn[w] : mn;
       mx = smx[w] > mx ? smx[w] : mx;
     }
+    // a value that would not move the level's current extreme skips the
+    // atomic: with one coupling value on the level (the usual case) only the
+    // first workgroups update it, instead of every workgroup queueing on the
+    // same address (a stale read only costs a redundant atomic)
     if (mx != 0u) {
-      atomicMin(&meta->minbits, mn);
-      atomicMax(&meta->maxbits, mx);
+      if (mn < __atomic_load_n(&meta->minbits, __ATOMIC_RELAXED)) atomicMin(&meta->minbits, mn);
+      if (mx > __atomic_load_n(&meta->maxbits, __ATOMIC_RELAXED)) atomicMax(&meta->maxbits, mx);
     }
   }
 }

@@ -30,7 +30,8 @@ constexpr int LB = 256;
 struct Grid {
   int dim;
   int n[3];
-  long st[3];  // C-order strides
+  long st[3];          // C-order strides
+  mg::FastDiv dn[3];   // division by n[a] (coords below 2^31)
   __host__ __device__ long cells() const { return (long)n[0] * n[1] * n[2]; }
 };
 
@@ -41,10 +42,21 @@ Grid make_grid(int dim, const int32_t* res) {
   g.st[dim - 1] = 1;
   for (int a = dim - 2; a >= 0; --a) g.st[a] = g.st[a + 1] * g.n[a + 1];
   for (int a = dim; a < 3; ++a) g.st[a] = 0;
+  for (int a = 0; a < 3; ++a) g.dn[a] = mg::FastDiv((uint32_t)g.n[a]);
   return g;
 }
 
 __device__ __forceinline__ void coords(const Grid& g, long t, int* I) {
+  if (t < 0x80000000L) {  // multiply-high division, no 64-bit divide
+    uint32_t u = (uint32_t)t;
+    for (int a = g.dim - 1; a >= 0; --a) {
+      const uint32_t q = g.dn[a].div(u);
+      I[a] = (int)(u - q * (uint32_t)g.n[a]);
+      u = q;
+    }
+    for (int a = g.dim; a < 3; ++a) I[a] = 0;
+    return;
+  }
   for (int a = g.dim - 1; a >= 0; --a) {
     I[a] = (int)(t % g.n[a]);
     t /= g.n[a];

@@ -24,6 +24,7 @@ constexpr int SB = 256;
 struct Shape {
   int dim;
   int n[3];  // C-order dims (n[2] unused in 2D)
+  mg::FastDiv dn1, dn2;  // division by n[1] and n[2] (coords below 2^31)
   __host__ __device__ long cells() const {
     return dim == 3 ? (long)n[0] * n[1] * n[2] : (long)n[0] * n[1];
   }
@@ -31,6 +32,20 @@ struct Shape {
     return dim == 3 ? ((long)I[0] * n[1] + I[1]) * n[2] + I[2] : (long)I[0] * n[1] + I[1];
   }
   __host__ __device__ void coords(long t, int* I) const {
+    if (t < 0x80000000L) {  // multiply-high division, no 64-bit divide
+      const uint32_t u = (uint32_t)t;
+      if (dim == 3) {
+        const uint32_t r = dn2.div(u);
+        I[2] = (int)(u - r * (uint32_t)n[2]);
+        I[0] = (int)dn1.div(r);
+        I[1] = (int)(r - (uint32_t)I[0] * (uint32_t)n[1]);
+      } else {
+        I[0] = (int)dn1.div(u);
+        I[1] = (int)(u - (uint32_t)I[0] * (uint32_t)n[1]);
+        I[2] = 0;
+      }
+      return;
+    }
     if (dim == 3) {
       I[2] = (int)(t % n[2]);
       long r = t / n[2];
@@ -54,6 +69,8 @@ Shape make_shape(int dim, const int32_t* res, int level = 0, int grow_axis = -1)
   s.dim = dim;
   for (int a = 0; a < 3; ++a) s.n[a] = 1;
   for (int a = 0; a < dim; ++a) s.n[a] = (res[a] >> level) + (a == grow_axis ? 1 : 0);
+  s.dn1 = mg::FastDiv((uint32_t)s.n[1]);
+  s.dn2 = mg::FastDiv((uint32_t)s.n[2]);
   return s;
 }
 
