@@ -2003,7 +2003,7 @@ __device__ __forceinline__ bool same_bits(T a, T b) {
 // first nonzero coupling of the slot itself: on a level whose couplings turn
 // out uniform that is the level's value, and the kernels read codes only on
 // such levels.
-template <typename T>
+template <typename T, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __restrict__ gt,
                                                       const float* __restrict__ Adiag,
                                                       const float* __restrict__ Ax,
@@ -2011,47 +2011,67 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   uint32_t mn = 0xffffffffu, mx = 0u;
   if (t < g.size) {
+    // every canonical value of both colours is loaded first (ghost slots and
+    // missing lower neighbours read cell 0 and discard it)
+    bool inn[2], hdn[2][DIM];
+    int8_t gv[2];
+    float adv[2], up[2][DIM], dn[2][DIM];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      Slot s = decode(g, t, c);
-      int8_t gv = MG_SOLID;
+      const Slot sl = decode(g, t, c);
+      inn[c] = sl.interior;
+      const long ci = sl.interior ? canon_index(g, sl.i, sl.j, sl.k) : 0;
+      const int v[3] = {sl.i, sl.j, sl.k};
+      long wi[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        // lower neighbour on original axis k (view axis k + 3 - DIM)
+        const int va = k + 3 - DIM;
+        hdn[c][k] = sl.interior && v[va] > 0;
+        int w[3] = {sl.i, sl.j, sl.k};
+        w[va] -= 1;
+        wi[k] = hdn[c][k] ? canon_index(g, w[0], w[1], w[2]) : 0;
+      }
+      gv[c] = gt[ci];
+      adv[c] = Adiag[ci];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        up[c][k] = Ax[ci * DIM + k];
+        dn[c][k] = Ax[wi[k] * DIM + k];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      int8_t gvc = MG_SOLID;
       T ad = (T)0, a[3] = {(T)0, (T)0, (T)0};
       T cl = (T)0;  // the slot's first nonzero coupling
       bool have = false;
       uint32_t f = 0u;
-      if (s.interior) {
-        const long ci = canon_index(g, s.i, s.j, s.k);
-        gv = gt[ci];
-        ad = (T)Adiag[ci];
-        const int v[3] = {s.i, s.j, s.k};
-        for (int k = 0; k < g.dim; ++k) {
-          const float up = Ax[ci * g.dim + k];
-          a[k] = (T)up;
-          const uint32_t ub = __float_as_uint(up);
+      if (inn[c]) {
+        gvc = gv[c];
+        ad = (T)adv[c];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+          a[k] = (T)up[c][k];
+          const uint32_t ub = __float_as_uint(up[c][k]);
           if (ub != 0u) {
             f |= 1u << k;
             mn = ub < mn ? ub : mn;
             mx = ub > mx ? ub : mx;
-            if (!have) cl = (T)up;
+            if (!have) cl = (T)up[c][k];
             have = true;
           }
-          // lower neighbour on original axis k (view axis k + 3 - dim)
-          const int va = k + 3 - g.dim;
-          if (v[va] > 0) {
-            int w[3] = {s.i, s.j, s.k};
-            w[va] -= 1;
-            const float dn = Ax[canon_index(g, w[0], w[1], w[2]) * g.dim + k];
-            if (__float_as_uint(dn) != 0u) {
-              f |= 1u << (NB_DN + k);
-              if (!have) cl = (T)dn;
-              have = true;
-            }
+          if (hdn[c][k] && __float_as_uint(dn[c][k]) != 0u) {
+            f |= 1u << (NB_DN + k);
+            if (!have) cl = (T)dn[c][k];
+            have = true;
           }
         }
       }
-      ((int8_t*)P.gt[c])[t] = gv;
+      ((int8_t*)P.gt[c])[t] = gvc;
       ((T*)P.Adiag[c])[t] = ad;
-      for (int k = 0; k < g.dim; ++k) ((T*)P.Ax[k][c])[t] = a[k];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) ((T*)P.Ax[k][c])[t] = a[k];
       uint32_t code = NB_AD_STORED;
       for (uint32_t n = 0; n < NB_AD_STORED; ++n) {
         if (n > 0 && !have) break;
@@ -2083,8 +2103,10 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
     // first workgroups update it, instead of every workgroup queueing on the
     // same address (a stale read only costs a redundant atomic)
     if (mx != 0u) {
-      if (mn < __atomic_load_n(&meta->minbits, __ATOMIC_RELAXED)) atomicMin(&meta->minbits, mn);
-      if (mx > __atomic_load_n(&meta->maxbits, __ATOMIC_RELAXED)) atomicMax(&meta->maxbits, mx);
+      const uint32_t cur_mn = *(volatile const uint32_t*)&meta->minbits;
+      const uint32_t cur_mx = *(volatile const uint32_t*)&meta->maxbits;
+      if (mn < cur_mn) atomicMin(&meta->minbits, mn);
+      if (mx > cur_mx) atomicMax(&meta->maxbits, mx);
     }
   }
 }

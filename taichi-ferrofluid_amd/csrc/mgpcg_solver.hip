@@ -1188,14 +1188,22 @@ extern "C" int mgpcg_load_level(mgpcg_solver_t* s, int level, const int8_t* grid
   const Geo& g = s->geo[level];
   LevelMeta* meta = s->lv[level].meta;
   hipLaunchKernelGGL(k_meta_init, dim3(1), dim3(64), 0, s->user_stream, meta);
-  if (s->precision == F64)
-    hipLaunchKernelGGL((k_pack_level<double>), dim3(nblk(g.size)), dim3(BLOCK), 0,
-                       s->user_stream, g, grid_type, Adiag, Ax, level_ptrs<double>(s, level),
-                       meta);
-  else
-    hipLaunchKernelGGL((k_pack_level<float>), dim3(nblk(g.size)), dim3(BLOCK), 0,
-                       s->user_stream, g, grid_type, Adiag, Ax, level_ptrs<float>(s, level),
-                       meta);
+  const dim3 grid(nblk(g.size)), block(BLOCK);
+  if (s->precision == F64) {
+    if (s->dim == 3)
+      hipLaunchKernelGGL((k_pack_level<double, 3>), grid, block, 0, s->user_stream, g,
+                         grid_type, Adiag, Ax, level_ptrs<double>(s, level), meta);
+    else
+      hipLaunchKernelGGL((k_pack_level<double, 2>), grid, block, 0, s->user_stream, g,
+                         grid_type, Adiag, Ax, level_ptrs<double>(s, level), meta);
+  } else {
+    if (s->dim == 3)
+      hipLaunchKernelGGL((k_pack_level<float, 3>), grid, block, 0, s->user_stream, g,
+                         grid_type, Adiag, Ax, level_ptrs<float>(s, level), meta);
+    else
+      hipLaunchKernelGGL((k_pack_level<float, 2>), grid, block, 0, s->user_stream, g,
+                         grid_type, Adiag, Ax, level_ptrs<float>(s, level), meta);
+  }
   MG_HIP(hipGetLastError());
   return 0;
 }

@@ -238,7 +238,43 @@ __global__ void k_pressure_b(Shape s, const int8_t* __restrict__ ct, Vel V,
   b[t] = bv;
 }
 
+// The 2 * DIM face neighbours of cell I: their types (out of range counts as
+// SOLID) and, when `phi` is given, their level-set values (0 out of range,
+// where the type is SOLID and phi is not used).  All loads are issued before
+// any is used: one round trip per cell instead of one per neighbour.
+template <int DIM>
+struct Faces {
+  int type[DIM][2];  // [axis][0: -e_k, 1: +e_k]
+  float phi[DIM][2];
+};
+template <int DIM>
+__device__ __forceinline__ void load_faces(const Shape& s, const int8_t* __restrict__ ct,
+                                           const float* __restrict__ phi, const int* I, long t,
+                                           Faces<DIM>& f) {
+  bool in[DIM][2];
+  long at[DIM][2];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      int J[3] = {I[0], I[1], I[2]};
+      J[k] += h ? 1 : -1;
+      in[k][h] = s.inside(J);
+      at[k][h] = in[k][h] ? s.idx(J) : t;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < DIM; ++k)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int g = ct[at[k][h]];
+      f.type[k][h] = in[k][h] ? g : MG_SOLID;
+      f.phi[k][h] = phi ? phi[at[k][h]] : 0.0f;
+    }
+}
+
 // PressureProjectStrategy.build_A_kernel (pressure_project.py:45-64).
+template <int DIM>
 __global__ void k_pressure_A(Shape s, int level, const int8_t* __restrict__ gt,
                              const float* __restrict__ phi, float sA, int gfm, float c_max,
                              float* __restrict__ Adiag, float* __restrict__ Ax) {
@@ -248,17 +284,22 @@ __global__ void k_pressure_A(Shape s, int level, const int8_t* __restrict__ gt,
   s.coords(t, I);
   float ad = 0.0f, ax[3] = {0.0f, 0.0f, 0.0f};
   if (gt[t] == MG_FLUID) {
-    for (int k = 0; k < s.dim; ++k) {
-      for (int sgn = -1; sgn <= 1; sgn += 2) {
-        const int nb = nb_type(gt, s, I, k, sgn);
+    const bool use_phi = gfm && level == 0;
+    const float* ph = use_phi ? phi : nullptr;
+    Faces<DIM> f;
+    load_faces<DIM>(s, gt, ph, I, t, f);
+    const float phi_t = use_phi ? phi[t] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {  // sgn = -1, then +1
+        const int nb = f.type[k][h];
         if (nb == MG_FLUID) {
           ad = ad + sA;
-          if (sgn > 0) ax[k] = -sA;
+          if (h) ax[k] = -sA;
         } else if (nb == MG_AIR) {
-          if (gfm && level == 0) {
-            int J[3] = {I[0], I[1], I[2]};
-            J[k] += sgn;
-            float c = gfm_c(phi[t], phi[s.idx(J)], c_max);
+          if (use_phi) {
+            float c = gfm_c(phi_t, f.phi[k][h], c_max);
             ad = ad + sA * c;
           } else {
             ad = ad + sA;
@@ -268,7 +309,68 @@ __global__ void k_pressure_A(Shape s, int level, const int8_t* __restrict__ gt,
     }
   }
   Adiag[t] = ad;
-  for (int k = 0; k < s.dim; ++k) Ax[t * s.dim + k] = ax[k];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) Ax[t * DIM + k] = ax[k];
+}
+
+// build_b_kernel of the pressure strategy (mode 0 of k_pressure_b) in the
+// same operation order, every load of a FLUID cell issued up front.
+template <int DIM>
+__global__ void k_pressure_b0(Shape s, const int8_t* __restrict__ ct, Vel V,
+                              const float* __restrict__ phi, float sA, float sb, float p0,
+                              int gfm, float c_max, float* __restrict__ b) {
+  const long t = (long)blockIdx.x * SB + threadIdx.x;
+  if (t >= s.cells()) return;
+  int I[3];
+  s.coords(t, I);
+  if (ct[t] != MG_FLUID) {
+    b[t] = 0.0f;
+    return;
+  }
+  float ulo[DIM], uhi[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    int J[3] = {I[0], I[1], I[2]};
+    J[k] += 1;
+    ulo[k] = V.u[k].p[V.u[k].s.idx(I)];
+    uhi[k] = V.u[k].p[V.u[k].s.idx(J)];
+  }
+  Faces<DIM> f;
+  load_faces<DIM>(s, ct, gfm ? phi : nullptr, I, t, f);
+  const float phi_t = gfm ? phi[t] : 0.0f;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    float d = ulo[k] - uhi[k];
+    acc = acc + d;
+  }
+  float bv = acc * sb;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // sgn = -1, then +1
+      const int nb = f.type[k][h];
+      if (nb == MG_SOLID) {
+        if (!h) {
+          float u = ulo[k] - 0.0f;
+          bv = bv - sb * u;
+        } else {
+          float u = uhi[k] - 0.0f;
+          bv = bv + sb * u;
+        }
+      } else if (nb == MG_AIR) {
+        float term;
+        if (gfm) {
+          float c = gfm_c(phi_t, f.phi[k][h], c_max);
+          term = (sA * c) * p0;
+        } else {
+          term = sA * p0;
+        }
+        bv = bv + term;
+      }
+    }
+  }
+  b[t] = bv;
 }
 
 // PressureProjectWithVolumeControlStrategy.calc_volume (volume_control.py:22-31):
@@ -674,12 +776,16 @@ extern "C" int mgpcg_pressure_build_b(void* stream, int dim, const int32_t* res,
   for (int k = 0; k < dim; ++k) MG_REQUIRE(velocity[k], "velocity[%d] is NULL", k);
   MG_REQUIRE(!ghost_fluid_method || phi, "ghost-fluid method needs phi");
   Shape s = make_shape(dim, res);
-  HField none;
-  memset(&none, 0, sizeof(none));
-  hipLaunchKernelGGL(k_pressure_b, dim3(nb_for(s.cells())), dim3(SB), 0, (hipStream_t)stream, s,
-                     cell_type, make_vel(dim, res, velocity), phi, (float)scale_A,
-                     (float)scale_b, (float)p0, ghost_fluid_method, (float)c_max, 0, none, 0.0f,
-                     b);
+  if (dim == 3)
+    hipLaunchKernelGGL((k_pressure_b0<3>), dim3(nb_for(s.cells())), dim3(SB), 0,
+                       (hipStream_t)stream, s, cell_type, make_vel(dim, res, velocity), phi,
+                       (float)scale_A, (float)scale_b, (float)p0, ghost_fluid_method,
+                       (float)c_max, b);
+  else
+    hipLaunchKernelGGL((k_pressure_b0<2>), dim3(nb_for(s.cells())), dim3(SB), 0,
+                       (hipStream_t)stream, s, cell_type, make_vel(dim, res, velocity), phi,
+                       (float)scale_A, (float)scale_b, (float)p0, ghost_fluid_method,
+                       (float)c_max, b);
   MG_HIP(hipGetLastError());
   return 0;
 }
@@ -710,9 +816,14 @@ extern "C" int mgpcg_pressure_build_A(void* stream, int dim, const int32_t* res,
   MG_REQUIRE(level >= 0, "level must be >= 0");
   MG_REQUIRE(!(ghost_fluid_method && level == 0) || phi, "ghost-fluid method needs phi");
   Shape s = make_shape(dim, res, level);
-  hipLaunchKernelGGL(k_pressure_A, dim3(nb_for(s.cells())), dim3(SB), 0, (hipStream_t)stream, s,
-                     level, grid_type, phi, (float)scale_A, ghost_fluid_method, (float)c_max,
-                     Adiag, Ax);
+  if (dim == 3)
+    hipLaunchKernelGGL((k_pressure_A<3>), dim3(nb_for(s.cells())), dim3(SB), 0,
+                       (hipStream_t)stream, s, level, grid_type, phi, (float)scale_A,
+                       ghost_fluid_method, (float)c_max, Adiag, Ax);
+  else
+    hipLaunchKernelGGL((k_pressure_A<2>), dim3(nb_for(s.cells())), dim3(SB), 0,
+                       (hipStream_t)stream, s, level, grid_type, phi, (float)scale_A,
+                       ghost_fluid_method, (float)c_max, Adiag, Ax);
   MG_HIP(hipGetLastError());
   return 0;
 }
