@@ -628,32 +628,38 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
   T ad = (T)0;
   if constexpr (!COMPACT) ad = ldg(P.Adiag[1] + t);
   const T* z0 = P.z[0];
-  T zl[DIM], zh[DIM], cl[DIM], ch[DIM], al[DIM], ah[DIM];
-  bool il[DIM], ih[DIM];
+  // The six colour-0 neighbours have four distinct parents: on every axis the
+  // neighbour inside the cell's own 2-block shares the cell's parent p, the
+  // other one (lower when the coordinate is even, upper when odd) has the
+  // adjacent parent p -+ e_a, of the other colour; a neighbour outside the
+  // level (only that second one can be) has no parent and keeps its z.
+  // A ghost slot (not interior) decodes to -1 / n coordinates: its indices are
+  // clamped into the coarse arrays and its result is discarded.
+  const int pi = (DIM == 3 && c.i > 0) ? c.i >> 1 : 0;
+  const int pj = c.j > 0 ? c.j >> 1 : 0, pk = c.k > 0 ? c.k >> 1 : 0;
+  const int cp = colour_of(pi, pj, pk);
+  const int sp = min(slot_of(gc, pi, pj, pk), gc.size - 1);
+  const T* zcp = cp ? zc1 : zc0;
+  const T* zcq = cp ? zc0 : zc1;
+  const T vp = ldg(zcp + sp);
+  T zl[DIM], zh[DIM], vq[DIM], al[DIM], ah[DIM];
+  bool up[DIM], inq[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     const int va = a + 3 - DIM;  // view axis of original axis a
-#pragma unroll
-    for (int dir = -1; dir <= 1; dir += 2) {
-      int i = c.i, j = c.j, kk = c.k;
-      if (va == 0) i += dir;
-      else if (va == 1) j += dir;
-      else kk += dir;
-      // a neighbour outside the level has no parent and keeps its z
-      const bool in = !(i < 0 || i >= g.n0 || j < 0 || j >= g.n1 || kk < 0 || kk >= g.n2);
-      const int pi = in ? i >> 1 : 0, pj = in ? j >> 1 : 0, pk = in ? kk >> 1 : 0;
-      const int ps = slot_of(gc, pi, pj, pk);
-      const T zcv = ldg((colour_of(pi, pj, pk) ? zc1 : zc0) + ps);
-      if (dir < 0) {
-        zl[a] = ldg(z0 + n.lo[a]);
-        cl[a] = zcv;
-        il[a] = in;
-      } else {
-        zh[a] = ldg(z0 + n.hi[a]);
-        ch[a] = zcv;
-        ih[a] = in;
-      }
-    }
+    const int x = va == 0 ? c.i : (va == 1 ? c.j : c.k);
+    const int nx = va == 0 ? g.n0 : (va == 1 ? g.n1 : g.n2);
+    up[a] = (x & 1) != 0;  // the other parent lies above (odd x) or below (even x)
+    const int d = up[a] ? 1 : -1;
+    inq[a] = x + d >= 0 && x + d < nx;
+    int sq;
+    if (va == 0) sq = sp + d * gc.sx;
+    else if (va == 1) sq = sp + d * gc.W;
+    else sq = sp - ((pk + 1) >> 1) + ((pk + d + 1) >> 1);
+    sq = (inq[a] && sq >= 0 && sq < gc.size) ? sq : sp;
+    vq[a] = ldg(zcq + sq);
+    zl[a] = ldg(z0 + n.lo[a]);
+    zh[a] = ldg(z0 + n.hi[a]);
   }
   uint32_t f = 0u;
   if constexpr (COMPACT) {
@@ -670,8 +676,13 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
   if (gt != MG_FLUID) return;
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    if (il[a]) zl[a] = zl[a] + cl[a];
-    if (ih[a]) zh[a] = zh[a] + ch[a];
+    if (up[a]) {
+      zl[a] = zl[a] + vp;
+      if (inq[a]) zh[a] = zh[a] + vq[a];
+    } else {
+      if (inq[a]) zl[a] = zl[a] + vq[a];
+      zh[a] = zh[a] + vp;
+    }
   }
   T ret = nsum_vals<T, DIM>(al, ah, zl, zh);
   T num = rv - ret;
