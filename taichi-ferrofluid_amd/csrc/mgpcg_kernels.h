@@ -1374,6 +1374,22 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
   }
 }
 
+// The V-cycle's first level-0 half-sweep from z = 0 (z.fill(0); smooth(0, 0),
+// mgpcg.py:126-127), z0 = r0 / Adiag0 on FLUID and 0 elsewhere, computed by
+// the kernel that has just written r0 (the r of every slot then passes
+// through registers once instead of being read back).
+template <typename T>
+__device__ __forceinline__ void first_z0(const LevelPtrs<T>& P, const Coupling<T>& k, int t,
+                                         bool fluid, T r0) {
+  T z = (T)0;
+  if (fluid) {
+    const T ad = k.uniform ? slot_diag<T, true>(P, k, 0, t, ldg(P.nb[0] + t), (T)0)
+                           : ldg(P.Adiag[0] + t);
+    z = r0 / ad;
+  }
+  stg(P.z[0] + t, z);
+}
+
 // ---------------------------------------------------------------------------
 // CG kernels on level 0 (mgpcg.py:150-238).  Reductions: one FP64 partial per
 // virtual block; a per-system finalize kernel sums them in a fixed order.
@@ -1388,8 +1404,12 @@ __global__ void __launch_bounds__(BLOCK) k_cg_init(const SysDesc<T, TX>* __restr
                                                    Stage S) {
   ItemCache ic;
   RunSum rs;
+  Coupling<T> kc{false, (T)0};
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    if (item_at(S, it.vb, ic)) run_flush(rs);
+    if (item_at(S, it.vb, ic)) {
+      run_flush(rs);
+      kc = coupling_g(sys[ic.desc].P);
+    }
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
@@ -1403,7 +1423,9 @@ __global__ void __launch_bounds__(BLOCK) k_cg_init(const SysDesc<T, TX>* __restr
         D.p[0][c][t] = (T)0;
         D.p[1][c][t] = (T)0;
         D.Ap[c][t] = (T)0;
-        if (D.P.gt[c][t] == MG_FLUID) rs.acc = dd_add(rs.acc, dd_prod(bv, bv));
+        const bool fl = D.P.gt[c][t] == MG_FLUID;
+        if (fl) rs.acc = dd_add(rs.acc, dd_prod(bv, bv));
+        if (c == 0) first_z0<T>(D.P, kc, t, fl, bv);
       }
     }
   }
@@ -1562,6 +1584,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
                                                  Stage S) {
   ItemCache ic;
   RunSum rs;
+  Coupling<T> kc{false, (T)0};
   bool done = true, refine = false;
   int par = 0, prec = PLAIN;
   float a32 = 0.0f;
@@ -1570,6 +1593,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     if (item_at(S, it.vb, ic)) {
       run_flush(rs);
       const CGState* st = sys[ic.desc].st;
+      kc = coupling_g(sys[ic.desc].P);
       done = ldg(&st->done) != 0;
       par = (ldg(&st->iter) & 1) ^ 1;  // the p written by k_cg_update_p / k_cg_ap
       prec = ldg(&st->precision);
@@ -1592,7 +1616,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
         const T apv = ldg(D.Ap[c] + t);
         const TX xv = ldg(D.x[c] + t);
         const T r0 = ldg(D.P.r[c] + t);
-        if (gt != MG_FLUID) continue;
+        if (gt != MG_FLUID) {
+          if (c == 0 && !refine) first_z0<T>(D.P, kc, t, false, (T)0);
+          continue;
+        }
         if (prec == PLAIN) {
           T tp = (T)a32 * pv;
           stg(D.x[c] + t, (TX)((T)xv + tp));
@@ -1600,6 +1627,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
           T rv = r0 - ta;
           stg(D.P.r[c] + t, rv);
           acc = dd_add(acc, dd_prod(rv, rv));
+          if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
         } else if (prec == F64) {
           T tp = (T)(alpha * (double)pv);
           stg(D.x[c] + t, (TX)((double)xv + (double)tp));
@@ -1607,12 +1635,14 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
           T rv = r0 - ta;
           stg(D.P.r[c] + t, rv);
           acc = dd_add(acc, dd_prod(rv, rv));
+          if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
         } else {
           stg(D.x[c] + t, (TX)((double)xv + alpha * (double)pv));
           if (!refine) {
             T rv = (T)((double)r0 - alpha * (double)apv);
             stg(D.P.r[c] + t, rv);
             acc = dd_add(acc, dd_prod(rv, rv));
+            if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
           }
         }
       }
@@ -1660,7 +1690,10 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
         const T ad = ldg(D.P.Adiag[c] + t);
         const TX xv = ldg(D.x[c] + t);
         const T bv = ldg(D.b[c] + t);
-        if (gt != MG_FLUID) continue;
+        if (gt != MG_FLUID) {
+          if (c == 0) stg(D.P.z[0] + t, (T)0);  // the V-cycle's first half-sweep
+          continue;
+        }
         double ret = 0.0;
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
@@ -1675,6 +1708,7 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
         T r32 = (T)rv;
         stg(D.P.r[c] + t, r32);
         acc = dd_add(acc, dd_prod(r32, r32));
+        if (c == 0) stg(D.P.z[0] + t, r32 / ad);  // the V-cycle's first half-sweep
       }
     }
   }

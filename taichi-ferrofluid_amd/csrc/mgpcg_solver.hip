@@ -640,8 +640,10 @@ struct Sched {
                        (const Item*)(e->d_items + r.off), r.n);
   }
 
-  // v_cycle (mgpcg.py:125-148) for every member
-  static void vcycle(Engine* e) {
+  // v_cycle (mgpcg.py:125-148) for every member.  first0: level 0's first
+  // half-sweep from z = 0 was done by the kernel that wrote r (k_cg_init,
+  // k_cg_xr, k_true_residual).
+  static void vcycle(Engine* e, bool first0 = false) {
     const int L = e->L, nu = e->nu;
     for (int l = 0; l + 1 < L; ++l) {
       const Engine::StageRef& S = e->st_slot[l];
@@ -650,7 +652,7 @@ struct Sched {
         if (g_fuse_first) {
           stage<ST_PAIR_FROM_ZERO>(e, S, l == 0);
         } else {
-          stage<ST_FIRST>(e, S, l == 0);
+          if (!(first0 && l == 0)) stage<ST_FIRST>(e, S, l == 0);
           stage<ST_SMOOTH1>(e, W, l == 0);
         }
         for (int i = 1; i < nu; ++i) {
@@ -758,7 +760,7 @@ struct Sched {
       }
       sync();
     }
-    vcycle(e);
+    vcycle(e, true);
     fin<FIN_BETA>(e, e->fin_all);
     sync();
   }
