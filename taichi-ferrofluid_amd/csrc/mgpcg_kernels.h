@@ -524,8 +524,41 @@ __device__ __forceinline__ T smooth_slot_g(const Geo& g, const LevelPtrs<T>& P,
   return zn;
 }
 
-// SLOT_LANES slots per lane from t0 (stride BLOCK), all loaded before any is stored
+// smooth(l, 1) at a colour-1 slot followed by the slot's residual
+// r - (Adiag z + neighbour_sum) into res[1] (0 off FLUID), the operation
+// order of residual_slot: the colour-0 values do not change during the
+// half-sweep, so its neighbour sum is the residual's
 template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void smooth_store_res(const LevelPtrs<T>& P, const Coupling<T>& k,
+                                                 int t, const SmoothVals<T, DIM, COMPACT>& v) {
+  T res = (T)0;
+  if (v.gt == MG_FLUID) {
+    T al[DIM], ah[DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      if constexpr (COMPACT) {
+        al[a] = ((v.f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
+        ah[a] = ((v.f >> a) & 1u) ? k.coef : (T)0;
+      } else {
+        al[a] = v.al[a];
+        ah[a] = v.ah[a];
+      }
+    }
+    const T sm = nsum_vals<T, DIM>(al, ah, v.zl, v.zh);
+    const T num = v.rv - sm;
+    const T ad = slot_diag<T, COMPACT>(P, k, 1, t, v.f, v.ad);
+    const T z = num / ad;
+    stg(P.z[1] + t, z);
+    T Az = ad * z;
+    Az = Az + sm;
+    res = v.rv - Az;
+  }
+  stg(P.res[1] + t, res);
+}
+
+// SLOT_LANES slots per lane from t0 (stride BLOCK), all loaded before any is
+// stored; RES: colour 1 with its residual (smooth_store_res)
+template <typename T, int DIM, bool COMPACT, bool RES = false>
 __device__ __forceinline__ void smooth_slots(const Geo& g, const LevelPtrs<T>& P,
                                              const Coupling<T>& k, int c, int t0) {
   SmoothVals<T, DIM, COMPACT> v[SLOT_LANES];
@@ -539,7 +572,9 @@ __device__ __forceinline__ void smooth_slots(const Geo& g, const LevelPtrs<T>& P
 #pragma unroll
   for (int u = 0; u < SLOT_LANES; ++u) {
     const int t = t0 + u * BLOCK;
-    if (t < g.size) smooth_store<T, DIM, COMPACT>(P, k, c, t, v[u]);
+    if (t >= g.size) continue;
+    if constexpr (RES) smooth_store_res<T, DIM, COMPACT>(P, k, t, v[u]);
+    else smooth_store<T, DIM, COMPACT>(P, k, c, t, v[u]);
   }
 }
 
@@ -895,6 +930,7 @@ enum StageOp {
   ST_PROLONGATE = 5,
   ST_FIRST = 6,       // fill(0); smooth(l, 0), colour 0 only
   ST_PROLONGATE0 = 7,  // prolongate(l), colour 0 only
+  ST_SMOOTH1_RES = 8,  // smooth(l, 1), and the colour-1 residual into res[1]
 };
 
 // L0 only separates the level-0 instantiation (the dominant launches) from
@@ -921,11 +957,12 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
       if (S.swept != nullptr && it.vb == ic.block0 && threadIdx.x == 0)
         atomicAdd(S.swept, (unsigned long long)D.g.ncells);
     }
-    if constexpr (OP == ST_SMOOTH0 || OP == ST_SMOOTH1) {
+    if constexpr (OP == ST_SMOOTH0 || OP == ST_SMOOTH1 || OP == ST_SMOOTH1_RES) {
       constexpr int c = OP == ST_SMOOTH0 ? 0 : 1;
+      constexpr bool RES = OP == ST_SMOOTH1_RES;
       const int t0 = (it.vb - ic.block0) * (BLOCK * SLOT_LANES) + threadIdx.x;
-      if (kc.uniform) smooth_slots<T, DIM, true>(D.g, D.P, kc, c, t0);
-      else smooth_slots<T, DIM, false>(D.g, D.P, kc, c, t0);
+      if (kc.uniform) smooth_slots<T, DIM, true, RES>(D.g, D.P, kc, c, t0);
+      else smooth_slots<T, DIM, false, RES>(D.g, D.P, kc, c, t0);
       continue;
     }
     const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
@@ -1056,10 +1093,38 @@ __device__ __forceinline__ void residual_slot(const LevelDesc<T>& D, const Coupl
   }
 }
 
+// the colour-0 residual at SLOT_LANES slots per lane from t0 (stride BLOCK),
+// all loaded before any is stored
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void residual0_slots(const LevelDesc<T>& D, const Coupling<T>& k,
+                                                int t0) {
+  StencilVals<T, DIM, COMPACT> s[SLOT_LANES];
+  T rv[SLOT_LANES];
+#pragma unroll
+  for (int u = 0; u < SLOT_LANES; ++u) {
+    const int t = t0 + u * BLOCK;
+    const int tt = t < D.g.size ? t : D.g.size - 1;
+    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.P.z, 0, tt, s[u]);
+    rv[u] = ldg(D.P.r[0] + tt);
+  }
+#pragma unroll
+  for (int u = 0; u < SLOT_LANES; ++u) stencil_keep<T, DIM, COMPACT>(s[u]);
+#pragma unroll
+  for (int u = 0; u < SLOT_LANES; ++u) {
+    const int t = t0 + u * BLOCK;
+    if (t >= D.g.size) continue;
+    T v = (T)0;
+    if (s[u].gt == MG_FLUID) v = rv[u] - stencil_apply<T, DIM, COMPACT>(D.P, k, 0, t, s[u]);
+    stg(D.P.res[0] + t, v);
+  }
+}
+
 // restrict(l) (mgpcg.py:111-118) in two coalesced passes.  Pass 1: the
 // residual r - (Adiag z + neighbour_sum) of every FLUID cell, 0 elsewhere,
-// into the level's scratch arrays, both colours of a slot per thread.
-template <typename T, int DIM>
+// into the level's scratch arrays, both colours of a slot per thread; C0:
+// colour 0 only (the last smooth(l, 1) wrote colour 1's, ST_SMOOTH1_RES), on
+// a half-sweep stage of BLOCK * SLOT_LANES slots per virtual block.
+template <typename T, int DIM, bool C0 = false>
 __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restrict__ descs,
                                                     Stage S) {
   ItemCache ic;
@@ -1073,10 +1138,16 @@ __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restri
     }
     if (done) continue;
     const LevelDesc<T>& D = descs[ic.desc];
-    const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
-    if (t >= D.g.size) continue;
-    if (kc.uniform) residual_slot<T, DIM, true>(D, kc, t);
-    else residual_slot<T, DIM, false>(D, kc, t);
+    if constexpr (C0) {
+      const int t0 = (it.vb - ic.block0) * (BLOCK * SLOT_LANES) + threadIdx.x;
+      if (kc.uniform) residual0_slots<T, DIM, true>(D, kc, t0);
+      else residual0_slots<T, DIM, false>(D, kc, t0);
+    } else {
+      const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
+      if (t >= D.g.size) continue;
+      if (kc.uniform) residual_slot<T, DIM, true>(D, kc, t);
+      else residual_slot<T, DIM, false>(D, kc, t);
+    }
   }
 }
 

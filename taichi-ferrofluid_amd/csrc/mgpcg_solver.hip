@@ -617,12 +617,19 @@ struct Sched {
                          levels(e), stage_of(e, r));
     if (probe) probe_end(e);
   }
-  // fine: the level's slot stage (the residual pass), r: the restriction stage
-  static void restrict_(Engine* e, const Engine::StageRef& fine, const Engine::StageRef& r) {
+  // fine: the level's slot stage (the residual pass), r: the restriction
+  // stage; sweep (not NULL): the level's half-sweep stage, the last
+  // smooth(l, 1) having written the colour-1 residual (ST_SMOOTH1_RES)
+  static void restrict_(Engine* e, const Engine::StageRef& fine, const Engine::StageRef& r,
+                        const Engine::StageRef* sweep = nullptr) {
     if (r.nblocks == 0) return;
     if (e->restrict_mode == 2) {
-      hipLaunchKernelGGL((k_residual<T, DIM>), dim3(grid_of(fine)), dim3(BLOCK), 0, e->stream,
-                         levels(e), stage_of(e, fine));
+      if (sweep)
+        hipLaunchKernelGGL((k_residual<T, DIM, true>), dim3(grid_of(*sweep)), dim3(BLOCK), 0,
+                           e->stream, levels(e), stage_of(e, *sweep));
+      else
+        hipLaunchKernelGGL((k_residual<T, DIM>), dim3(grid_of(fine)), dim3(BLOCK), 0,
+                           e->stream, levels(e), stage_of(e, fine));
       hipLaunchKernelGGL((k_restrict_gather<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0,
                          e->stream, levels(e), stage_of(e, r));
     } else if (e->restrict_mode == 1) {
@@ -648,21 +655,26 @@ struct Sched {
     for (int l = 0; l + 1 < L; ++l) {
       const Engine::StageRef& S = e->st_slot[l];
       const Engine::StageRef& W = e->st_sweep[l];
+      // the last pre-smoothing smooth(l, 1) also forms the colour-1 residual
+      // when the restriction is the two-pass one and that half-sweep exists
+      const bool res1 = e->restrict_mode == 2 && nu > 0 && !(g_fuse_first && nu == 1);
       if (nu > 0) {
         if (g_fuse_first) {
           stage<ST_PAIR_FROM_ZERO>(e, S, l == 0);
         } else {
           if (!(first0 && l == 0)) stage<ST_FIRST>(e, S, l == 0);
-          stage<ST_SMOOTH1>(e, W, l == 0);
+          if (nu == 1 && res1) stage<ST_SMOOTH1_RES>(e, W, l == 0);
+          else stage<ST_SMOOTH1>(e, W, l == 0);
         }
         for (int i = 1; i < nu; ++i) {
           stage<ST_SMOOTH0>(e, W, l == 0);
-          stage<ST_SMOOTH1>(e, W, l == 0);
+          if (i == nu - 1 && res1) stage<ST_SMOOTH1_RES>(e, W, l == 0);
+          else stage<ST_SMOOTH1>(e, W, l == 0);
         }
       } else {
         stage<ST_ZERO>(e, S, l == 0);
       }
-      restrict_(e, e->st_slot[l], e->st_restrict[l]);
+      restrict_(e, e->st_slot[l], e->st_restrict[l], res1 ? &W : nullptr);
     }
     // bottom level on the batched kernels (only when it is too big for k_coarse)
     const Engine::StageRef& B = e->st_bottom[0];

@@ -118,6 +118,26 @@ def set_knobs(**kw):
         _lib.check(lib.mgpcg_set_option(None, opt[k], v))
 
 
+@pytest.mark.parametrize("nu", [1, 3])
+@pytest.mark.parametrize("fuse_first", [0, 1])
+def test_smoothing_counts_match_oracle(dev, nu, fuse_first):
+    """Smoothing counts other than 2 per side (which half-sweep forms the
+    colour-1 residual, and whether level 0's first half-sweep comes from the
+    kernel that wrote r, depend on them and on fusing fill(0) + smooth(0) +
+    smooth(1)): PLAIN results are the oracle's bits."""
+    sysd = dict(SYSTEMS["c2_3d_small"](_emu(dev)), nu=nu)
+    print(f"case {sysd['name']} nu={nu} fuse_first={fuse_first} on {dev}", flush=True)
+    set_knobs(fuse_first=fuse_first)
+    try:
+        x, out, _ = gpu_solve(sysd, dev, "plain")
+    finally:
+        set_knobs(**KNOB_DEFAULTS)
+    xr, outr, _ = osys.solve(sysd, core.PLAIN, max_iters=500, rel_tol=1e-12)
+    assert out["status"] == 1 == outr["status"], (out, outr)
+    assert out["iters"] == outr["iters"]
+    assert np.array_equal(x, xr)
+
+
 @pytest.mark.parametrize("knobs", [
     {"coarse_max": 0}, {"coarse_max": 1 << 30}, {"max_grid": 8}, {"max_grid": 64},
     {"fuse_first": 1}, {"fuse_prolong": 0}, {"fuse_p": 1},
