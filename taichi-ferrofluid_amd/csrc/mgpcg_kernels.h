@@ -64,6 +64,12 @@ struct LevelPtrs {
   const uint16_t* nb[2];   // flag word per slot
   const LevelMeta* meta;   // NULL: always read Ax
   T* res[2];               // restriction scratch: the level's residual (not the coarsest)
+  // per block of BLOCK slots (slot / BLOCK): 1 when a slot of either colour is
+  // FLUID.  A kernel whose work is on FLUID cells only skips the others'
+  // blocks ("dry" blocks); what such a block would hold (0 residuals, 0 z on
+  // colour 0 before the first half-sweep) is written by the packing kernel or
+  // stored without loads.
+  const uint8_t* wet;
 };
 
 // The level's single coupling value when its couplings are uniform.
@@ -372,6 +378,18 @@ __device__ __forceinline__ T nsum_vals(const T* al, const T* ah, const T* vl, co
     ret = ret + s;
   }
   return ret;
+}
+
+// whether blocks [b0, b0 + n) of BLOCK slots of a level hold a FLUID slot
+// (blocks past the level's end do not count; no flags: every block is wet)
+template <typename T>
+__device__ __forceinline__ bool wet_blocks(const LevelPtrs<T>& P, const Geo& g, int b0, int n) {
+  if (P.wet == nullptr) return true;
+  const int nblocks = (g.size + BLOCK - 1) / BLOCK;
+  bool w = false;
+  for (int i = 0; i < n; ++i)
+    if (b0 + i < nblocks) w = w || ldg(P.wet + b0 + i) != 0;
+  return w;
 }
 
 // the item of virtual block vb, kept while the workgroup's next virtual
@@ -960,7 +978,9 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     if constexpr (OP == ST_SMOOTH0 || OP == ST_SMOOTH1 || OP == ST_SMOOTH1_RES) {
       constexpr int c = OP == ST_SMOOTH0 ? 0 : 1;
       constexpr bool RES = OP == ST_SMOOTH1_RES;
-      const int t0 = (it.vb - ic.block0) * (BLOCK * SLOT_LANES) + threadIdx.x;
+      const int lb = it.vb - ic.block0;
+      if (!wet_blocks(D.P, D.g, lb * SLOT_LANES, SLOT_LANES)) continue;  // no FLUID cell
+      const int t0 = lb * (BLOCK * SLOT_LANES) + threadIdx.x;
       if (kc.uniform) smooth_slots<T, DIM, true, RES>(D.g, D.P, kc, c, t0);
       else smooth_slots<T, DIM, false, RES>(D.g, D.P, kc, c, t0);
       continue;
@@ -981,6 +1001,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     } else if (OP == ST_PAIR_FROM_ZERO) {
       smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
     } else if (OP == ST_PROLONG_SMOOTH1) {
+      if (!wet_blocks(D.P, D.g, it.vb - ic.block0, 1)) continue;  // no FLUID cell
       if (kc.uniform)
         prolong_smooth1_slot_g<T, DIM, true>(D.g, D.gc, D.P, kc, D.C.z[0], D.C.z[1], t);
       else
@@ -1138,11 +1159,14 @@ __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restri
     }
     if (done) continue;
     const LevelDesc<T>& D = descs[ic.desc];
+    // a block without FLUID cells keeps the 0 residuals the packing wrote
     if constexpr (C0) {
+      if (!wet_blocks(D.P, D.g, (it.vb - ic.block0) * SLOT_LANES, SLOT_LANES)) continue;
       const int t0 = (it.vb - ic.block0) * (BLOCK * SLOT_LANES) + threadIdx.x;
       if (kc.uniform) residual0_slots<T, DIM, true>(D, kc, t0);
       else residual0_slots<T, DIM, false>(D, kc, t0);
     } else {
+      if (!wet_blocks(D.P, D.g, it.vb - ic.block0, 1)) continue;
       const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
       if (t >= D.g.size) continue;
       if (kc.uniform) residual_slot<T, DIM, true>(D, kc, t);
@@ -1163,6 +1187,9 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_gather(const LevelDesc<T>* _
     if (item_at(S, it.vb, ic)) done = ldg(&descs[ic.desc].st->done) != 0;
     if (done) continue;
     const LevelDesc<T>& D = descs[ic.desc];
+    // a coarse cell that is not FLUID never has its r read: a block of such
+    // coarse cells is skipped
+    if (!wet_blocks(D.C, D.gc, it.vb - ic.block0, 1)) continue;
     const int tc = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (tc >= D.gc.size) continue;
 #pragma unroll
@@ -1572,6 +1599,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
     }
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
+    if (!wet_blocks(D.P, D.g, it.vb - ic.block0, 1)) continue;  // p changes on FLUID only
     const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
     int gts[2];
@@ -1639,6 +1667,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
+    if (!wet_blocks(D.P, D.g, lb, 1)) continue;  // Ap and p.Ap on FLUID only
     const int t = lb * BLOCK + threadIdx.x;
     if (t < D.g.size) {
       if (kc.uniform) ap_slot<T, TX, DIM, true>(D, kc, par, t, rs.acc);
@@ -1678,6 +1707,11 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
+    if (!wet_blocks(D.P, D.g, lb, 1)) {
+      // x and r change on FLUID only; the first half-sweep's z0 is 0 here
+      if (t < D.g.size && !refine) stg(D.P.z[0] + t, (T)0);
+      continue;
+    }
     DD& acc = rs.acc;
     if (t < D.g.size) {
 #pragma unroll
@@ -1742,6 +1776,11 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
+    if (!wet_blocks(D.P, D.g, lb, 1)) {
+      // r changes on FLUID only; the first half-sweep's z0 is 0 here
+      if (t < D.g.size) stg(D.P.z[0] + t, (T)0);
+      continue;
+    }
     DD& acc = rs.acc;
     if (t < D.g.size) {
 #pragma unroll
@@ -1819,6 +1858,7 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
+    if (!wet_blocks(D.P, D.g, lb, 1)) continue;  // z0 and z.r on FLUID only
     const int t = lb * BLOCK + threadIdx.x;
     DD& acc = rs.acc;
     if (t < D.g.size) {
@@ -2125,7 +2165,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
                                                       const float* __restrict__ Ax,
                                                       LevelPtrs<T> P, LevelMeta* meta) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
-  uint32_t mn = 0xffffffffu, mx = 0u;
+  uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u;
   if (t < g.size) {
     // every canonical value of both colours is loaded first (ghost slots and
     // missing lower neighbours read cell 0 and discard it)
@@ -2199,21 +2239,27 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
       ((uint16_t*)P.nb[c])[t] = (uint16_t)(f | (code << NB_AD));
       P.r[c][t] = (T)0;
       P.z[c][t] = (T)0;
+      if (P.res[c] != nullptr) P.res[c][t] = (T)0;
+      if (gvc == MG_FLUID) wet = 1u;
     }
   }
-  __shared__ uint32_t smn[BLOCK / 64], smx[BLOCK / 64];
+  __shared__ uint32_t smn[BLOCK / 64], smx[BLOCK / 64], swet[BLOCK / 64];
   mn = wave_min_u32(mn);
   mx = wave_max_u32(mx);
+  wet = wave_max_u32(wet);
   if ((threadIdx.x & 63) == 0) {
     smn[threadIdx.x >> 6] = mn;
     smx[threadIdx.x >> 6] = mx;
+    swet[threadIdx.x >> 6] = wet;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < BLOCK / 64; ++w) {
       mn = smn[w] < mn ? smn[w] : mn;
       mx = smx[w] > mx ? smx[w] : mx;
+      wet = swet[w] > wet ? swet[w] : wet;
     }
+    if (P.wet != nullptr) ((uint8_t*)P.wet)[blockIdx.x] = (uint8_t)wet;
     // a value that would not move the level's current extreme skips the
     // atomic: with one coupling value on the level (the usual case) only the
     // first workgroups update it, instead of every workgroup queueing on the

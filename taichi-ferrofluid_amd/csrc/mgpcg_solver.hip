@@ -48,6 +48,7 @@ struct LevelBuf {
   void* r[2];
   void* z[2];
   uint16_t* nb[2];   // flag words (coupling flags, diagonal code)
+  uint8_t* wet;      // per block of BLOCK slots: 1 when one of them is FLUID
   LevelMeta* meta;   // nonzero-coupling summary, written when the level is loaded
   void* res[2];      // restriction scratch (NULL on the coarsest level)
 };
@@ -115,6 +116,7 @@ LevelPtrs<T> level_ptrs(const mgpcg_solver* s, int l) {
     P.nb[c] = B.nb[c];
     P.res[c] = (T*)B.res[c];
   }
+  P.wet = B.wet;
   P.meta = (g_compact && !s->full_coupling) ? B.meta : nullptr;
   return P;
 }
@@ -1022,7 +1024,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     return o;
   };
   struct Offs {
-    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta;
+    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta, wet;
   };
   std::vector<Offs> lo;
   s->coarse_from = n_levels;
@@ -1043,6 +1045,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       o.res[c] = (l + 1 < n_levels) ? take(g.size * s->esz) : (size_t)-1;
     }
     o.meta = take(sizeof(LevelMeta));
+    o.wet = take(nblk(g.size));
     lo.push_back(o);
   }
   const Geo& g0 = s->geo[0];
@@ -1079,6 +1082,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       B.res[c] = (l + 1 < n_levels) ? (void*)(s->pool + lo[l].res[c]) : nullptr;
     }
     B.meta = (LevelMeta*)(s->pool + lo[l].meta);
+    B.wet = (uint8_t*)(s->pool + lo[l].wet);
     s->lv.push_back(B);
     const Geo& g = s->geo[l];
     // ghost cells are SOLID (the zero fill made them FLUID)

@@ -138,7 +138,10 @@ def test_prolong_smooth1_bit_exact(dev, case):
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 def test_restrict_bit_exact(dev, case, coupling, rows):
     """Every restriction kernel (residual + gather, row-tiled, child-parallel), with stored and
-    with uniform (flag-byte) couplings, against the oracle bit for bit."""
+    with uniform (flag-byte) couplings, against the oracle bit for bit.  The
+    gather skips blocks of coarse cells none of which is FLUID (their r is
+    never read), so it is compared on the FLUID coarse cells; the other two
+    write every coarse cell and are compared everywhere."""
     import _lib
 
     res, L = case
@@ -148,6 +151,7 @@ def test_restrict_bit_exact(dev, case, coupling, rows):
     _lib.check(lib.mgpcg_set_option(None, _lib.OPT_RESTRICT_ROWS, rows))
     try:
         rng = np.random.default_rng(7)
+        compared = 0
         for l in range(L - 1):
             if coupling is not None:
                 assert s.level_coupling(l) == (True, np.float32(coupling)) or \
@@ -160,7 +164,14 @@ def test_restrict_bit_exact(dev, case, coupling, rows):
             s.debug_op(_lib.OP_RESTRICT, l, 0)
             got = _get(s, _lib.FIELD_R, l + 1)
             ref = core.restrict(h["gt"][l], h["Adiag"][l], h["Ax"][l], r, z)
-            assert np.array_equal(got, ref), f"level {l}"
+            if rows == 2:
+                fl = h["gt"][l + 1] == 0
+                compared += int(fl.sum())
+                assert np.array_equal(got[fl], ref[fl]), f"level {l}"
+            else:
+                compared += got.size
+                assert np.array_equal(got, ref), f"level {l}"
+        assert compared > 0
     finally:
         _lib.check(lib.mgpcg_set_option(None, _lib.OPT_RESTRICT_ROWS, 2))
 
