@@ -35,8 +35,12 @@ namespace mg {
 constexpr int BLOCK = 256;
 constexpr int COARSE_THREADS = 1024;
 // slots per lane of the half-sweep stages: a virtual block of a half-sweep
-// covers BLOCK * SLOT_LANES slots
-constexpr int SLOT_LANES = 2;
+// covers BLOCK * SLOT_LANES slots (a build-time choice)
+#ifndef MG_SLOT_LANES
+#define MG_SLOT_LANES 2
+#endif
+constexpr int SLOT_LANES = MG_SLOT_LANES;
+static_assert(SLOT_LANES >= 1 && SLOT_LANES <= 8, "slots per lane");
 
 // Nonzero-coupling summary of a level, accumulated on the device while the
 // level is packed: the smallest and largest FP32 bit pattern of its nonzero
