@@ -396,6 +396,32 @@ __device__ __forceinline__ bool wet_blocks(const LevelPtrs<T>& P, const Geo& g, 
   return w;
 }
 
+// The FLUID flags of a workgroup's next 64 virtual blocks, fetched at once
+// (lane l loads the flags of the block l iterations ahead, a wave ballot
+// gathers them) instead of one dependent load per block.  `iter` counts the
+// workgroup's loop iterations; the mask is refetched when the item changes
+// (invalidate) or after 64 iterations.  Every call is block-uniform.
+struct WetScan {
+  unsigned long long mask = 0ull;
+  int base = 0;
+  bool valid = false;
+};
+template <typename T>
+__device__ __forceinline__ bool wet_here(WetScan& w, int iter, int vb, int step, int end,
+                                         int item_begin, int item_end, const LevelPtrs<T>& P,
+                                         const Geo& g, int span) {
+  if (!w.valid || iter - w.base >= 64) {
+    const int l = threadIdx.x & 63;
+    const int v = vb + l * step;
+    bool f = true;  // beyond the item: decided when reached
+    if (v < end && v < item_end) f = wet_blocks(P, g, (v - item_begin) * span, span);
+    w.mask = __ballot(f);
+    w.base = iter;
+    w.valid = true;
+  }
+  return ((w.mask >> (iter - w.base)) & 1ull) != 0ull;
+}
+
 // the item of virtual block vb, kept while the workgroup's next virtual
 // blocks fall into it; returns true when it changed
 struct ItemCache {
@@ -967,8 +993,11 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     if (S.t0 != nullptr && threadIdx.x == 0)
       atomicMin(S.t0 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
   }
-  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+  WetScan ws;
+  int iter = 0;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
+      ws.valid = false;
       const LevelDesc<T>& D = descs[ic.desc];
       done = ldg(&D.st->done) != 0;
       kc = coupling_g(D.P);
@@ -983,11 +1012,14 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
       constexpr int c = OP == ST_SMOOTH0 ? 0 : 1;
       constexpr bool RES = OP == ST_SMOOTH1_RES;
       const int lb = it.vb - ic.block0;
-      if (!wet_blocks(D.P, D.g, lb * SLOT_LANES, SLOT_LANES)) continue;  // no FLUID cell
+      if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, SLOT_LANES)) continue;  // no FLUID cell
       const int t0 = lb * (BLOCK * SLOT_LANES) + threadIdx.x;
       if (kc.uniform) smooth_slots<T, DIM, true, RES>(D.g, D.P, kc, c, t0);
       else smooth_slots<T, DIM, false, RES>(D.g, D.P, kc, c, t0);
       continue;
+    }
+    if constexpr (OP == ST_PROLONG_SMOOTH1) {  // before any per-lane exit: the scan is a ballot
+      if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;
     }
     const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
@@ -1005,7 +1037,6 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     } else if (OP == ST_PAIR_FROM_ZERO) {
       smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
     } else if (OP == ST_PROLONG_SMOOTH1) {
-      if (!wet_blocks(D.P, D.g, it.vb - ic.block0, 1)) continue;  // no FLUID cell
       if (kc.uniform)
         prolong_smooth1_slot_g<T, DIM, true>(D.g, D.gc, D.P, kc, D.C.z[0], D.C.z[1], t);
       else
@@ -1155,8 +1186,11 @@ __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restri
   ItemCache ic;
   bool done = true;
   Coupling<T> kc{false, (T)0};
-  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+  WetScan ws;
+  int iter = 0;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
+      ws.valid = false;
       const LevelDesc<T>& D = descs[ic.desc];
       done = ldg(&D.st->done) != 0;
       kc = coupling_g(D.P);
@@ -1165,12 +1199,12 @@ __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restri
     const LevelDesc<T>& D = descs[ic.desc];
     // a block without FLUID cells keeps the 0 residuals the packing wrote
     if constexpr (C0) {
-      if (!wet_blocks(D.P, D.g, (it.vb - ic.block0) * SLOT_LANES, SLOT_LANES)) continue;
+      if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, SLOT_LANES)) continue;
       const int t0 = (it.vb - ic.block0) * (BLOCK * SLOT_LANES) + threadIdx.x;
       if (kc.uniform) residual0_slots<T, DIM, true>(D, kc, t0);
       else residual0_slots<T, DIM, false>(D, kc, t0);
     } else {
-      if (!wet_blocks(D.P, D.g, it.vb - ic.block0, 1)) continue;
+      if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;
       const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
       if (t >= D.g.size) continue;
       if (kc.uniform) residual_slot<T, DIM, true>(D, kc, t);
@@ -1187,13 +1221,18 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_gather(const LevelDesc<T>* _
                                                            Stage S) {
   ItemCache ic;
   bool done = true;
-  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
-    if (item_at(S, it.vb, ic)) done = ldg(&descs[ic.desc].st->done) != 0;
+  WetScan ws;
+  int iter = 0;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
+    if (item_at(S, it.vb, ic)) {
+      ws.valid = false;
+      done = ldg(&descs[ic.desc].st->done) != 0;
+    }
     if (done) continue;
     const LevelDesc<T>& D = descs[ic.desc];
     // a coarse cell that is not FLUID never has its r read: a block of such
     // coarse cells is skipped
-    if (!wet_blocks(D.C, D.gc, it.vb - ic.block0, 1)) continue;
+    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.C, D.gc, 1)) continue;
     const int tc = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (tc >= D.gc.size) continue;
 #pragma unroll
@@ -1592,8 +1631,11 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
   int par = 0;
   float b32 = 0.0f;
   double beta = 0.0;
-  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+  WetScan ws;
+  int iter = 0;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
+      ws.valid = false;
       const CGState* st = sys[ic.desc].st;
       done = ldg(&st->done) != 0;
       par = ldg(&st->iter) & 1;
@@ -1603,7 +1645,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
     }
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
-    if (!wet_blocks(D.P, D.g, it.vb - ic.block0, 1)) continue;  // p changes on FLUID only
+    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;  // p changes on FLUID only
     const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
     int gts[2];
@@ -1659,8 +1701,11 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
   bool done = true;
   int par = 0;
   Coupling<T> kc{false, (T)0};
-  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+  WetScan ws;
+  int iter = 0;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
+      ws.valid = false;
       run_flush(rs);
       const SysDesc<T, TX>& D = sys[ic.desc];
       done = ldg(&D.st->done) != 0;
@@ -1671,7 +1716,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
-    if (!wet_blocks(D.P, D.g, lb, 1)) continue;  // Ap and p.Ap on FLUID only
+    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;  // Ap and p.Ap on FLUID only
     const int t = lb * BLOCK + threadIdx.x;
     if (t < D.g.size) {
       if (kc.uniform) ap_slot<T, TX, DIM, true>(D, kc, par, t, rs.acc);
@@ -1693,8 +1738,11 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
   int par = 0, prec = PLAIN;
   float a32 = 0.0f;
   double alpha = 0.0;
-  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+  WetScan ws;
+  int iter = 0;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
+      ws.valid = false;
       run_flush(rs);
       const CGState* st = sys[ic.desc].st;
       kc = coupling_g(sys[ic.desc].P);
@@ -1711,7 +1759,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
-    if (!wet_blocks(D.P, D.g, lb, 1)) {
+    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) {
       // x and r change on FLUID only; the first half-sweep's z0 is 0 here
       if (t < D.g.size && !refine) stg(D.P.z[0] + t, (T)0);
       continue;
@@ -1768,8 +1816,11 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
   RunSum rs;
   bool skip = true;
   Coupling<T> kc{false, (T)0};
-  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+  WetScan ws;
+  int iter = 0;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
+      ws.valid = false;
       run_flush(rs);
       const SysDesc<T, TX>& D = sys[ic.desc];
       skip = ldg(&D.st->done) != 0 || ldg(&D.st->pending_true) == 0;
@@ -1780,7 +1831,7 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
     const int t = lb * BLOCK + threadIdx.x;
-    if (!wet_blocks(D.P, D.g, lb, 1)) {
+    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) {
       // r changes on FLUID only; the first half-sweep's z0 is 0 here
       if (t < D.g.size) stg(D.P.z[0] + t, (T)0);
       continue;
@@ -1851,8 +1902,11 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
   RunSum rs;
   bool done = true;
   Coupling<T> kc{false, (T)0};
-  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step) {
+  WetScan ws;
+  int iter = 0;
+  for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
+      ws.valid = false;
       run_flush(rs);
       const SysDesc<T, TX>& D = sys[ic.desc];
       done = ldg(&D.st->done) != 0;
@@ -1862,7 +1916,7 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
     const SysDesc<T, TX>& D = sys[ic.desc];
     const int lb = it.vb - ic.block0;
     run_mark(rs, D.partials, lb);
-    if (!wet_blocks(D.P, D.g, lb, 1)) continue;  // z0 and z.r on FLUID only
+    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;  // z0 and z.r on FLUID only
     const int t = lb * BLOCK + threadIdx.x;
     DD& acc = rs.acc;
     if (t < D.g.size) {

@@ -139,6 +139,19 @@ double shfl_idx(double v, int src, int width) { return shfl_src(g_xd, v, src, wi
 double shfl_xor_d(double v, int mask) { return shfl(g_xd, v, mask); }
 float shfl_xor_f(float v, int mask) { return shfl(g_xf, v, mask); }
 int shfl_xor_i(int v, int mask) { return shfl(g_xi, v, mask); }
+// bit l of the result: lane l of the caller's 64-lane wave had pred != 0
+unsigned long long ballot(int pred) {
+  const unsigned tid = g_cur->tid;
+  if (g_xi.size() < blockDim.x) g_xi.resize(blockDim.x);
+  g_xi[tid] = pred != 0;
+  syncthreads();
+  const unsigned base = tid & ~63u;
+  unsigned long long m = 0;
+  for (unsigned l = 0; l < 64 && base + l < blockDim.x; ++l)
+    if (g_xi[base + l]) m |= 1ull << l;
+  syncthreads();
+  return m;
+}
 
 void run_grid(dim3 grid, dim3 block, void (*tramp)(void*), void* ctx) {
   gridDim = grid;

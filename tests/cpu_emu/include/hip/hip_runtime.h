@@ -64,6 +64,7 @@ int shfl_xor_i(int v, int mask);
 float shfl_idx(float v, int src, int width);
 double shfl_idx(double v, int src, int width);
 float shfl_xor_f(float v, int mask);
+unsigned long long ballot(int pred);
 // run `body(thread)` for every thread of every block of the grid
 void run_grid(dim3 grid, dim3 block, void (*tramp)(void*), void* ctx);
 void* guarded_alloc(size_t bytes);
@@ -84,6 +85,7 @@ inline float __shfl_xor(float v, int mask, int width = 64) {
   (void)width;
   return hipemu::shfl_xor_f(v, mask);
 }
+inline unsigned long long __ballot(int pred) { return hipemu::ballot(pred); }
 inline int __shfl_xor(int v, int mask, int width = 64) {
   (void)width;
   return hipemu::shfl_xor_i(v, mask);
