@@ -81,9 +81,6 @@ template <typename T>
 struct Coupling {
   bool uniform;
   T coef;
-  // the diagonals of codes 0..7 (diag_of_code) in the workgroup's LDS, when
-  // the kernel set them up (diag_table); NULL: decoded arithmetically
-  const T* tab = nullptr;
 };
 // The diagonal encoded by code n < NB_AD_STORED on a level whose couplings
 // are all `coef`: -coef added n times to 0, in order, as the operator
@@ -488,22 +485,10 @@ __device__ __forceinline__ T slot_diag(const LevelPtrs<T>& P, const Coupling<T>&
                                        uint32_t f, T stored) {
   if constexpr (COMPACT) {
     const uint32_t n = (f >> NB_AD) & 7u;
-    if (n == NB_AD_STORED) return ldg(P.Adiag[c] + t);
-    return k.tab != nullptr ? k.tab[n] : diag_of_code<T>(n, k.coef);
+    return n == NB_AD_STORED ? ldg(P.Adiag[c] + t) : diag_of_code<T>(n, k.coef);
   } else {
     return stored;
   }
-}
-
-// Fills the workgroup's LDS table of decoded diagonals for coupling k (the
-// first barrier: no wave still reads the previous item's table) and points
-// k at it.  Block-uniform.
-template <typename T>
-__device__ __forceinline__ void diag_table(T* tab, Coupling<T>& k) {
-  __syncthreads();
-  if (threadIdx.x < 8) tab[threadIdx.x] = diag_of_code<T>(threadIdx.x, k.coef);
-  __syncthreads();
-  k.tab = tab;
 }
 
 template <typename T, int DIM, bool COMPACT>
@@ -1001,7 +986,6 @@ enum StageOp {
 template <typename T, int DIM, int OP, int L0>
 __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict__ descs,
                                                  Stage S) {
-  __shared__ T dtab[8];
   ItemCache ic;
   bool done = true;
   Coupling<T> kc{false, (T)0};
@@ -1017,7 +1001,6 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
       const LevelDesc<T>& D = descs[ic.desc];
       done = ldg(&D.st->done) != 0;
       kc = coupling_g(D.P);
-      diag_table(dtab, kc);
     }
     if (done) continue;
     const LevelDesc<T>& D = descs[ic.desc];
@@ -1200,7 +1183,6 @@ __device__ __forceinline__ void residual0_slots(const LevelDesc<T>& D, const Cou
 template <typename T, int DIM, bool C0 = false>
 __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restrict__ descs,
                                                     Stage S) {
-  __shared__ T dtab[8];
   ItemCache ic;
   bool done = true;
   Coupling<T> kc{false, (T)0};
@@ -1212,7 +1194,6 @@ __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restri
       const LevelDesc<T>& D = descs[ic.desc];
       done = ldg(&D.st->done) != 0;
       kc = coupling_g(D.P);
-      diag_table(dtab, kc);
     }
     if (done) continue;
     const LevelDesc<T>& D = descs[ic.desc];
@@ -1715,7 +1696,6 @@ __device__ __forceinline__ void ap_slot(const SysDesc<T, TX>& D, const Coupling<
 template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __restrict__ sys,
                                                       Stage S) {
-  __shared__ T dtab[8];
   ItemCache ic;
   RunSum rs;
   bool done = true;
@@ -1731,7 +1711,6 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
       done = ldg(&D.st->done) != 0;
       par = (ldg(&D.st->iter) & 1) ^ 1;
       kc = coupling_g(D.P);
-      diag_table(dtab, kc);
     }
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
@@ -1752,7 +1731,6 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
 template <typename T, typename TX>
 __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restrict__ sys,
                                                  Stage S) {
-  __shared__ T dtab[8];
   ItemCache ic;
   RunSum rs;
   Coupling<T> kc{false, (T)0};
@@ -1768,7 +1746,6 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
       run_flush(rs);
       const CGState* st = sys[ic.desc].st;
       kc = coupling_g(sys[ic.desc].P);
-      diag_table(dtab, kc);
       done = ldg(&st->done) != 0;
       par = (ldg(&st->iter) & 1) ^ 1;  // the p written by k_cg_update_p / k_cg_ap
       prec = ldg(&st->precision);
@@ -1921,7 +1898,6 @@ __device__ __forceinline__ void smooth0_dot_slot(const SysDesc<T, TX>& D, const 
 template <typename T, typename TX, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __restrict__ sys,
                                                        Stage S) {
-  __shared__ T dtab[8];
   ItemCache ic;
   RunSum rs;
   bool done = true;
@@ -1935,7 +1911,6 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
       const SysDesc<T, TX>& D = sys[ic.desc];
       done = ldg(&D.st->done) != 0;
       kc = coupling_g(D.P);
-      diag_table(dtab, kc);
     }
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
