@@ -39,6 +39,19 @@ template <typename T>
 __device__ __forceinline__ void stg(T* p, T v) {
   *(MG_GLOBAL T*)p = v;
 }
+// Element i (0 <= i, i * sizeof(T) < 2^32) of an array whose base is uniform:
+// a 32-bit byte offset from the base lets the load use the scalar-base,
+// vector-offset addressing form instead of 64-bit address arithmetic.
+template <typename T>
+__device__ __forceinline__ T ldgi(const T* base, int i) {
+  const uint32_t off = (uint32_t)i * (uint32_t)sizeof(T);
+  return *(const MG_GLOBAL T*)((const MG_GLOBAL char*)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void stgi(T* base, int i, T v) {
+  const uint32_t off = (uint32_t)i * (uint32_t)sizeof(T);
+  *(MG_GLOBAL T*)((MG_GLOBAL char*)base + off) = v;
+}
 // Makes a loaded value live at this point of the program, so the compiler
 // cannot sink its load into a later branch (all of a slot's loads are then in
 // flight together).  Emits no instruction.

@@ -350,7 +350,7 @@ template <typename T, int DIM>
 __device__ __forceinline__ void couplings_g(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
                                             int t, const Nbr<DIM>& n, T* al, T* ah) {
   if (k.uniform) {
-    const uint32_t f = ldg(P.nb[c] + t);
+    const uint32_t f = ldgi(P.nb[c], t);
     MG_KEEP(f);
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
@@ -360,8 +360,8 @@ __device__ __forceinline__ void couplings_g(const LevelPtrs<T>& P, const Couplin
   } else {
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
-      al[a] = ldg(P.Ax[a][c ^ 1] + n.lo[a]);
-      ah[a] = ldg(P.Ax[a][c] + t);
+      al[a] = ldgi(P.Ax[a][c ^ 1], n.lo[a]);
+      ah[a] = ldgi(P.Ax[a][c], t);
     }
     if constexpr (DIM == 3)
       MG_KEEP6(al[0], al[1], al[2], ah[0], ah[1], ah[2]);
@@ -485,7 +485,7 @@ __device__ __forceinline__ T slot_diag(const LevelPtrs<T>& P, const Coupling<T>&
                                        uint32_t f, T stored) {
   if constexpr (COMPACT) {
     const uint32_t n = (f >> NB_AD) & 7u;
-    return n == NB_AD_STORED ? ldg(P.Adiag[c] + t) : diag_of_code<T>(n, k.coef);
+    return n == NB_AD_STORED ? ldgi(P.Adiag[c], t) : diag_of_code<T>(n, k.coef);
   } else {
     return stored;
   }
@@ -496,22 +496,22 @@ __device__ __forceinline__ void smooth_load(const Geo& g, const LevelPtrs<T>& P,
                                             SmoothVals<T, DIM, COMPACT>& v) {
   const int q = row_q(g, t, c);
   const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
-  v.gt = ldg(P.gt[c] + t);
-  v.rv = ldg(P.r[c] + t);
-  if constexpr (!COMPACT) v.ad = ldg(P.Adiag[c] + t);
+  v.gt = ldgi(P.gt[c], t);
+  v.rv = ldgi(P.r[c], t);
+  if constexpr (!COMPACT) v.ad = ldgi(P.Adiag[c], t);
   const T* zo = P.z[c ^ 1];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    v.zl[a] = ldg(zo + n.lo[a]);
-    v.zh[a] = ldg(zo + n.hi[a]);
+    v.zl[a] = ldgi(zo, n.lo[a]);
+    v.zh[a] = ldgi(zo, n.hi[a]);
   }
   if constexpr (COMPACT) {
-    v.f = ldg(P.nb[c] + t);
+    v.f = ldgi(P.nb[c], t);
   } else {
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
-      v.al[a] = ldg(P.Ax[a][c ^ 1] + n.lo[a]);
-      v.ah[a] = ldg(P.Ax[a][c] + t);
+      v.al[a] = ldgi(P.Ax[a][c ^ 1], n.lo[a]);
+      v.ah[a] = ldgi(P.Ax[a][c], t);
     }
   }
 }
@@ -555,7 +555,7 @@ template <typename T, int DIM, bool COMPACT>
 __device__ __forceinline__ void smooth_store(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
                                              int t, const SmoothVals<T, DIM, COMPACT>& v) {
   if (v.gt != MG_FLUID) return;
-  stg(P.z[c] + t, smooth_value<T, DIM, COMPACT>(P, k, c, t, v));
+  stgi(P.z[c], t, smooth_value<T, DIM, COMPACT>(P, k, c, t, v));
 }
 
 // smooth(l, c) at slot t in the latency-oriented form; returns the slot's z
@@ -568,7 +568,7 @@ __device__ __forceinline__ T smooth_slot_g(const Geo& g, const LevelPtrs<T>& P,
   smooth_keep<T, DIM, COMPACT>(v);
   if (v.gt != MG_FLUID) return (T)0;  // unused: callers mask with the cell type
   const T zn = smooth_value<T, DIM, COMPACT>(P, k, c, t, v);
-  stg(P.z[c] + t, zn);
+  stgi(P.z[c], t, zn);
   return zn;
 }
 
@@ -596,12 +596,12 @@ __device__ __forceinline__ void smooth_store_res(const LevelPtrs<T>& P, const Co
     const T num = v.rv - sm;
     const T ad = slot_diag<T, COMPACT>(P, k, 1, t, v.f, v.ad);
     const T z = num / ad;
-    stg(P.z[1] + t, z);
+    stgi(P.z[1], t, z);
     T Az = ad * z;
     Az = Az + sm;
     res = v.rv - Az;
   }
-  stg(P.res[1] + t, res);
+  stgi(P.res[1], t, res);
 }
 
 // SLOT_LANES slots per lane from t0 (stride BLOCK), all loaded before any is
@@ -642,22 +642,22 @@ __device__ __forceinline__ void stencil_load(const Geo& g, const LevelPtrs<T>& P
                                              int c, int t, StencilVals<T, DIM, COMPACT>& s) {
   const int q = row_q(g, t, c);
   const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
-  s.gt = ldg(P.gt[c] + t);
-  s.own = ldg(v[c] + t);
-  if constexpr (!COMPACT) s.ad = ldg(P.Adiag[c] + t);
+  s.gt = ldgi(P.gt[c], t);
+  s.own = ldgi(v[c], t);
+  if constexpr (!COMPACT) s.ad = ldgi(P.Adiag[c], t);
   const T* vo = v[c ^ 1];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    s.vl[a] = ldg(vo + n.lo[a]);
-    s.vh[a] = ldg(vo + n.hi[a]);
+    s.vl[a] = ldgi(vo, n.lo[a]);
+    s.vh[a] = ldgi(vo, n.hi[a]);
   }
   if constexpr (COMPACT) {
-    s.f = ldg(P.nb[c] + t);
+    s.f = ldgi(P.nb[c], t);
   } else {
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
-      s.al[a] = ldg(P.Ax[a][c ^ 1] + n.lo[a]);
-      s.ah[a] = ldg(P.Ax[a][c] + t);
+      s.al[a] = ldgi(P.Ax[a][c ^ 1], n.lo[a]);
+      s.ah[a] = ldgi(P.Ax[a][c], t);
     }
   }
 }
@@ -706,10 +706,10 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
                                                        const T* zc1, int t) {
   const Slot c = decode(g, t, 1);
   const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, c.q);
-  const int8_t gt = ldg(P.gt[1] + t);
-  const T rv = ldg(P.r[1] + t);
+  const int8_t gt = ldgi(P.gt[1], t);
+  const T rv = ldgi(P.r[1], t);
   T ad = (T)0;
-  if constexpr (!COMPACT) ad = ldg(P.Adiag[1] + t);
+  if constexpr (!COMPACT) ad = ldgi(P.Adiag[1], t);
   const T* z0 = P.z[0];
   // The six colour-0 neighbours have four distinct parents: on every axis the
   // neighbour inside the cell's own 2-block shares the cell's parent p, the
@@ -724,7 +724,7 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
   const int sp = min(slot_of(gc, pi, pj, pk), gc.size - 1);
   const T* zcp = cp ? zc1 : zc0;
   const T* zcq = cp ? zc0 : zc1;
-  const T vp = ldg(zcp + sp);
+  const T vp = ldgi(zcp, sp);
   T zl[DIM], zh[DIM], vq[DIM], al[DIM], ah[DIM];
   bool up[DIM], inq[DIM];
 #pragma unroll
@@ -740,13 +740,13 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
     else if (va == 1) sq = sp + d * gc.W;
     else sq = sp - ((pk + 1) >> 1) + ((pk + d + 1) >> 1);
     sq = (inq[a] && sq >= 0 && sq < gc.size) ? sq : sp;
-    vq[a] = ldg(zcq + sq);
-    zl[a] = ldg(z0 + n.lo[a]);
-    zh[a] = ldg(z0 + n.hi[a]);
+    vq[a] = ldgi(zcq, sq);
+    zl[a] = ldgi(z0, n.lo[a]);
+    zh[a] = ldgi(z0, n.hi[a]);
   }
   uint32_t f = 0u;
   if constexpr (COMPACT) {
-    f = ldg(P.nb[1] + t);
+    f = ldgi(P.nb[1], t);
     MG_KEEP(f);
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
@@ -769,22 +769,22 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
   }
   T ret = nsum_vals<T, DIM>(al, ah, zl, zh);
   T num = rv - ret;
-  stg(P.z[1] + t, num / slot_diag<T, COMPACT>(P, k, 1, t, f, ad));
+  stgi(P.z[1], t, num / slot_diag<T, COMPACT>(P, k, 1, t, f, ad));
 }
 
 // z.fill(0); smooth(l, 0) on colour 0 at slot t in the latency-oriented form:
 // z0 = r0 / Adiag0 on FLUID, else 0
 template <typename T, bool COMPACT>
 __device__ __forceinline__ void first_slot_g(const LevelPtrs<T>& P, const Coupling<T>& k, int t) {
-  const int8_t gt = ldg(P.gt[0] + t);
-  const T rv = ldg(P.r[0] + t);
+  const int8_t gt = ldgi(P.gt[0], t);
+  const T rv = ldgi(P.r[0], t);
   uint32_t f = 0u;
   T ad = (T)0;
-  if constexpr (COMPACT) f = ldg(P.nb[0] + t);
-  else ad = ldg(P.Adiag[0] + t);
+  if constexpr (COMPACT) f = ldgi(P.nb[0], t);
+  else ad = ldgi(P.Adiag[0], t);
   T z = (T)0;
   if (gt == MG_FLUID) z = rv / slot_diag<T, COMPACT>(P, k, 0, t, f, ad);
-  stg(P.z[0] + t, z);
+  stgi(P.z[0], t, z);
 }
 
 // parent slot in the coarser level of fine-level slot `t` of colour `c`
@@ -1032,8 +1032,8 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
       const int pi = sl.i >> 1, pj = sl.j >> 1, pk = sl.k >> 1;
       const int ps = slot_of(D.gc, pi, pj, pk);
       const T zc = ldg((colour_of(pi, pj, pk) ? D.C.z[1] : D.C.z[0]) + ps);
-      const T zv = ldg(D.P.z[0] + t);
-      stg(D.P.z[0] + t, zv + zc);
+      const T zv = ldgi(D.P.z[0], t);
+      stgi(D.P.z[0], t, zv + zc);
     } else if (OP == ST_PAIR_FROM_ZERO) {
       smooth_pair_from_zero_slot<T, DIM>(D.g, D.P, t);
     } else if (OP == ST_PROLONG_SMOOTH1) {
@@ -1098,16 +1098,16 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_rows(const LevelDesc<T>* __r
       const int t = slot_of(gf, i, j, k);
       const int q = (k + 1) & 1;  // padded k' = k + 1 = 2m + q
       const Nbr<DIM> n = neighbours<DIM>(gf, t, q);  // an interior cell: in bounds
-      const int8_t gt = ldg(D.P.gt[c] + t);
-      const T zv = ldg(D.P.z[c] + t);
-      const T ad = ldg(D.P.Adiag[c] + t);
-      const T rv = ldg(D.P.r[c] + t);
+      const int8_t gt = ldgi(D.P.gt[c], t);
+      const T zv = ldgi(D.P.z[c], t);
+      const T ad = ldgi(D.P.Adiag[c], t);
+      const T rv = ldgi(D.P.r[c], t);
       const T* zo = D.P.z[c ^ 1];
       T zl[DIM], zh[DIM], al[DIM], ah[DIM];
 #pragma unroll
       for (int a = 0; a < DIM; ++a) {
-        zl[a] = ldg(zo + n.lo[a]);
-        zh[a] = ldg(zo + n.hi[a]);
+        zl[a] = ldgi(zo, n.lo[a]);
+        zh[a] = ldgi(zo, n.hi[a]);
       }
       couplings_g<T, DIM>(D.P, kc, c, t, n, al, ah);
       T v = (T)0;
@@ -1141,11 +1141,11 @@ __device__ __forceinline__ void residual_slot(const LevelDesc<T>& D, const Coupl
   for (int c = 0; c < 2; ++c) {  // one colour after the other
     StencilVals<T, DIM, COMPACT> s;
     stencil_load<T, DIM, COMPACT>(D.g, D.P, D.P.z, c, t, s);
-    const T rv = ldg(D.P.r[c] + t);
+    const T rv = ldgi(D.P.r[c], t);
     stencil_keep<T, DIM, COMPACT>(s);
     T v = (T)0;
     if (s.gt == MG_FLUID) v = rv - stencil_apply<T, DIM, COMPACT>(D.P, k, c, t, s);
-    stg(D.P.res[c] + t, v);
+    stgi(D.P.res[c], t, v);
   }
 }
 
@@ -1161,7 +1161,7 @@ __device__ __forceinline__ void residual0_slots(const LevelDesc<T>& D, const Cou
     const int t = t0 + u * BLOCK;
     const int tt = t < D.g.size ? t : D.g.size - 1;
     stencil_load<T, DIM, COMPACT>(D.g, D.P, D.P.z, 0, tt, s[u]);
-    rv[u] = ldg(D.P.r[0] + tt);
+    rv[u] = ldgi(D.P.r[0], tt);
   }
 #pragma unroll
   for (int u = 0; u < SLOT_LANES; ++u) stencil_keep<T, DIM, COMPACT>(s[u]);
@@ -1171,7 +1171,7 @@ __device__ __forceinline__ void residual0_slots(const LevelDesc<T>& D, const Cou
     if (t >= D.g.size) continue;
     T v = (T)0;
     if (s[u].gt == MG_FLUID) v = rv[u] - stencil_apply<T, DIM, COMPACT>(D.P, k, 0, t, s[u]);
-    stg(D.P.res[0] + t, v);
+    stgi(D.P.res[0], t, v);
   }
 }
 
@@ -1249,12 +1249,12 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_gather(const LevelDesc<T>* _
           for (int o2 = 0; o2 < 2; ++o2) {
             const int i = (DIM == 3) ? 2 * s.i + o0 : 0;
             const int j = 2 * s.j + o1, k = 2 * s.k + o2;
-            v[nch++] = ldg(D.P.res[colour_of(i, j, k)] + slot_of(D.g, i, j, k));
+            v[nch++] = ldgi(D.P.res[colour_of(i, j, k)], slot_of(D.g, i, j, k));
           }
       T sum = (T)0;
 #pragma unroll
       for (int o = 0; o < (DIM == 3 ? 8 : 4); ++o) sum = sum + v[o];
-      stg((cc ? D.C.r[1] : D.C.r[0]) + tc, sum);
+      stgi(cc ? D.C.r[1] : D.C.r[0], tc, sum);
     }
   }
 }
@@ -1277,16 +1277,16 @@ __device__ __forceinline__ T child_res(const Geo& gf, const LevelPtrs<T>& F, con
   const int t = slot_of(gf, i, j, kk);
   const int q = (kk + 1) & 1;  // padded k' = k + 1 = 2m + q
   const Nbr<DIM> n = neighbours<DIM>(gf, t, q);  // a real cell: in bounds
-  const int8_t gt = ldg(F.gt[c] + t);
-  const T zv = ldg(F.z[c] + t);
-  const T ad = ldg(F.Adiag[c] + t);
-  const T rv = ldg(F.r[c] + t);
+  const int8_t gt = ldgi(F.gt[c], t);
+  const T zv = ldgi(F.z[c], t);
+  const T ad = ldgi(F.Adiag[c], t);
+  const T rv = ldgi(F.r[c], t);
   const T* zo = F.z[c ^ 1];
   T zl[DIM], zh[DIM], al[DIM], ah[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    zl[a] = ldg(zo + n.lo[a]);
-    zh[a] = ldg(zo + n.hi[a]);
+    zl[a] = ldgi(zo, n.lo[a]);
+    zh[a] = ldgi(zo, n.hi[a]);
   }
   couplings_g<T, DIM>(F, k, c, t, n, al, ah);
   if (gt != MG_FLUID) return (T)0;
@@ -1328,7 +1328,7 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(const LevelDesc<T>* __restri
       T sum = (T)0;
 #pragma unroll
       for (int o = 0; o < 8; ++o) sum = sum + xch[threadIdx.x + o];
-      stg((cc ? D.C.r[1] : D.C.r[0]) + tc, sum);
+      stgi(cc ? D.C.r[1] : D.C.r[0], tc, sum);
     }
     __syncthreads();  // xch is reused by the next virtual block
   }
@@ -1470,10 +1470,10 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
       for (int t = threadIdx.x; t < n; t += COARSE_THREADS) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-          P.r[c][t] = ldg(G.r[c] + t);
-          const_cast<T*>(P.Adiag[c])[t] = ldg(G.Adiag[c] + t);
-          const_cast<int8_t*>(P.gt[c])[t] = ldg(G.gt[c] + t);
-          if (G.meta != nullptr) const_cast<uint16_t*>(P.nb[c])[t] = ldg(G.nb[c] + t);
+          P.r[c][t] = ldgi(G.r[c], t);
+          const_cast<T*>(P.Adiag[c])[t] = ldgi(G.Adiag[c], t);
+          const_cast<int8_t*>(P.gt[c])[t] = ldgi(G.gt[c], t);
+          if (G.meta != nullptr) const_cast<uint16_t*>(P.nb[c])[t] = ldgi(G.nb[c], t);
         }
       }
       if (G.meta != nullptr && threadIdx.x == 0) *const_cast<LevelMeta*>(P.meta) = *G.meta;
@@ -1507,8 +1507,8 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
       for (int t = threadIdx.x; t < n; t += COARSE_THREADS) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-          stg(G.z[c] + t, P.z[c][t]);
-          if (k > 0) stg(G.r[c] + t, P.r[c][t]);
+          stgi(G.z[c], t, P.z[c][t]);
+          if (k > 0) stgi(G.r[c], t, P.r[c][t]);
         }
       }
     }
@@ -1524,11 +1524,11 @@ __device__ __forceinline__ void first_z0(const LevelPtrs<T>& P, const Coupling<T
                                          bool fluid, T r0) {
   T z = (T)0;
   if (fluid) {
-    const T ad = k.uniform ? slot_diag<T, true>(P, k, 0, t, ldg(P.nb[0] + t), (T)0)
-                           : ldg(P.Adiag[0] + t);
+    const T ad = k.uniform ? slot_diag<T, true>(P, k, 0, t, ldgi(P.nb[0], t), (T)0)
+                           : ldgi(P.Adiag[0], t);
     z = r0 / ad;
   }
-  stg(P.z[0] + t, z);
+  stgi(P.z[0], t, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1652,9 +1652,9 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
     T polds[2], zvs[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {  // both colours loaded before either is stored
-      gts[c] = ldg(D.P.gt[c] + t);
-      polds[c] = ldg(D.p[par][c] + t);
-      zvs[c] = ldg(D.P.z[c] + t);
+      gts[c] = ldgi(D.P.gt[c], t);
+      polds[c] = ldgi(D.p[par][c], t);
+      zvs[c] = ldgi(D.P.z[c], t);
     }
     MG_KEEP(gts[0]);
     MG_KEEP(gts[1]);
@@ -1670,7 +1670,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
       } else {
         pv = (T)((double)zv + beta * (double)pold);
       }
-      stg(D.p[par ^ 1][c] + t, pv);
+      stgi(D.p[par ^ 1][c], t, pv);
     }
   }
 }
@@ -1687,7 +1687,7 @@ __device__ __forceinline__ void ap_slot(const SysDesc<T, TX>& D, const Coupling<
     stencil_keep<T, DIM, COMPACT>(s);
     if (s.gt != MG_FLUID) continue;
     const T ap = stencil_apply<T, DIM, COMPACT>(D.P, k, c, t, s);
-    stg(D.Ap[c] + t, ap);
+    stgi(D.Ap[c], t, ap);
     acc = dd_add(acc, dd_prod(s.own, ap));
   }
 }
@@ -1761,43 +1761,43 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     const int t = lb * BLOCK + threadIdx.x;
     if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) {
       // x and r change on FLUID only; the first half-sweep's z0 is 0 here
-      if (t < D.g.size && !refine) stg(D.P.z[0] + t, (T)0);
+      if (t < D.g.size && !refine) stgi(D.P.z[0], t, (T)0);
       continue;
     }
     DD& acc = rs.acc;
     if (t < D.g.size) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        const int8_t gt = ldg(D.P.gt[c] + t);
-        const T pv = ldg(D.p[par][c] + t);
-        const T apv = ldg(D.Ap[c] + t);
-        const TX xv = ldg(D.x[c] + t);
-        const T r0 = ldg(D.P.r[c] + t);
+        const int8_t gt = ldgi(D.P.gt[c], t);
+        const T pv = ldgi(D.p[par][c], t);
+        const T apv = ldgi(D.Ap[c], t);
+        const TX xv = ldgi(D.x[c], t);
+        const T r0 = ldgi(D.P.r[c], t);
         if (gt != MG_FLUID) {
           if (c == 0 && !refine) first_z0<T>(D.P, kc, t, false, (T)0);
           continue;
         }
         if (prec == PLAIN) {
           T tp = (T)a32 * pv;
-          stg(D.x[c] + t, (TX)((T)xv + tp));
+          stgi(D.x[c], t, (TX)((T)xv + tp));
           T ta = (T)a32 * apv;
           T rv = r0 - ta;
-          stg(D.P.r[c] + t, rv);
+          stgi(D.P.r[c], t, rv);
           acc = dd_add(acc, dd_prod(rv, rv));
           if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
         } else if (prec == F64) {
           T tp = (T)(alpha * (double)pv);
-          stg(D.x[c] + t, (TX)((double)xv + (double)tp));
+          stgi(D.x[c], t, (TX)((double)xv + (double)tp));
           T ta = (T)(alpha * (double)apv);
           T rv = r0 - ta;
-          stg(D.P.r[c] + t, rv);
+          stgi(D.P.r[c], t, rv);
           acc = dd_add(acc, dd_prod(rv, rv));
           if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
         } else {
-          stg(D.x[c] + t, (TX)((double)xv + alpha * (double)pv));
+          stgi(D.x[c], t, (TX)((double)xv + alpha * (double)pv));
           if (!refine) {
             T rv = (T)((double)r0 - alpha * (double)apv);
-            stg(D.P.r[c] + t, rv);
+            stgi(D.P.r[c], t, rv);
             acc = dd_add(acc, dd_prod(rv, rv));
             if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
           }
@@ -1833,7 +1833,7 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
     const int t = lb * BLOCK + threadIdx.x;
     if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) {
       // r changes on FLUID only; the first half-sweep's z0 is 0 here
-      if (t < D.g.size) stg(D.P.z[0] + t, (T)0);
+      if (t < D.g.size) stgi(D.P.z[0], t, (T)0);
       continue;
     }
     DD& acc = rs.acc;
@@ -1842,21 +1842,21 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
       for (int c = 0; c < 2; ++c) {
         const int q = row_q(D.g, t, c);
         const Nbr<DIM> n = neighbours_clamped<DIM>(D.g, t, q);
-        const int8_t gt = ldg(D.P.gt[c] + t);
+        const int8_t gt = ldgi(D.P.gt[c], t);
         const TX* xo = D.x[c ^ 1];
         TX xl[DIM], xh[DIM];
         T al[DIM], ah[DIM];
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
-          xl[a] = ldg(xo + n.lo[a]);
-          xh[a] = ldg(xo + n.hi[a]);
+          xl[a] = ldgi(xo, n.lo[a]);
+          xh[a] = ldgi(xo, n.hi[a]);
         }
         couplings_g<T, DIM>(D.P, kc, c, t, n, al, ah);
-        const T ad = ldg(D.P.Adiag[c] + t);
-        const TX xv = ldg(D.x[c] + t);
-        const T bv = ldg(D.b[c] + t);
+        const T ad = ldgi(D.P.Adiag[c], t);
+        const TX xv = ldgi(D.x[c], t);
+        const T bv = ldgi(D.b[c], t);
         if (gt != MG_FLUID) {
-          if (c == 0) stg(D.P.z[0] + t, (T)0);  // the V-cycle's first half-sweep
+          if (c == 0) stgi(D.P.z[0], t, (T)0);  // the V-cycle's first half-sweep
           continue;
         }
         double ret = 0.0;
@@ -1871,9 +1871,9 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
         Axv = Axv + ret;
         double rv = (double)bv - Axv;
         T r32 = (T)rv;
-        stg(D.P.r[c] + t, r32);
+        stgi(D.P.r[c], t, r32);
         acc = dd_add(acc, dd_prod(r32, r32));
-        if (c == 0) stg(D.P.z[0] + t, r32 / ad);  // the V-cycle's first half-sweep
+        if (c == 0) stgi(D.P.z[0], t, r32 / ad);  // the V-cycle's first half-sweep
       }
     }
   }
@@ -1883,11 +1883,11 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
 template <typename T, typename TX, int DIM, bool COMPACT>
 __device__ __forceinline__ void smooth0_dot_slot(const SysDesc<T, TX>& D, const Coupling<T>& kc,
                                                  int t, DD& acc) {
-  const int8_t g0 = ldg(D.P.gt[0] + t);
-  const int8_t g1 = ldg(D.P.gt[1] + t);
-  const T r0 = ldg(D.P.r[0] + t);
-  const T r1 = ldg(D.P.r[1] + t);
-  const T z1 = ldg(D.P.z[1] + t);
+  const int8_t g0 = ldgi(D.P.gt[0], t);
+  const int8_t g1 = ldgi(D.P.gt[1], t);
+  const T r0 = ldgi(D.P.r[0], t);
+  const T r1 = ldgi(D.P.r[1], t);
+  const T z1 = ldgi(D.P.z[1], t);
   const T z0 = smooth_slot_g<T, DIM, COMPACT>(D.g, D.P, kc, 0, t);
   if (g0 == MG_FLUID) acc = dd_add(acc, dd_prod(z0, r0));
   if (g1 == MG_FLUID) acc = dd_add(acc, dd_prod(z1, r1));
