@@ -81,6 +81,7 @@ template <typename T>
 struct Coupling {
   bool uniform;
   T coef;
+  T dg[8];  // diag_of_code(n, coef) for n < 8 (uniform levels; set by with_diags)
 };
 // The diagonal encoded by code n < NB_AD_STORED on a level whose couplings
 // are all `coef`: -coef added n times to 0, in order, as the operator
@@ -96,6 +97,19 @@ __host__ __device__ __forceinline__ T diag_of_code(uint32_t n, T coef) {
     a = i <= n ? b : a;
   }
   return a;
+}
+
+// the diagonals of every code, computed once for a coupling (the same
+// operations as diag_of_code, on values that are uniform over the launch)
+template <typename T>
+__host__ __device__ __forceinline__ void with_diags(Coupling<T>& k) {
+  T a = (T)0;
+  k.dg[0] = a;
+#pragma unroll
+  for (int i = 1; i < 8; ++i) {
+    a = a + (-k.coef);
+    k.dg[i] = a;
+  }
 }
 
 template <typename T>
@@ -342,6 +356,7 @@ __device__ __forceinline__ Coupling<T> coupling_g(const LevelPtrs<T>& P) {
       k.coef = (T)__uint_as_float(mn);
     }
   }
+  with_diags(k);
   return k;
 }
 
@@ -485,7 +500,13 @@ __device__ __forceinline__ T slot_diag(const LevelPtrs<T>& P, const Coupling<T>&
                                        uint32_t f, T stored) {
   if constexpr (COMPACT) {
     const uint32_t n = (f >> NB_AD) & 7u;
-    return n == NB_AD_STORED ? ldgi(P.Adiag[c], t) : diag_of_code<T>(n, k.coef);
+    if (n == NB_AD_STORED) return ldgi(P.Adiag[c], t);
+    // k.dg[n] by a select tree on the code's bits (no per-lane indexing)
+    const bool b0 = (n & 1u) != 0u, b1 = (n & 2u) != 0u, b2 = (n & 4u) != 0u;
+    const T d01 = b0 ? k.dg[1] : k.dg[0], d23 = b0 ? k.dg[3] : k.dg[2];
+    const T d45 = b0 ? k.dg[5] : k.dg[4], d67 = b0 ? k.dg[7] : k.dg[6];
+    const T lo = b1 ? d23 : d01, hi = b1 ? d67 : d45;
+    return b2 ? hi : lo;
   } else {
     return stored;
   }
