@@ -108,6 +108,15 @@ int mgpcg_load_level(mgpcg_solver_t* solver, int level, const int8_t* grid_type,
  * value (then *coef is it, or 0 when the level has no coupling); waits for
  * the handle's stream.  No reference counterpart: inspection only. */
 int mgpcg_level_coupling(mgpcg_solver_t* solver, int level, int* uniform, float* coef);
+/* Copies a loaded level's per-slot flag words to flags (device memory, 2 *
+ * slots-per-colour uint16: colour 0 then colour 1) and its per-block FLUID
+ * flags to wet (device memory, ceil(slots / 256) bytes), in the padded
+ * colour-split layout (mgpcg_info).  A flag word holds the nonzero-coupling
+ * bits (upper neighbour on axis a: bit a, lower: bit 3 + a) and in bits 6..8
+ * the diagonal code: n < 7 when Adiag equals n couplings of the slot's
+ * magnitude added in order from 0, 7 when it is read from Adiag.  Waits for
+ * the handle's stream.  No reference counterpart: inspection only. */
+int mgpcg_level_flags(mgpcg_solver_t* solver, int level, uint16_t* flags, uint8_t* wet);
 /* Loads b (the strategy's build_b output, mgpcg.py:20). */
 int mgpcg_load_rhs(mgpcg_solver_t* solver, const float* b);
 

@@ -1226,6 +1226,21 @@ extern "C" int mgpcg_load_level(mgpcg_solver_t* s, int level, const int8_t* grid
   return 0;
 }
 
+extern "C" int mgpcg_level_flags(mgpcg_solver_t* s, int level, uint16_t* flags, uint8_t* wet) {
+  MG_REQUIRE(s != nullptr, "solver is NULL");
+  MG_REQUIRE(level >= 0 && level < s->L, "level %d out of range [0, %d)", level, s->L);
+  MG_REQUIRE(flags != nullptr && wet != nullptr, "NULL argument");
+  MG_HIP(hipSetDevice(s->device));
+  const Geo& g = s->geo[level];
+  const LevelBuf& B = s->lv[level];
+  for (int c = 0; c < 2; ++c)
+    MG_HIP(hipMemcpyAsync(flags + (size_t)c * g.size, B.nb[c], sizeof(uint16_t) * g.size,
+                          hipMemcpyDeviceToDevice, s->user_stream));
+  MG_HIP(hipMemcpyAsync(wet, B.wet, nblk(g.size), hipMemcpyDeviceToDevice, s->user_stream));
+  MG_HIP(hipStreamSynchronize(s->user_stream));
+  return 0;
+}
+
 extern "C" int mgpcg_level_coupling(mgpcg_solver_t* s, int level, int* uniform, float* coef) {
   MG_REQUIRE(s != nullptr, "solver is NULL");
   MG_REQUIRE(level >= 0 && level < s->L, "level %d out of range [0, %d)", level, s->L);

@@ -200,6 +200,17 @@ class MGPCGPoissonSolver:
         _lib.check(self._lib.mgpcg_level_coupling(self._h, level, ctypes.byref(u), ctypes.byref(c)))
         return bool(u.value), c.value
 
+    def level_flags(self, level):
+        """(flags, wet) of a loaded level as numpy arrays: the per-slot flag words
+        [colour][slot] (uint16) and the per-block FLUID flags (uint8) of the
+        padded colour-split layout."""
+        n = self.info(level)["slots"]
+        flags = torch.empty(2 * n, dtype=torch.int16, device=self.device)
+        wet = torch.empty((n + 255) // 256, dtype=torch.uint8, device=self.device)
+        _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
+        _lib.check(self._lib.mgpcg_level_flags(self._h, level, _lib.ptr(flags), _lib.ptr(wet)))
+        return (flags.cpu().numpy().view("uint16").reshape(2, n), wet.cpu().numpy())
+
     # -- kernel-level access (parity tests) --------------------------------------
     def _vec_dtype(self, field):
         if self.precision == "f64" or (field == _lib.FIELD_X and self.precision != "plain"):
