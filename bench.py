@@ -25,6 +25,9 @@ Also reported, on the same requests:
                                 to last workgroup end) during the timed steps;
   roofline_isolated             the same with one engine stream, where no
                                 other lane's kernels share the device;
+  roofline_dense                the same kernel on a dense 256^3 system (every
+                                block has FLUID cells): its bandwidth when it
+                                cannot skip work;
   cpu_baseline                  the oracle's C restatement of mgpcg.py on the
                                 host cores (OpenMP), a bounded sample.
 """
@@ -51,6 +54,8 @@ SMOOTH_BYTES_PER_CELL_3D = 20.5
 # 4 / 2, own z written 4 / 2), and it skips blocks of 256 slots without a
 # FLUID cell.  Its physical HBM bytes per cell are measured with rocprofv3
 # counters (the committed traffic summary) and reported beside the model's.
+# On a dense system (every block FLUID) it moves 7.5 B per cell.
+SMOOTH_BYTES_PER_CELL_3D_DENSE = 7.5
 
 
 def parse():
@@ -216,6 +221,42 @@ def measure_served(args, workload, n, L, seeds, concurrency, steps, warmup, dev,
             "frac_traffic": (round(tpc * cells_tot / nl / launch_s / 1e9 / HBM_PEAK_GBS, 4)
                              if tpc is not None else None)}
     return out
+
+
+def measure_dense(args, dev, n=256, levels=6, reps=100):
+    """The level-0 half-sweep on one dense system: 256^3, every interior cell
+    FLUID (a 1-cell SOLID border), no ghost-fluid terms, so every block has work
+    and every diagonal is a plain sum.  Launch time from HIP events around
+    `reps` back-to-back launches (the gaps between them included)."""
+    import _lib
+    import mgpcg
+    import numpy as np
+    import scenes
+    import systems
+    import torch
+
+    res = [n] * 3
+    dx = 1.0 / n
+    phi = np.full(res, -1.0, np.float32)
+    sysd = scenes._pressure(res, dx, phi, scenes.cell_types(phi),
+                            scenes.mac_velocity(res, np.random.default_rng(5)), levels, False,
+                            100.0, 1e-3, 5, f"dense_{n}")
+    s = mgpcg.MGPCGPoissonSolver(3, res, levels, 2, 10, precision=args.precision, device=dev)
+    systems.assemble(s, systems.upload(sysd, dev))
+    s.load()
+    torch.cuda.synchronize()
+    ms = s.time_op(_lib.OP_SMOOTH, 0, 0, reps=reps)
+    cells = n ** 3
+    launch_s = ms * 1e-3
+    achieved = SMOOTH_BYTES_PER_CELL_3D * cells / launch_s / 1e9
+    own = SMOOTH_BYTES_PER_CELL_3D_DENSE * cells / launch_s / 1e9
+    del s
+    return {"system": f"{n}^3, every interior cell FLUID, uniform couplings, plain-sum diagonals",
+            "kernel": "mg::k_stage<float, 3, 0, 1>", "launch_us": round(ms * 1e3, 3),
+            "cells_per_launch": cells, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "kernel_bytes_per_cell": SMOOTH_BYTES_PER_CELL_3D_DENSE,
+            "achieved_kernel_bytes": round(own, 1), "frac_kernel_bytes": round(own / HBM_PEAK_GBS, 4)}
 
 
 def measure_batch64(args, dev):
@@ -414,6 +455,7 @@ def main():
                               "p50_ms": round(big["lat"]["p50_ms"], 3),
                               "p99_ms": round(big["lat"]["p99_ms"], 3), "iters": big["iters"],
                               "concurrency": c3["concurrency"], "roofline": big["roofline"]}
+            line["roofline_dense"] = measure_dense(args, dev)
             if not args.no_mix:
                 line["c4_batch64"] = measure_batch64(args, dev)
                 line["c5_poisson"] = measure_poisson(args, dev)
