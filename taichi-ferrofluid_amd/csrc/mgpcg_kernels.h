@@ -1551,6 +1551,15 @@ __device__ __forceinline__ void first_z0(const LevelPtrs<T>& P, const Coupling<T
   }
   stgi(P.z[0], t, z);
 }
+// the same with the slot's flag word (uniform couplings) or Adiag (otherwise)
+// loaded beforehand, with the kernel's other loads of the slot
+template <typename T>
+__device__ __forceinline__ void first_z0(const LevelPtrs<T>& P, const Coupling<T>& k, int t,
+                                         bool fluid, T r0, uint32_t f, T adl) {
+  T z = (T)0;
+  if (fluid) z = r0 / (k.uniform ? slot_diag<T, true>(P, k, 0, t, f, (T)0) : adl);
+  stgi(P.z[0], t, z);
+}
 
 // ---------------------------------------------------------------------------
 // CG kernels on level 0 (mgpcg.py:150-238).  Reductions: one FP64 partial per
@@ -1794,8 +1803,14 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
         const T apv = ldgi(D.Ap[c], t);
         const TX xv = ldgi(D.x[c], t);
         const T r0 = ldgi(D.P.r[c], t);
+        uint32_t f0 = 0u;  // the first half-sweep's diagonal operand (colour 0)
+        T ad0 = (T)0;
+        if (c == 0 && !refine) {
+          if (kc.uniform) f0 = ldgi(D.P.nb[0], t);
+          else ad0 = ldgi(D.P.Adiag[0], t);
+        }
         if (gt != MG_FLUID) {
-          if (c == 0 && !refine) first_z0<T>(D.P, kc, t, false, (T)0);
+          if (c == 0 && !refine) first_z0<T>(D.P, kc, t, false, (T)0, f0, ad0);
           continue;
         }
         if (prec == PLAIN) {
@@ -1805,7 +1820,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
           T rv = r0 - ta;
           stgi(D.P.r[c], t, rv);
           acc = dd_add(acc, dd_prod(rv, rv));
-          if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
+          if (c == 0) first_z0<T>(D.P, kc, t, true, rv, f0, ad0);
         } else if (prec == F64) {
           T tp = (T)(alpha * (double)pv);
           stgi(D.x[c], t, (TX)((double)xv + (double)tp));
@@ -1813,14 +1828,14 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
           T rv = r0 - ta;
           stgi(D.P.r[c], t, rv);
           acc = dd_add(acc, dd_prod(rv, rv));
-          if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
+          if (c == 0) first_z0<T>(D.P, kc, t, true, rv, f0, ad0);
         } else {
           stgi(D.x[c], t, (TX)((double)xv + alpha * (double)pv));
           if (!refine) {
             T rv = (T)((double)r0 - alpha * (double)apv);
             stgi(D.P.r[c], t, rv);
             acc = dd_add(acc, dd_prod(rv, rv));
-            if (c == 0) first_z0<T>(D.P, kc, t, true, rv);
+            if (c == 0) first_z0<T>(D.P, kc, t, true, rv, f0, ad0);
           }
         }
       }
