@@ -24,9 +24,12 @@
 // Compact couplings: when every nonzero off-diagonal coefficient of a level
 // has the same FP32 bit pattern (the pressure-type operators: -scale_A on
 // every FLUID-FLUID face, pressure_project.py:55-57), the stencil reads a
-// per-slot flag byte instead of the 2 * dim coefficients and uses
-// (flag ? coef : 0).  That value is bit-identical to the stored coefficient,
-// and the same multiply and add follow, so the result does not change.
+// 16-bit per-slot flag word instead of the 2 * dim coefficients and uses
+// (bit ? coef : 0).  That value is bit-identical to the stored coefficient,
+// and the same multiply and add follow, so the result does not change.  The
+// word also carries a 3-bit diagonal code: code n < 7 says Adiag equals -coef
+// added n times to 0 in FP32, so the diagonal is rebuilt instead of read;
+// code 7 falls back to the stored Adiag.
 #pragma once
 #include "mgpcg_common.h"
 
@@ -145,7 +148,7 @@ struct SysDesc {
   T* p[2][2];  // [iteration parity][colour]
   T* Ap[2];
   CGState* st;
-  DD* partials;  // one per virtual block of the level-0 CG kernels
+  DD* partials;  // one per 256-slot block of level 0 (a workgroup run's total in its first)
   int nparts;
   int coarse_from;  // first level handled by the single-workgroup coarse kernel
   int L;
@@ -1562,8 +1565,9 @@ __device__ __forceinline__ void first_z0(const LevelPtrs<T>& P, const Coupling<T
 }
 
 // ---------------------------------------------------------------------------
-// CG kernels on level 0 (mgpcg.py:150-238).  Reductions: one FP64 partial per
-// virtual block; a per-system finalize kernel sums them in a fixed order.
+// CG kernels on level 0 (mgpcg.py:150-238).  Reductions: one double-double
+// partial per virtual block (a workgroup run's total in its first virtual
+// block, zero in the others); a per-system finalize kernel sums them.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double rnd_plain(const CGState* st, double v) {
   return st->precision == PLAIN ? (double)(float)v : v;

@@ -58,7 +58,7 @@ int g_max_grid = 2048;          // workgroups of a persistent stage launch
 int g_fuse_first = 0;    // first pre-smoothing pair in one kernel (else two)
 int g_fuse_prolong = 1;  // prolongation inside the first post-smoothing half-sweep
 int g_fuse_p = 0;        // update_p inside compute_Ap
-int g_compact = 1;       // flag-byte stencils on levels with uniform couplings
+int g_compact = 1;       // flag-word stencils on levels with uniform couplings
 int g_coarse_lds = 1;    // coarse kernel works on an LDS image of its levels when they fit
 // restriction kernels: 0 one lane per child, 1 row-tiled with LDS, 2 residual
 // pass + child gather (both coalesced)

@@ -138,7 +138,7 @@ def test_prolong_smooth1_bit_exact(dev, case):
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 def test_restrict_bit_exact(dev, case, coupling, rows):
     """Every restriction kernel (residual + gather, row-tiled, child-parallel), with stored and
-    with uniform (flag-byte) couplings, against the oracle bit for bit.  The
+    with uniform (flag-word) couplings, against the oracle bit for bit.  The
     gather skips blocks of coarse cells none of which is FLUID (their r is
     never read), so it is compared on the FLUID coarse cells; the other two
     write every coarse cell and are compared everywhere."""
