@@ -88,6 +88,9 @@ def parse():
     ap.add_argument("--probe-warmup", action="store_true",
                     help="count the warm-up requests' level-0 sweeps too (counter runs, which "
                          "attribute every dispatch of the process)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="first tune the schedule knobs on this workload (autotune.py) and "
+                         "measure with the tuned settings")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain copy kernel over this many MiB (3 launches), the "
                          "known-size reference for HBM byte counters")
@@ -357,6 +360,19 @@ def main():
     wl, n, L = args.workload, args.n, args.levels
     what = WORKLOADS[wl]["what"]
 
+    tuned = None
+    if args.autotune:
+        import autotune
+
+        scs = [systems.upload(make_scene(wl, i, n, L), dev) for i in range(args.seeds)]
+        tuned = autotune.tune(lambda i: scs[i % len(scs)], L, concurrency=args.concurrency,
+                              precision=args.precision, device=dev,
+                              log=lambda t: print("autotune", json.dumps(t), file=sys.stderr,
+                                                  flush=True))
+        args.server_depth = tuned["knobs"]["server_depth"]
+        args.server_streams = tuned["knobs"]["server_streams"]
+        del scs
+
     # headline: continuous batching, closed loop
     head = measure_served(args, wl, n, L, args.seeds, args.concurrency, args.steps, args.warmup,
                           dev)
@@ -381,6 +397,9 @@ def main():
                    "n": n, "levels": L, "seeds": args.seeds, "concurrency": args.concurrency,
                    "admit_per_step": args.admit_per_step, "server_depth": args.server_depth,
                    "server_streams": args.server_streams},
+        **({"autotune": {"knobs": tuned["knobs"], "solves_per_s": tuned["solves_per_s"],
+                         "start": tuned["start"], "trials": len(tuned["trials"])}}
+           if tuned else {}),
         "p50_ms": round(head["lat"]["p50_ms"], 3),
         "p99_ms": round(head["lat"]["p99_ms"], 3),
         "iters": head["iters"],

@@ -91,7 +91,14 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
                                      the host, 1..3 (default 2) */
 #define MGPCG_OPT_SERVER_STREAMS 13 /* engines (each with its own stream) a server created
                                        afterwards spreads its members over, 1..4 (default 2) */
+#define MGPCG_OPT_GRAPH 14        /* 1: an engine captures one iteration of its current member
+                                     set into a hipGraph on the set's second iteration and
+                                     replays it until the set or an option changes (default
+                                     0); the launch probe and mgpcg_solve_hostsync launch
+                                     kernel by kernel */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
+/* The current value of a knob (no reference counterpart; `solver` is unused). */
+int mgpcg_get_option(const mgpcg_solver_t* solver, int option, int* value);
 
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,
  * first coarse level}; bytes = device bytes held by the handle. */
@@ -199,6 +206,12 @@ int mgpcg_server_probe(mgpcg_server_t* server, int enable, int* n_launches, doub
                        int64_t* total_cells);
 int mgpcg_batch_probe(mgpcg_batch_t* batch, int enable, int* n_launches, double* total_ms,
                       int64_t* total_cells);
+/* Measurement (no reference counterpart): iteration graphs captured and
+ * iterations launched from a graph so far by a solver's own engine, a batch
+ * or a server (summed over its engine streams).  Exactly one handle is given;
+ * the other two are NULL. */
+int mgpcg_graph_counts(const mgpcg_solver_t* solver, const mgpcg_batch_t* batch,
+                       const mgpcg_server_t* server, int64_t* captures, int64_t* launches);
 
 /* Kernel-level access for parity tests (no reference counterpart): move a
  * level vector between canonical layout (float, or double for MGPCG_F64 and

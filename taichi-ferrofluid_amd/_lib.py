@@ -19,7 +19,7 @@ FIELD_R, FIELD_Z, FIELD_P, FIELD_AP, FIELD_X, FIELD_P_NEXT = 0, 1, 2, 3, 4, 5
  OP_PROLONG_SMOOTH1) = range(7)
 (OPT_COARSE_MAX_SLOTS, OPT_MAX_GRID, OPT_FUSE_FIRST, OPT_FUSE_PROLONG, OPT_FUSE_P,
  OPT_COMPACT, OPT_COARSE_LDS, OPT_RESTRICT_ROWS, OPT_PROBE_EVERY,
- OPT_SERVER_DEPTH, OPT_SERVER_STREAMS) = 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13
+ OPT_SERVER_DEPTH, OPT_SERVER_STREAMS, OPT_GRAPH) = 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -48,6 +48,7 @@ SIGNATURES = {
     "mgpcg_destroy": (_I, [_P]),
     "mgpcg_set_stream": (_I, [_P, _P]),
     "mgpcg_set_option": (_I, [_P, _I, _I]),
+    "mgpcg_get_option": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "mgpcg_info": (_I, [_P, _I, _P, _P]),
     "mgpcg_load_level": (_I, [_P, _I, _P, _P, _P]),
     "mgpcg_load_rhs": (_I, [_P, _P]),
@@ -96,6 +97,8 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_int64)]),
     "mgpcg_batch_probe": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_D),
                                ctypes.POINTER(ctypes.c_int64)]),
+    "mgpcg_graph_counts": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_int64)]),
     "mgpcg_last_error": (ctypes.c_char_p, []),
     "mgpcg_version": (_I, []),
 }
@@ -170,6 +173,13 @@ def stream_handle(device=None):
 def synchronize(device):
     if torch.device(device).type != "cpu":
         torch.cuda.synchronize(device)
+
+
+def graph_counts(solver=None, batch=None, server=None):
+    """(captures, graph launches) of exactly one handle's engines."""
+    c, n = ctypes.c_int64(), ctypes.c_int64()
+    check(load().mgpcg_graph_counts(solver, batch, server, ctypes.byref(c), ctypes.byref(n)))
+    return c.value, n.value
 
 
 def hip_runtimes_mapped():

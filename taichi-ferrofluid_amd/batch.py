@@ -29,6 +29,10 @@ class BatchSolver:
                 pass
             self._h = None
 
+    def graph_counts(self):
+        """(iteration graphs captured, iterations launched from a graph)."""
+        return _lib.graph_counts(batch=self._h)
+
     def solve(self, solvers, max_iters=500, rel_tol=1e-12, abs_tol=1e-14, eps=1e-12,
               refine_every=None):
         """Solve every system in ``solvers``; returns the list of result dicts."""

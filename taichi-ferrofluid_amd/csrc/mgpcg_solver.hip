@@ -66,6 +66,8 @@ int g_restrict_mode = 2;
 int g_probe_every = 4;    // the launch probe times one eligible launch in this many
 int g_server_depth = 2;   // server iterations queued on the device ahead of the host
 int g_server_streams = 2; // engines (streams) a server spreads its members over
+int g_graph = 0;          // replay iterations of an unchanged member set from a hipGraph
+long g_knob_epoch = 0;    // bumped by every option change (captured graphs go stale)
 
 }  // namespace
 
@@ -237,9 +239,36 @@ struct Engine {
   // [i * SPAN_SLOTS], ends into maxima at [(PROBE_POOL + i) * SPAN_SLOTS]
   unsigned long long* d_span = nullptr;
 
+  // hipGraph of one iteration of the current member set (g_graph): captured
+  // on the second iteration of an item layout, replayed until the layout or
+  // an option changes.  Replaced graphs are destroyed once the stream has
+  // passed their last launch.
+  long epoch = 0;        // bumped by every engine_build_items
+  int epoch_eager = 0;   // iterations launched one kernel at a time in this epoch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  long graph_epoch = -1, graph_knobs = -1;
+  long graph_launches = 0, graph_captures = 0;
+  struct RetiredGraph {
+    hipGraph_t g;
+    hipGraphExec_t x;
+    hipEvent_t ev;
+  };
+  std::vector<RetiredGraph> graph_retired;
+
   ~Engine() { release(); }
   void release() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (RetiredGraph& r : graph_retired) {
+      (void)hipGraphExecDestroy(r.x);
+      (void)hipGraphDestroy(r.g);
+      (void)hipEventDestroy(r.ev);
+    }
+    graph_retired.clear();
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
     if (h_desc) (void)hipHostFree(h_desc);
     if (h_stinit) (void)hipHostFree(h_stinit);
     h_desc = nullptr;
@@ -564,6 +593,8 @@ int engine_build_items(Engine* e) {
   }
   MG_HIP(hipEventRecord(e->ev_items[sl], e->stream));
   e->dirty = false;
+  e->epoch++;
+  e->epoch_eager = 0;
   return 0;
 }
 
@@ -837,10 +868,81 @@ void fill_result(const CGState& h, mgpcg_result_t* out) {
   out->beta = h.beta;
 }
 
-// Enqueue one iteration plus the readback of the states of its members.
+// Destroy the replaced graphs whose last launch the stream has passed (all of
+// them when `all`, after synchronizing).
+int engine_graph_reap(Engine* e, bool all) {
+  if (all && !e->graph_retired.empty()) MG_HIP(hipStreamSynchronize(e->stream));
+  size_t k = 0;
+  for (size_t i = 0; i < e->graph_retired.size(); ++i) {
+    Engine::RetiredGraph& r = e->graph_retired[i];
+    const hipError_t q = all ? hipSuccess : hipEventQuery(r.ev);
+    if (q == hipSuccess) {
+      MG_HIP(hipGraphExecDestroy(r.x));
+      MG_HIP(hipGraphDestroy(r.g));
+      MG_HIP(hipEventDestroy(r.ev));
+    } else if (q == hipErrorNotReady) {
+      e->graph_retired[k++] = r;
+    } else {
+      MG_HIP(q);
+    }
+  }
+  e->graph_retired.resize(k);
+  return 0;
+}
+
+// Launch the current iteration from the engine's graph, capturing it first
+// when the item layout or an option changed since the last capture.  The
+// iteration's kernels take only the engine's tables and item lists as
+// arguments and read every per-system scalar (iteration parity, alpha, beta,
+// done) from the device, so one capture serves every iteration of a layout.
+int engine_graph_launch(Engine* e) {
+  if (!(e->graph_exec && e->graph_epoch == e->epoch && e->graph_knobs == g_knob_epoch)) {
+    if (e->graph_exec) {
+      Engine::RetiredGraph r{e->graph, e->graph_exec, nullptr};
+      MG_HIP(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+      MG_HIP(hipEventRecord(r.ev, e->stream));
+      e->graph_retired.push_back(r);
+      e->graph = nullptr;
+      e->graph_exec = nullptr;
+    }
+    MG_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+    dispatch<RunIteration>(e->precision, e->dim, e, false);
+    hipGraph_t g = nullptr;
+    const hipError_t rc = hipStreamEndCapture(e->stream, &g);
+    if (rc != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      MG_HIP(rc);
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ri = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ri != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      MG_HIP(ri);
+    }
+    e->graph = g;
+    e->graph_exec = x;
+    e->graph_epoch = e->epoch;
+    e->graph_knobs = g_knob_epoch;
+    e->graph_captures++;
+  }
+  MG_HIP(hipGraphLaunch(e->graph_exec, e->stream));
+  e->graph_launches++;
+  return 0;
+}
+
+// Enqueue one iteration plus the readback of the states of its members.  With
+// g_graph the iteration comes from a graph once its item layout has run
+// eagerly once (a layout with newly admitted members lasts one iteration);
+// the launch probe and host-synchronized iterations always launch eagerly.
 int engine_enqueue(Engine* e, int slot, bool sync_each) {
   if (e->dirty && engine_build_items(e)) return -2;
-  dispatch<RunIteration>(e->precision, e->dim, e, sync_each);
+  if (!e->graph_retired.empty() && engine_graph_reap(e, false)) return -2;
+  if (g_graph && !sync_each && !e->probe_on && e->epoch_eager > 0) {
+    if (engine_graph_launch(e)) return -2;
+  } else {
+    dispatch<RunIteration>(e->precision, e->dim, e, sync_each);
+    e->epoch_eager++;
+  }
   dispatch<RunGather>(e->precision, e->dim, e, slot);
   e->gathered[slot].clear();
   for (int m = 0; m < (int)e->mem.size(); ++m)
@@ -1125,6 +1227,7 @@ extern "C" int mgpcg_set_stream(mgpcg_solver_t* s, void* stream) {
 }
 
 extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
+  g_knob_epoch++;
   switch (option) {
     case MGPCG_OPT_COARSE_MAX_SLOTS:
       MG_REQUIRE(value >= 0, "coarse slot limit must be >= 0");
@@ -1166,8 +1269,33 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
       MG_REQUIRE(value >= 1 && value <= 4, "server streams must be in [1, 4]");
       g_server_streams = value;
       return 0;
+    case MGPCG_OPT_GRAPH:
+      g_graph = value != 0;
+      return 0;
     default:
       (void)s;
+      mg_set_error("unknown option %d", option);
+      return -1;
+  }
+}
+
+extern "C" int mgpcg_get_option(const mgpcg_solver_t* s, int option, int* value) {
+  (void)s;
+  MG_REQUIRE(value != nullptr, "value is NULL");
+  switch (option) {
+    case MGPCG_OPT_COARSE_MAX_SLOTS: *value = g_coarse_max_slots; return 0;
+    case MGPCG_OPT_MAX_GRID: *value = g_max_grid; return 0;
+    case MGPCG_OPT_FUSE_FIRST: *value = g_fuse_first; return 0;
+    case MGPCG_OPT_FUSE_PROLONG: *value = g_fuse_prolong; return 0;
+    case MGPCG_OPT_FUSE_P: *value = g_fuse_p; return 0;
+    case MGPCG_OPT_COMPACT: *value = g_compact; return 0;
+    case MGPCG_OPT_COARSE_LDS: *value = g_coarse_lds; return 0;
+    case MGPCG_OPT_RESTRICT_ROWS: *value = g_restrict_mode; return 0;
+    case MGPCG_OPT_PROBE_EVERY: *value = g_probe_every; return 0;
+    case MGPCG_OPT_SERVER_DEPTH: *value = g_server_depth; return 0;
+    case MGPCG_OPT_SERVER_STREAMS: *value = g_server_streams; return 0;
+    case MGPCG_OPT_GRAPH: *value = g_graph; return 0;
+    default:
       mg_set_error("unknown option %d", option);
       return -1;
   }
@@ -1978,6 +2106,26 @@ extern "C" int mgpcg_server_probe(mgpcg_server_t* sv, int enable, int* n_launche
     if (total_ms) *total_ms = ms;
     if (total_cells) *total_cells = cells;
   }
+  return 0;
+}
+
+extern "C" int mgpcg_graph_counts(const mgpcg_solver_t* s, const mgpcg_batch_t* b,
+                                  const mgpcg_server_t* sv, int64_t* captures,
+                                  int64_t* launches) {
+  MG_REQUIRE((s != nullptr) + (b != nullptr) + (sv != nullptr) == 1,
+             "exactly one of solver, batch and server must be given");
+  int64_t c = 0, n = 0;
+  auto add = [&](const Engine* e) {
+    if (!e) return;
+    c += e->graph_captures;
+    n += e->graph_launches;
+  };
+  if (s) add(s->engine);
+  if (b) add(b->e);
+  if (sv)
+    for (const mgpcg_server::Lane& ln : sv->lanes) add(ln.e);
+  if (captures) *captures = c;
+  if (launches) *launches = n;
   return 0;
 }
 

@@ -186,6 +186,11 @@ class MGPCGPoissonSolver:
     def set_option(self, option, value):
         _lib.check(self._lib.mgpcg_set_option(self._h, option, value))
 
+    def graph_counts(self):
+        """(iteration graphs captured, iterations launched from a graph) by this
+        solver's own engine (MGPCG_OPT_GRAPH)."""
+        return _lib.graph_counts(solver=self._h)
+
     def info(self, level=0):
         out = (ctypes.c_int32 * 6)()
         nbytes = ctypes.c_int64()

@@ -184,6 +184,11 @@ class Server:
                                                 ctypes.byref(ms), ctypes.byref(cells)))
         return n.value, ms.value, cells.value
 
+    def graph_counts(self):
+        """(iteration graphs captured, iterations launched from a graph), summed
+        over the engine streams."""
+        return _lib.graph_counts(server=self._h)
+
     def busy(self):
         npend, nact, infl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _lib.check(self._lib.mgpcg_server_pending(self._h, ctypes.byref(npend),

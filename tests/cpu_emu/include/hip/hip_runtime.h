@@ -14,6 +14,8 @@
 #include <string.h>
 
 #include <cmath>
+#include <functional>
+#include <memory>
 #include <tuple>
 #include <utility>
 
@@ -32,6 +34,7 @@ typedef enum hipError_t {
   hipSuccess = 0,
   hipErrorInvalidValue = 1,
   hipErrorOutOfMemory = 2,
+  hipErrorNotReady = 600,
   hipErrorLaunchFailure = 719,
 } hipError_t;
 
@@ -45,8 +48,19 @@ typedef enum hipMemcpyKind {
 
 struct ihipStream_t;
 struct ihipEvent_t;
+struct ihipGraph_t;
+struct ihipGraphExec_t;
+struct ihipGraphNode_t;
 typedef ihipStream_t* hipStream_t;
 typedef ihipEvent_t* hipEvent_t;
+typedef ihipGraph_t* hipGraph_t;
+typedef ihipGraphExec_t* hipGraphExec_t;
+typedef ihipGraphNode_t* hipGraphNode_t;
+enum hipStreamCaptureMode {
+  hipStreamCaptureModeGlobal = 0,
+  hipStreamCaptureModeThreadLocal = 1,
+  hipStreamCaptureModeRelaxed = 2,
+};
 
 #define hipStreamNonBlocking 1
 #define hipEventDisableTiming 2
@@ -69,6 +83,10 @@ unsigned long long ballot(int pred);
 void run_grid(dim3 grid, dim3 block, void (*tramp)(void*), void* ctx);
 void* guarded_alloc(size_t bytes);
 void guarded_free(void* p);
+// stream capture: while a capture is open, launches and async copies are
+// recorded into the graph instead of running
+bool capturing();
+void record(std::function<void()> op);
 }  // namespace hipemu
 
 #define threadIdx (::hipemu::threadIdx)
@@ -170,6 +188,14 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
+hipError_t hipEventQuery(hipEvent_t e);
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode mode);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g);
+hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t g, hipGraphNode_t* err_node,
+                               char* log, size_t log_size);
+hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s);
+hipError_t hipGraphExecDestroy(hipGraphExec_t x);
+hipError_t hipGraphDestroy(hipGraph_t g);
 
 // kernel launch: synchronous, every block's threads as coroutines
 namespace hipemu {
@@ -186,8 +212,15 @@ void tramp(void* p) {
 template <typename... KArgs, typename... Args>
 void launch(void (*kernel)(KArgs...), dim3 grid, dim3 block, Args&&... args) {
   typedef std::tuple<typename std::decay<KArgs>::type...> Tup;
-  LaunchCtx<void (*)(KArgs...), Tup> ctx{kernel, Tup(std::forward<Args>(args)...)};
-  run_grid(grid, block, &tramp<void (*)(KArgs...), Tup>, &ctx);
+  typedef LaunchCtx<void (*)(KArgs...), Tup> Ctx;
+  auto ctx = std::make_shared<Ctx>(Ctx{kernel, Tup(std::forward<Args>(args)...)});
+  auto run = [ctx, grid, block]() {
+    run_grid(grid, block, &tramp<void (*)(KArgs...), Tup>, ctx.get());
+  };
+  if (capturing())
+    record(run);
+  else
+    run();
 }
 }  // namespace hipemu
 
