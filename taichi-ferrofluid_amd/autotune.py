@@ -76,7 +76,8 @@ def _served(make_scene, n_levels, precision, concurrency, requests, warmup, devi
     effect) in steady state: `concurrency` requests stay in flight throughout,
     and the clock runs from the `warmup`-th completion to the
     (`warmup` + `requests`)-th.  Returns (solves/s, {request index: x of the
-    timed requests as CPU tensors})."""
+    timed requests}); x stays on the device (a copy in stream order), so
+    keeping it does not stall the host's submissions."""
     import serve
 
     srv = serve.Server(64 << 30, precision=precision, n_levels=n_levels, device=device)
@@ -90,7 +91,7 @@ def _served(make_scene, n_levels, precision, concurrency, requests, warmup, devi
         for req in srv.step():
             if warmup <= finished < total:
                 sv = req["solver"]
-                xs[req["meta"]] = (sv.x64 if sv.x64 is not None else sv.x).cpu()
+                xs[req["meta"]] = (sv.x64 if sv.x64 is not None else sv.x).clone()
             srv.retire(req)
             finished += 1
             if finished == warmup:
@@ -136,9 +137,10 @@ def tune(make_scene, n_levels, concurrency=8, requests=48, warmup=8, precision="
         return rate
 
     try:
-        # the first server of a process pays for code loading and first-touch
-        # allocation: one untimed round first
-        _served(make_scene, n_levels, precision, concurrency, 1, 1, device)
+        # the first servers of a process pay for code loading and first-touch
+        # allocation: one untimed trial of the starting settings first
+        apply(best)
+        _served(make_scene, n_levels, precision, concurrency, concurrency, concurrency, device)
         best_rate = measure(best)
         start_rate = best_rate
         for _ in range(max_passes):
