@@ -124,6 +124,15 @@ int mgpcg_level_coupling(mgpcg_solver_t* solver, int level, int* uniform, float*
  * magnitude added in order from 0, 7 when it is read from Adiag.  Waits for
  * the handle's stream.  No reference counterpart: inspection only. */
 int mgpcg_level_flags(mgpcg_solver_t* solver, int level, uint16_t* flags, uint8_t* wet);
+/* Copies a loaded level's work orders (device memory, int32): ord1 gets one
+ * entry per block of 256 slots (natural order), ord2 one per half-sweep
+ * virtual block of 512 slots (a permutation with the FLUID-holding ones
+ * spread evenly over the positions; only level 0, when the batched kernels
+ * solve it, has one: ord2 is left alone for the others and may be NULL).  Entry k
+ * names the block (low 31 bits) that the k-th virtual block of a system's
+ * stage processes; bit 31 is set when that block holds a FLUID slot.  Waits
+ * for the handle's stream.  No reference counterpart: inspection only. */
+int mgpcg_level_order(mgpcg_solver_t* solver, int level, int32_t* ord1, int32_t* ord2);
 /* Loads b (the strategy's build_b output, mgpcg.py:20). */
 int mgpcg_load_rhs(mgpcg_solver_t* solver, const float* b);
 

@@ -54,6 +54,7 @@ SIGNATURES = {
     "mgpcg_load_rhs": (_I, [_P, _P]),
     "mgpcg_level_coupling": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float)]),
     "mgpcg_level_flags": (_I, [_P, _I, _P, _P]),
+    "mgpcg_level_order": (_I, [_P, _I, _P, _P]),
     "mgpcg_coarsen": (_I, [_P, _I, _P, _P, _P]),
     "mgpcg_solve": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(MgpcgResult)]),
     "mgpcg_solve_hostsync": (_I, [_P, _I, _D, _D, _D, _I, ctypes.POINTER(MgpcgResult)]),

@@ -55,6 +55,11 @@ __device__ __forceinline__ void stgi(T* base, int i, T v) {
 // Makes a loaded value live at this point of the program, so the compiler
 // cannot sink its load into a later branch (all of a slot's loads are then in
 // flight together).  Emits no instruction.
+// Lane `l` (wave-uniform) of a 32-bit value held across the wave.
+#ifndef MG_READLANE
+#define MG_READLANE(v, l) __builtin_amdgcn_readlane((v), (l))
+#endif
+
 #ifndef MG_KEEP
 #define MG_KEEP(x) asm volatile("" ::"v"(x))
 #define MG_KEEP4(a, b, c, d) asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d))

@@ -77,6 +77,13 @@ struct LevelPtrs {
   // colour 0 before the first half-sweep) is written by the packing kernel or
   // stored without loads.
   const uint8_t* wet;
+  // the level's work orders: entry k says which block the k-th virtual block
+  // of a system's stage processes and whether it is wet.  [0]: blocks of
+  // BLOCK slots, in natural order (written by the packing kernel); [1]:
+  // half-sweep virtual blocks of BLOCK * SLOT_LANES slots, the wet ones
+  // spread evenly (k_level_order; level 0 on the batched kernels only).
+  // NULL: natural order, wet from the block flags.
+  const int32_t* ord[2];
 };
 
 // The level's single coupling value when its couplings are uniform.
@@ -414,30 +421,43 @@ __device__ __forceinline__ bool wet_blocks(const LevelPtrs<T>& P, const Geo& g, 
   return w;
 }
 
-// The FLUID flags of a workgroup's next 64 virtual blocks, fetched at once
-// (lane l loads the flags of the block l iterations ahead, a wave ballot
-// gathers them) instead of one dependent load per block.  `iter` counts the
-// workgroup's loop iterations; the mask is refetched when the item changes
-// (invalidate) or after 64 iterations.  Every call is block-uniform.
-struct WetScan {
-  unsigned long long mask = 0ull;
+// Work order entries: the block to process (ORD_BLOCK bits) and whether it
+// holds a FLUID slot (ORD_WET).  The packing pass spreads a level's wet blocks
+// evenly over the positions of its order, so every workgroup's share of
+// positions holds about the same number of wet blocks, whatever the layout of
+// the scene's FLUID cells.
+constexpr int32_t ORD_WET = (int32_t)0x80000000u;
+constexpr int32_t ORD_BLOCK = 0x7fffffff;
+
+// The order entries of a workgroup's next 64 virtual blocks, fetched at once
+// (lane l loads the entry of the virtual block l iterations ahead) and read
+// back one per iteration from the lane that holds it, instead of one
+// dependent load per block.  `iter` counts the workgroup's loop iterations;
+// the entries are refetched when the item changes (invalidate) or after 64
+// iterations.  Every call is block-uniform.
+struct OrdScan {
+  int32_t v = 0;
   int base = 0;
   bool valid = false;
 };
 template <typename T>
-__device__ __forceinline__ bool wet_here(WetScan& w, int iter, int vb, int step, int end,
-                                         int item_begin, int item_end, const LevelPtrs<T>& P,
-                                         const Geo& g, int span) {
+__device__ __forceinline__ int32_t ord_here(OrdScan& w, int iter, int vb, int step, int end,
+                                           int item_begin, int item_end, const LevelPtrs<T>& P,
+                                           const Geo& g, int span) {
   if (!w.valid || iter - w.base >= 64) {
     const int l = threadIdx.x & 63;
     const int v = vb + l * step;
-    bool f = true;  // beyond the item: decided when reached
-    if (v < end && v < item_end) f = wet_blocks(P, g, (v - item_begin) * span, span);
-    w.mask = __ballot(f);
+    int32_t e = 0;  // beyond the item: refetched when reached
+    if (v < end && v < item_end) {
+      const int k = v - item_begin;
+      const int32_t* o = P.ord[span == 1 ? 0 : 1];
+      e = o != nullptr ? ldg(o + k) : (k | (wet_blocks(P, g, k * span, span) ? ORD_WET : 0));
+    }
+    w.v = e;
     w.base = iter;
     w.valid = true;
   }
-  return ((w.mask >> (iter - w.base)) & 1ull) != 0ull;
+  return MG_READLANE(w.v, iter - w.base);
 }
 
 // the item of virtual block vb, kept while the workgroup's next virtual
@@ -1017,7 +1037,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     if (S.t0 != nullptr && threadIdx.x == 0)
       atomicMin(S.t0 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
   }
-  WetScan ws;
+  OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
@@ -1035,17 +1055,20 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     if constexpr (OP == ST_SMOOTH0 || OP == ST_SMOOTH1 || OP == ST_SMOOTH1_RES) {
       constexpr int c = OP == ST_SMOOTH0 ? 0 : 1;
       constexpr bool RES = OP == ST_SMOOTH1_RES;
-      const int lb = it.vb - ic.block0;
-      if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, SLOT_LANES)) continue;  // no FLUID cell
-      const int t0 = lb * (BLOCK * SLOT_LANES) + threadIdx.x;
+      const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, SLOT_LANES);
+      if (e >= 0) continue;  // no FLUID cell
+      const int t0 = (e & ORD_BLOCK) * (BLOCK * SLOT_LANES) + threadIdx.x;
       if (kc.uniform) smooth_slots<T, DIM, true, RES>(D.g, D.P, kc, c, t0);
       else smooth_slots<T, DIM, false, RES>(D.g, D.P, kc, c, t0);
       continue;
     }
-    if constexpr (OP == ST_PROLONG_SMOOTH1) {  // before any per-lane exit: the scan is a ballot
-      if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;
+    int lb = it.vb - ic.block0;
+    if constexpr (OP == ST_PROLONG_SMOOTH1) {  // before any per-lane exit: the scan reads lanes
+      const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
+      if (e >= 0) continue;
+      lb = e & ORD_BLOCK;
     }
-    const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
+    const int t = lb * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
     if (OP == ST_FIRST) {
       if (kc.uniform) first_slot_g<T, true>(D.P, kc, t);
@@ -1210,7 +1233,7 @@ __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restri
   ItemCache ic;
   bool done = true;
   Coupling<T> kc{false, (T)0};
-  WetScan ws;
+  OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
@@ -1223,13 +1246,15 @@ __global__ void __launch_bounds__(BLOCK) k_residual(const LevelDesc<T>* __restri
     const LevelDesc<T>& D = descs[ic.desc];
     // a block without FLUID cells keeps the 0 residuals the packing wrote
     if constexpr (C0) {
-      if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, SLOT_LANES)) continue;
-      const int t0 = (it.vb - ic.block0) * (BLOCK * SLOT_LANES) + threadIdx.x;
+      const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, SLOT_LANES);
+      if (e >= 0) continue;
+      const int t0 = (e & ORD_BLOCK) * (BLOCK * SLOT_LANES) + threadIdx.x;
       if (kc.uniform) residual0_slots<T, DIM, true>(D, kc, t0);
       else residual0_slots<T, DIM, false>(D, kc, t0);
     } else {
-      if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;
-      const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
+      const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
+      if (e >= 0) continue;
+      const int t = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
       if (t >= D.g.size) continue;
       if (kc.uniform) residual_slot<T, DIM, true>(D, kc, t);
       else residual_slot<T, DIM, false>(D, kc, t);
@@ -1245,7 +1270,7 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_gather(const LevelDesc<T>* _
                                                            Stage S) {
   ItemCache ic;
   bool done = true;
-  WetScan ws;
+  OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
@@ -1256,8 +1281,9 @@ __global__ void __launch_bounds__(BLOCK) k_restrict_gather(const LevelDesc<T>* _
     const LevelDesc<T>& D = descs[ic.desc];
     // a coarse cell that is not FLUID never has its r read: a block of such
     // coarse cells is skipped
-    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.C, D.gc, 1)) continue;
-    const int tc = (it.vb - ic.block0) * BLOCK + threadIdx.x;
+    const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.C, D.gc, 1);
+    if (e >= 0) continue;
+    const int tc = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
     if (tc >= D.gc.size) continue;
 #pragma unroll
     for (int cc = 0; cc < 2; ++cc) {
@@ -1665,7 +1691,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
   int par = 0;
   float b32 = 0.0f;
   double beta = 0.0;
-  WetScan ws;
+  OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
@@ -1679,8 +1705,9 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
     }
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
-    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;  // p changes on FLUID only
-    const int t = (it.vb - ic.block0) * BLOCK + threadIdx.x;
+    const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
+    if (e >= 0) continue;  // p changes on FLUID only
+    const int t = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
     if (t >= D.g.size) continue;
     int gts[2];
     T polds[2], zvs[2];
@@ -1735,7 +1762,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
   bool done = true;
   int par = 0;
   Coupling<T> kc{false, (T)0};
-  WetScan ws;
+  OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
@@ -1748,10 +1775,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
     }
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
-    const int lb = it.vb - ic.block0;
-    run_mark(rs, D.partials, lb);
-    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;  // Ap and p.Ap on FLUID only
-    const int t = lb * BLOCK + threadIdx.x;
+    run_mark(rs, D.partials, it.vb - ic.block0);
+    const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
+    if (e >= 0) continue;  // Ap and p.Ap on FLUID only
+    const int t = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
     if (t < D.g.size) {
       if (kc.uniform) ap_slot<T, TX, DIM, true>(D, kc, par, t, rs.acc);
       else ap_slot<T, TX, DIM, false>(D, kc, par, t, rs.acc);
@@ -1772,7 +1799,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
   int par = 0, prec = PLAIN;
   float a32 = 0.0f;
   double alpha = 0.0;
-  WetScan ws;
+  OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
@@ -1790,10 +1817,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     }
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
-    const int lb = it.vb - ic.block0;
-    run_mark(rs, D.partials, lb);
-    const int t = lb * BLOCK + threadIdx.x;
-    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) {
+    run_mark(rs, D.partials, it.vb - ic.block0);
+    const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
+    const int t = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
+    if (e >= 0) {
       // x and r change on FLUID only; the first half-sweep's z0 is 0 here
       if (t < D.g.size && !refine) stgi(D.P.z[0], t, (T)0);
       continue;
@@ -1856,7 +1883,7 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
   RunSum rs;
   bool skip = true;
   Coupling<T> kc{false, (T)0};
-  WetScan ws;
+  OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
@@ -1868,10 +1895,10 @@ __global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* _
     }
     if (skip) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
-    const int lb = it.vb - ic.block0;
-    run_mark(rs, D.partials, lb);
-    const int t = lb * BLOCK + threadIdx.x;
-    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) {
+    run_mark(rs, D.partials, it.vb - ic.block0);
+    const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
+    const int t = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
+    if (e >= 0) {
       // r changes on FLUID only; the first half-sweep's z0 is 0 here
       if (t < D.g.size) stgi(D.P.z[0], t, (T)0);
       continue;
@@ -1942,7 +1969,7 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
   RunSum rs;
   bool done = true;
   Coupling<T> kc{false, (T)0};
-  WetScan ws;
+  OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
     if (item_at(S, it.vb, ic)) {
@@ -1954,10 +1981,10 @@ __global__ void __launch_bounds__(BLOCK) k_smooth0_dot(const SysDesc<T, TX>* __r
     }
     if (done) continue;
     const SysDesc<T, TX>& D = sys[ic.desc];
-    const int lb = it.vb - ic.block0;
-    run_mark(rs, D.partials, lb);
-    if (!wet_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1)) continue;  // z0 and z.r on FLUID only
-    const int t = lb * BLOCK + threadIdx.x;
+    run_mark(rs, D.partials, it.vb - ic.block0);
+    const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
+    if (e >= 0) continue;  // z0 and z.r on FLUID only
+    const int t = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
     DD& acc = rs.acc;
     if (t < D.g.size) {
       if (kc.uniform) smooth0_dot_slot<T, TX, DIM, true>(D, kc, t, acc);
@@ -2358,6 +2385,8 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
       wet = swet[w] > wet ? swet[w] : wet;
     }
     if (P.wet != nullptr) ((uint8_t*)P.wet)[blockIdx.x] = (uint8_t)wet;
+    if (P.ord[0] != nullptr)  // per-block work order: natural, with the wet bit
+      ((int32_t*)P.ord[0])[blockIdx.x] = (int32_t)blockIdx.x | (wet ? ORD_WET : 0);
     // a value that would not move the level's current extreme skips the
     // atomic: with one coupling value on the level (the usual case) only the
     // first workgroups update it, instead of every workgroup queueing on the
@@ -2395,6 +2424,60 @@ __global__ void __launch_bounds__(BLOCK) k_unpack_vec(Geo g, const S* __restrict
     Slot s = decode(g, t, c);
     if (!s.interior) continue;
     dst[canon_index(g, s.i, s.j, s.k)] = (D)(c ? s1 : s0)[t];
+  }
+}
+
+// The half-sweep work order of one level (LevelPtrs::ord[1]) from its
+// per-block FLUID flags: n = ceil(n1 / SLOT_LANES) virtual blocks, one wet
+// when one of its blocks is.  With W wet entries, position k is wet when
+// floor((k+1) W / n) > floor(k W / n): the W wet positions are spread evenly
+// over [0, n).  The m-th wet position (m = floor(k W / n)) gets the m-th wet
+// virtual block in ascending order, the d-th dry position (d = k - m) the
+// d-th dry one, so the order is a permutation.  (A half-sweep stage strides
+// its workgroups over the virtual blocks; in natural order a stride close to
+// a multiple of a plane's length gives some workgroups only wet blocks and
+// others none.)  One workgroup; `tmp` holds n ints.
+constexpr int ORDER_THREADS = 1024;
+__global__ void __launch_bounds__(ORDER_THREADS) k_level_order(const uint8_t* __restrict__ wet,
+                                                               int n1, int32_t* ord,
+                                                               int32_t* tmp) {
+  __shared__ int sc[ORDER_THREADS];
+  const int tid = threadIdx.x;
+  constexpr int span = SLOT_LANES;
+  const int n = (n1 + span - 1) / span;
+  auto flag = [&](int v) {
+    int f = 0;
+    for (int i = 0; i < span; ++i)
+      if (v * span + i < n1) f |= wet[v * span + i] != 0;
+    return f;
+  };
+  // each thread takes a contiguous chunk of entries; the wet entries before
+  // its chunk come from an inclusive scan over the threads' counts
+  const int chunk = (n + ORDER_THREADS - 1) / ORDER_THREADS;
+  const int v0 = min(n, tid * chunk), v1 = min(n, v0 + chunk);
+  int cnt = 0;
+  for (int v = v0; v < v1; ++v) cnt += flag(v);
+  sc[tid] = cnt;
+  __syncthreads();
+  for (int d = 1; d < ORDER_THREADS; d <<= 1) {
+    const int add = tid >= d ? sc[tid - d] : 0;
+    __syncthreads();
+    sc[tid] += add;
+    __syncthreads();
+  }
+  const int W = sc[ORDER_THREADS - 1];
+  int wb = sc[tid] - cnt;  // wet entries before the chunk
+  // stable partition into tmp: wet entries at [0, W), dry ones at [W, n)
+  for (int v = v0; v < v1; ++v) {
+    if (flag(v)) tmp[wb++] = v;
+    else tmp[W + (v - wb)] = v;
+  }
+  __threadfence();
+  __syncthreads();
+  for (int k = tid; k < n; k += ORDER_THREADS) {
+    const int m = (int)((long long)k * W / n);
+    const int m1 = (int)((long long)(k + 1) * W / n);
+    ord[k] = m1 > m ? (tmp[m] | ORD_WET) : tmp[W + (k - m)];
   }
 }
 

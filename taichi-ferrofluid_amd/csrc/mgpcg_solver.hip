@@ -49,6 +49,8 @@ struct LevelBuf {
   void* z[2];
   uint16_t* nb[2];   // flag words (coupling flags, diagonal code)
   uint8_t* wet;      // per block of BLOCK slots: 1 when one of them is FLUID
+  int32_t* ord[2];   // work order per block / per half-sweep virtual block
+  int32_t* ordtmp;   // scratch of the order kernel (one int per block)
   LevelMeta* meta;   // nonzero-coupling summary, written when the level is loaded
   void* res[2];      // restriction scratch (NULL on the coarsest level)
 };
@@ -119,6 +121,8 @@ LevelPtrs<T> level_ptrs(const mgpcg_solver* s, int l) {
     P.res[c] = (T*)B.res[c];
   }
   P.wet = B.wet;
+  P.ord[0] = B.ord[0];
+  P.ord[1] = l == 0 && s->coarse_from > 0 ? B.ord[1] : nullptr;  // level 0 on the batched kernels
   P.meta = (g_compact && !s->full_coupling) ? B.meta : nullptr;
   return P;
 }
@@ -1126,7 +1130,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     return o;
   };
   struct Offs {
-    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta, wet;
+    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta, wet, ord[2], ordtmp;
   };
   std::vector<Offs> lo;
   s->coarse_from = n_levels;
@@ -1148,6 +1152,9 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     }
     o.meta = take(sizeof(LevelMeta));
     o.wet = take(nblk(g.size));
+    o.ord[0] = take(sizeof(int32_t) * nblk(g.size));
+    o.ord[1] = take(sizeof(int32_t) * nsweep(g.size));
+    o.ordtmp = take(sizeof(int32_t) * nsweep(g.size));
     lo.push_back(o);
   }
   const Geo& g0 = s->geo[0];
@@ -1185,6 +1192,9 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     }
     B.meta = (LevelMeta*)(s->pool + lo[l].meta);
     B.wet = (uint8_t*)(s->pool + lo[l].wet);
+    B.ord[0] = (int32_t*)(s->pool + lo[l].ord[0]);
+    B.ord[1] = (int32_t*)(s->pool + lo[l].ord[1]);
+    B.ordtmp = (int32_t*)(s->pool + lo[l].ordtmp);
     s->lv.push_back(B);
     const Geo& g = s->geo[l];
     // ghost cells are SOLID (the zero fill made them FLUID)
@@ -1350,6 +1360,11 @@ extern "C" int mgpcg_load_level(mgpcg_solver_t* s, int level, const int8_t* grid
       hipLaunchKernelGGL((k_pack_level<float, 2>), grid, block, 0, s->user_stream, g,
                          grid_type, Adiag, Ax, level_ptrs<float>(s, level), meta);
   }
+  if (level == 0 && s->coarse_from > 0) {
+    const LevelBuf& B = s->lv[level];
+    hipLaunchKernelGGL(k_level_order, dim3(1), dim3(ORDER_THREADS), 0, s->user_stream, B.wet,
+                       nblk(g.size), B.ord[1], B.ordtmp);
+  }
   MG_HIP(hipGetLastError());
   return 0;
 }
@@ -1365,6 +1380,22 @@ extern "C" int mgpcg_level_flags(mgpcg_solver_t* s, int level, uint16_t* flags, 
     MG_HIP(hipMemcpyAsync(flags + (size_t)c * g.size, B.nb[c], sizeof(uint16_t) * g.size,
                           hipMemcpyDeviceToDevice, s->user_stream));
   MG_HIP(hipMemcpyAsync(wet, B.wet, nblk(g.size), hipMemcpyDeviceToDevice, s->user_stream));
+  MG_HIP(hipStreamSynchronize(s->user_stream));
+  return 0;
+}
+
+extern "C" int mgpcg_level_order(mgpcg_solver_t* s, int level, int32_t* ord1, int32_t* ord2) {
+  MG_REQUIRE(s != nullptr, "solver is NULL");
+  MG_REQUIRE(level >= 0 && level < s->L, "level %d out of range [0, %d)", level, s->L);
+  MG_REQUIRE(ord1 != nullptr, "NULL argument");
+  MG_HIP(hipSetDevice(s->device));
+  const Geo& g = s->geo[level];
+  const LevelBuf& B = s->lv[level];
+  MG_HIP(hipMemcpyAsync(ord1, B.ord[0], sizeof(int32_t) * nblk(g.size), hipMemcpyDeviceToDevice,
+                        s->user_stream));
+  if (ord2 != nullptr && level == 0 && s->coarse_from > 0)
+    MG_HIP(hipMemcpyAsync(ord2, B.ord[1], sizeof(int32_t) * nsweep(g.size),
+                          hipMemcpyDeviceToDevice, s->user_stream));
   MG_HIP(hipStreamSynchronize(s->user_stream));
   return 0;
 }

@@ -216,6 +216,23 @@ class MGPCGPoissonSolver:
         _lib.check(self._lib.mgpcg_level_flags(self._h, level, _lib.ptr(flags), _lib.ptr(wet)))
         return (flags.cpu().numpy().view("uint16").reshape(2, n), wet.cpu().numpy())
 
+    def level_order(self, level):
+        """(ord1, ord2) of a loaded level as int64 numpy arrays (the int32 entries:
+        block index in the low 31 bits, bit 31 set for a block with a FLUID slot)
+        per block of 256 slots and per half-sweep virtual block of 512 slots;
+        ord2 is None except on level 0 when the batched kernels solve it."""
+        inf = self.info(level)
+        n = inf["slots"]
+        n1 = (n + 255) // 256
+        n2 = (n + 511) // 512
+        o1 = torch.empty(n1, dtype=torch.int32, device=self.device)
+        o2 = torch.empty(n2, dtype=torch.int32, device=self.device)
+        _lib.check(self._lib.mgpcg_set_stream(self._h, _lib.stream_handle(self.device)))
+        _lib.check(self._lib.mgpcg_level_order(self._h, level, _lib.ptr(o1), _lib.ptr(o2)))
+        o2 = o2.cpu().numpy().view("uint32").astype("int64")
+        return (o1.cpu().numpy().view("uint32").astype("int64"),
+                o2 if level == 0 and inf["coarse_from"] > 0 else None)
+
     # -- kernel-level access (parity tests) --------------------------------------
     def _vec_dtype(self, field):
         if self.precision == "f64" or (field == _lib.FIELD_X and self.precision != "plain"):

@@ -140,6 +140,8 @@ double shfl_idx(double v, int src, int width) { return shfl_src(g_xd, v, src, wi
 double shfl_xor_d(double v, int mask) { return shfl(g_xd, v, mask); }
 float shfl_xor_f(float v, int mask) { return shfl(g_xf, v, mask); }
 int shfl_xor_i(int v, int mask) { return shfl(g_xi, v, mask); }
+// lane `lane` of the caller's 64-lane wave
+int readlane_i(int v, int lane) { return shfl_src(g_xi, v, lane, 64); }
 // bit l of the result: lane l of the caller's 64-lane wave had pred != 0
 unsigned long long ballot(int pred) {
   const unsigned tid = g_cur->tid;

@@ -26,6 +26,7 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define MG_GLOBAL
+#define MG_READLANE(v, l) (::hipemu::readlane_i((v), (l)))
 #define MG_KEEP(x) ((void)(x))
 #define MG_KEEP4(a, b, c, d) ((void)(a), (void)(b), (void)(c), (void)(d))
 #define MG_KEEP6(a, b, c, d, e, f) ((void)(a), (void)(b), (void)(c), (void)(d), (void)(e), (void)(f))
@@ -79,6 +80,7 @@ float shfl_idx(float v, int src, int width);
 double shfl_idx(double v, int src, int width);
 float shfl_xor_f(float v, int mask);
 unsigned long long ballot(int pred);
+int readlane_i(int v, int lane);
 // run `body(thread)` for every thread of every block of the grid
 void run_grid(dim3 grid, dim3 block, void (*tramp)(void*), void* ctx);
 void* guarded_alloc(size_t bytes);

@@ -8,7 +8,11 @@ writes are recomputed here from the canonical arrays and compared exactly:
 - bits 6..8: the smallest n < 7 for which -c added n times to 0 in FP32
   reproduces Adiag bit for bit, c being the slot's first nonzero coupling
   (upper then lower, axis by axis; n = 0 needs no coupling), else 7;
-- one flag per 256 slots: 1 when a slot of either colour is FLUID.
+- one flag per 256 slots: 1 when a slot of either colour is FLUID;
+- the work orders: per block of 256 slots the natural order with the wet
+  bits; per half-sweep virtual block of 512 slots (level 0, on the batched
+  kernels) a permutation whose wet bits match the flags, with the wet virtual
+  blocks spread evenly over the positions.
 Runs on the MI355X (`dev` = "cuda:0") and on the CPU emulation build.
 """
 import numpy as np
@@ -81,12 +85,38 @@ def _expected(gt, Ad, Ax):
     return flags, wet, fluid
 
 
+def _check_order(o, wetv, what):
+    """o: a work order over len(wetv) entries whose FLUID flags are wetv."""
+    n = len(wetv)
+    blk = o & 0x7FFFFFFF
+    isw = (o >> 31) & 1
+    assert len(o) == n, what
+    assert np.array_equal(np.sort(blk), np.arange(n)), f"{what}: not a permutation"
+    assert np.array_equal(isw, wetv[blk]), f"{what}: wet bit"
+    # wet entries, and dry ones, keep ascending block order
+    assert np.all(np.diff(blk[isw == 1]) > 0) and np.all(np.diff(blk[isw == 0]) > 0), what
+    # evenly spread: every prefix holds its proportional share of wet entries, +-1
+    W = int(wetv.sum())
+    pref = np.concatenate([[0], np.cumsum(isw)])
+    share = np.arange(n + 1) * W / n
+    assert np.all(np.abs(pref - share) < 1.0), f"{what}: wet positions not spread"
+
+
 def _check(s, gts, Ads, Axs):
     for l in range(len(gts)):
         flags, wet = s.level_flags(l)
         ef, ew, _ = _expected(gts[l], Ads[l], Axs[l])
         assert np.array_equal(flags, ef), f"level {l} flag words"
         assert np.array_equal(wet, ew), f"level {l} block flags"
+        o1, o2 = s.level_order(l)
+        assert np.array_equal(o1 & 0x7FFFFFFF, np.arange(len(ew))), f"level {l} block order"
+        assert np.array_equal(o1 >> 31, ew), f"level {l} block order wet bits"
+        if o2 is None:
+            assert l > 0 or s.info(0)["coarse_from"] == 0
+            continue
+        w2 = np.zeros(2 * len(o2), np.int64)
+        w2[:len(ew)] = ew
+        _check_order(o2, w2.reshape(-1, 2).max(axis=1), f"level {l} half-sweep order")
 
 
 @pytest.mark.parametrize("coupling", [None, -0.75], ids=["random", "uniform"])
