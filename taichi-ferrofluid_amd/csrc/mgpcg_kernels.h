@@ -476,6 +476,14 @@ __device__ __forceinline__ bool item_at(const Stage& S, int vb, ItemCache& c) {
   return true;
 }
 
+// Waves per SIMD a kernel asks the compiler to fit (its VGPR budget): 8 (64
+// VGPRs) lets every workgroup of a full persistent launch (8 per CU) be
+// resident at once; FP64 hierarchies would spill there and take the default.
+template <typename T>
+__host__ __device__ constexpr int full_occupancy() {
+  return sizeof(T) == 4 ? 8 : 1;
+}
+
 // Dot-product partials of one workgroup over a run of its virtual blocks in
 // one system: summed in registers and reduced over the workgroup once per run
 // (instead of once per virtual block) into the partial of the run's first
@@ -1741,21 +1749,26 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update_p(const SysDesc<T, TX>* __r
 template <typename T, typename TX, int DIM, bool COMPACT>
 __device__ __forceinline__ void ap_slot(const SysDesc<T, TX>& D, const Coupling<T>& k, int par,
                                         int t, DD& acc) {
+  // both colours' operands are loaded before either is used (p does not
+  // change here), so the two stencils' loads are in flight together
+  StencilVals<T, DIM, COMPACT> s[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    StencilVals<T, DIM, COMPACT> s;
-    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.p[par], c, t, s);
-    stencil_keep<T, DIM, COMPACT>(s);
-    if (s.gt != MG_FLUID) continue;
-    const T ap = stencil_apply<T, DIM, COMPACT>(D.P, k, c, t, s);
+    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.p[par], c, t, s[c]);
+    stencil_keep<T, DIM, COMPACT>(s[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (s[c].gt != MG_FLUID) continue;
+    const T ap = stencil_apply<T, DIM, COMPACT>(D.P, k, c, t, s[c]);
     stgi(D.Ap[c], t, ap);
-    acc = dd_add(acc, dd_prod(s.own, ap));
+    acc = dd_add(acc, dd_prod(s[c].own, ap));
   }
 }
 
 // compute_Ap (mgpcg.py:218-224) on the p written by k_cg_update_p, partial p.Ap.
 template <typename T, typename TX, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __restrict__ sys,
+__global__ void __launch_bounds__(BLOCK, full_occupancy<T>()) k_cg_ap_only(const SysDesc<T, TX>* __restrict__ sys,
                                                       Stage S) {
   ItemCache ic;
   RunSum rs;
@@ -1790,7 +1803,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ap_only(const SysDesc<T, TX>* __re
 // update_xr (mgpcg.py:226-232) + partial r.r.  MIXED: x is FP64 and on a
 // refinement iteration r is left for k_true_residual.
 template <typename T, typename TX>
-__global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restrict__ sys,
+__global__ void __launch_bounds__(BLOCK, full_occupancy<T>()) k_cg_xr(const SysDesc<T, TX>* __restrict__ sys,
                                                  Stage S) {
   ItemCache ic;
   RunSum rs;
@@ -1827,19 +1840,32 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
     }
     DD& acc = rs.acc;
     if (t < D.g.size) {
+      // both colours' operands are loaded before either colour is stored
+      int8_t gts[2];
+      T pvs[2], apvs[2], r0s[2];
+      TX xvs[2];
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        const int8_t gt = ldgi(D.P.gt[c], t);
-        const T pv = ldgi(D.p[par][c], t);
-        const T apv = ldgi(D.Ap[c], t);
-        const TX xv = ldgi(D.x[c], t);
-        const T r0 = ldgi(D.P.r[c], t);
-        uint32_t f0 = 0u;  // the first half-sweep's diagonal operand (colour 0)
-        T ad0 = (T)0;
-        if (c == 0 && !refine) {
-          if (kc.uniform) f0 = ldgi(D.P.nb[0], t);
-          else ad0 = ldgi(D.P.Adiag[0], t);
-        }
+        gts[c] = ldgi(D.P.gt[c], t);
+        pvs[c] = ldgi(D.p[par][c], t);
+        apvs[c] = ldgi(D.Ap[c], t);
+        xvs[c] = ldgi(D.x[c], t);
+        r0s[c] = ldgi(D.P.r[c], t);
+      }
+      uint32_t f0 = 0u;  // the first half-sweep's diagonal operand (colour 0)
+      T ad0 = (T)0;
+      if (!refine) {
+        if (kc.uniform) f0 = ldgi(D.P.nb[0], t);
+        else ad0 = ldgi(D.P.Adiag[0], t);
+      }
+      MG_KEEP4(pvs[1], apvs[1], r0s[1], (int)gts[1]);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int8_t gt = gts[c];
+        const T pv = pvs[c];
+        const T apv = apvs[c];
+        const TX xv = xvs[c];
+        const T r0 = r0s[c];
         if (gt != MG_FLUID) {
           if (c == 0 && !refine) first_z0<T>(D.P, kc, t, false, (T)0, f0, ad0);
           continue;
@@ -1877,7 +1903,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_xr(const SysDesc<T, TX>* __restric
 
 // MIXED: r = b - A x with x in FP64, accumulated in FP64, stored as T; partial r.r.
 template <typename T, typename TX, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_true_residual(const SysDesc<T, TX>* __restrict__ sys,
+__global__ void __launch_bounds__(BLOCK, full_occupancy<T>()) k_true_residual(const SysDesc<T, TX>* __restrict__ sys,
                                                          Stage S) {
   ItemCache ic;
   RunSum rs;
