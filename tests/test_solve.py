@@ -229,7 +229,7 @@ def test_batch_uneven_equals_alone(dev, precision):
     import batch
 
     emu = _emu(dev)
-    count, div, L = (4, 4, 3) if emu else (8, 2, 4)
+    count, div, L = (3, 4, 3) if emu else (8, 2, 4)
     mix = scenes.c4_batch(first_seed=100, count=count, L=L)
     for sysd in mix:  # smaller sides (still divisible by 2^(L-1))
         sysd.update(scenes.pool_sphere_system([n // div for n in sysd["res"]], sysd["seed"], L=L,
@@ -262,7 +262,7 @@ def test_server_continuous_batching_equals_alone(dev, depth, streams):
     set_knobs(server_depth=depth, server_streams=streams)
 
     emu = _emu(dev)
-    nreq = 4 if emu else 6
+    nreq = 3 if emu else 6
     lo, hi = (16, 32) if emu else (32, 64)
     reqs = [scenes.c5_request(1000 + i, lo=lo, hi=hi, L=3) for i in range(nreq)]
     srv = serve.Server(hbm_budget_bytes=4 << 30, precision="mixed", n_levels=3,
