@@ -2057,8 +2057,19 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const SysDesc<T, TX>* __rest
   CGState* st = D.st;
   if (st->done) return;
   if (OP == FIN_TRUE && !st->pending_true) return;
-  DD v = dd_zero();
-  for (int i = threadIdx.x; i < D.nparts; i += BLOCK) v = dd_add(v, D.partials[i]);
+  // four independent accumulators: four partials per thread in flight
+  DD v0 = dd_zero(), v1 = dd_zero(), v2 = dd_zero(), v3 = dd_zero();
+  int i = threadIdx.x;
+  for (; i + 3 * BLOCK < D.nparts; i += 4 * BLOCK) {
+    const DD a = D.partials[i], b = D.partials[i + BLOCK];
+    const DD c = D.partials[i + 2 * BLOCK], d = D.partials[i + 3 * BLOCK];
+    v0 = dd_add(v0, a);
+    v1 = dd_add(v1, b);
+    v2 = dd_add(v2, c);
+    v3 = dd_add(v3, d);
+  }
+  for (; i < D.nparts; i += BLOCK) v0 = dd_add(v0, D.partials[i]);
+  const DD v = dd_add(dd_add(v0, v1), dd_add(v2, v3));
   const double tot = dd_value(block_sum_dd<BLOCK>(v));
   if (threadIdx.x != 0) return;
   if (OP == FIN_INIT) {
