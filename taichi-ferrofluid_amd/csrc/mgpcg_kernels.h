@@ -547,19 +547,29 @@ template <typename T, int DIM, bool COMPACT>
 __device__ __forceinline__ void smooth_load(const Geo& g, const LevelPtrs<T>& P, int c, int t,
                                             SmoothVals<T, DIM, COMPACT>& v) {
   const int q = row_q(g, t, c);
-  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
   v.gt = ldgi(P.gt[c], t);
   v.rv = ldgi(P.r[c], t);
-  if constexpr (!COMPACT) v.ad = ldgi(P.Adiag[c], t);
-  const T* zo = P.z[c ^ 1];
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    v.zl[a] = ldgi(zo, n.lo[a]);
-    v.zh[a] = ldgi(zo, n.hi[a]);
-  }
   if constexpr (COMPACT) {
+    // a z colour array has g.zpad zero slots before and after it, so the
+    // unclamped neighbours of any slot stay inside the allocation: indices
+    // are taken from the start of the front padding (never negative)
+    const Nbr<DIM> n = neighbours<DIM>(g, t, q);
+    const T* zb = P.z[c ^ 1] - g.zpad;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      v.zl[a] = ldgi(zb, n.lo[a] + g.zpad);
+      v.zh[a] = ldgi(zb, n.hi[a] + g.zpad);
+    }
     v.f = ldgi(P.nb[c], t);
   } else {
+    const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
+    v.ad = ldgi(P.Adiag[c], t);
+    const T* zo = P.z[c ^ 1];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      v.zl[a] = ldgi(zo, n.lo[a]);
+      v.zh[a] = ldgi(zo, n.hi[a]);
+    }
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
       v.al[a] = ldgi(P.Ax[a][c ^ 1], n.lo[a]);

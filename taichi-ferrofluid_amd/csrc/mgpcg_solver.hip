@@ -1146,7 +1146,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       o.Ad[c] = take(g.size * s->esz);
       for (int a = 0; a < 3; ++a) o.Ax[a][c] = (a < dim) ? take(g.size * s->esz) : (size_t)-1;
       o.r[c] = take(g.size * s->esz);
-      o.z[c] = take(g.size * s->esz);
+      o.z[c] = take((size_t)(g.size + 2 * g.zpad) * s->esz);  // zero pads both sides
       o.nb[c] = take(g.size * sizeof(uint16_t));
       o.res[c] = (l + 1 < n_levels) ? take(g.size * s->esz) : (size_t)-1;
     }
@@ -1186,7 +1186,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
       for (int a = 0; a < 3; ++a)
         B.Ax[a][c] = (a < dim) ? (void*)(s->pool + lo[l].Ax[a][c]) : nullptr;
       B.r[c] = s->pool + lo[l].r[c];
-      B.z[c] = s->pool + lo[l].z[c];
+      B.z[c] = s->pool + lo[l].z[c] + (size_t)s->geo[l].zpad * s->esz;
       B.nb[c] = (uint16_t*)(s->pool + lo[l].nb[c]);
       B.res[c] = (l + 1 < n_levels) ? (void*)(s->pool + lo[l].res[c]) : nullptr;
     }
