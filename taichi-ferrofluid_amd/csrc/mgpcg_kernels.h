@@ -699,19 +699,23 @@ struct StencilVals {
   T vl[DIM], vh[DIM], al[DIM], ah[DIM];
 };
 
-template <typename T, int DIM, bool COMPACT>
+// ZPAD: v is a level's z (g.zpad zero slots on both sides of each colour
+// array), so on a COMPACT level the neighbours are loaded unclamped
+template <typename T, int DIM, bool COMPACT, bool ZPAD = false>
 __device__ __forceinline__ void stencil_load(const Geo& g, const LevelPtrs<T>& P, const T* const* v,
                                              int c, int t, StencilVals<T, DIM, COMPACT>& s) {
+  constexpr bool UNCLAMPED = COMPACT && ZPAD;
   const int q = row_q(g, t, c);
-  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
+  const Nbr<DIM> n = UNCLAMPED ? neighbours<DIM>(g, t, q) : neighbours_clamped<DIM>(g, t, q);
   s.gt = ldgi(P.gt[c], t);
   s.own = ldgi(v[c], t);
   if constexpr (!COMPACT) s.ad = ldgi(P.Adiag[c], t);
-  const T* vo = v[c ^ 1];
+  const int off = UNCLAMPED ? g.zpad : 0;
+  const T* vo = v[c ^ 1] - off;
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    s.vl[a] = ldgi(vo, n.lo[a]);
-    s.vh[a] = ldgi(vo, n.hi[a]);
+    s.vl[a] = ldgi(vo, n.lo[a] + off);
+    s.vh[a] = ldgi(vo, n.hi[a] + off);
   }
   if constexpr (COMPACT) {
     s.f = ldgi(P.nb[c], t);
@@ -767,7 +771,9 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
                                                        const Coupling<T>& k, const T* zc0,
                                                        const T* zc1, int t) {
   const Slot c = decode(g, t, 1);
-  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, c.q);
+  // z0 has g.zpad zero slots on both sides: unclamped on COMPACT levels
+  const Nbr<DIM> n = COMPACT ? neighbours<DIM>(g, t, c.q) : neighbours_clamped<DIM>(g, t, c.q);
+  const int zoff = COMPACT ? g.zpad : 0;
   const int8_t gt = ldgi(P.gt[1], t);
   const T rv = ldgi(P.r[1], t);
   T ad = (T)0;
@@ -803,8 +809,8 @@ __device__ __forceinline__ void prolong_smooth1_slot_g(const Geo& g, const Geo& 
     else sq = sp - ((pk + 1) >> 1) + ((pk + d + 1) >> 1);
     sq = (inq[a] && sq >= 0 && sq < gc.size) ? sq : sp;
     vq[a] = ldgi(zcq, sq);
-    zl[a] = ldgi(z0, n.lo[a]);
-    zh[a] = ldgi(z0, n.hi[a]);
+    zl[a] = ldgi(z0 - zoff, n.lo[a] + zoff);
+    zh[a] = ldgi(z0 - zoff, n.hi[a] + zoff);
   }
   uint32_t f = 0u;
   if constexpr (COMPACT) {
@@ -1205,7 +1211,7 @@ __device__ __forceinline__ void residual_slot(const LevelDesc<T>& D, const Coupl
 #pragma unroll
   for (int c = 0; c < 2; ++c) {  // one colour after the other
     StencilVals<T, DIM, COMPACT> s;
-    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.P.z, c, t, s);
+    stencil_load<T, DIM, COMPACT, true>(D.g, D.P, D.P.z, c, t, s);
     const T rv = ldgi(D.P.r[c], t);
     stencil_keep<T, DIM, COMPACT>(s);
     T v = (T)0;
@@ -1225,7 +1231,7 @@ __device__ __forceinline__ void residual0_slots(const LevelDesc<T>& D, const Cou
   for (int u = 0; u < SLOT_LANES; ++u) {
     const int t = t0 + u * BLOCK;
     const int tt = t < D.g.size ? t : D.g.size - 1;
-    stencil_load<T, DIM, COMPACT>(D.g, D.P, D.P.z, 0, tt, s[u]);
+    stencil_load<T, DIM, COMPACT, true>(D.g, D.P, D.P.z, 0, tt, s[u]);
     rv[u] = ldgi(D.P.r[0], tt);
   }
 #pragma unroll
