@@ -103,6 +103,10 @@ int mgpcg_get_option(const mgpcg_solver_t* solver, int option, int* value);
 /* Padded layout of a level: out6 = {P0, P1, W, slots per colour, cells,
  * first coarse level}; bytes = device bytes held by the handle. */
 int mgpcg_info(const mgpcg_solver_t* solver, int level, int32_t* out6, int64_t* bytes);
+/* Device bytes mgpcg_create would allocate for this shape and precision (the
+ * handle's pool), without allocating: what an HBM-budget admission charges.
+ * No reference counterpart (the reference allocates Taichi fields). */
+int mgpcg_pool_bytes(int dim, const int32_t* res, int n_levels, int precision, int64_t* bytes);
 
 /* Loads one level of the operator from the canonical fields a strategy's
  * build_A wrote (pressure_project.py:45-64 and siblings) and zeroes that

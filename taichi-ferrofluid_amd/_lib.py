@@ -50,6 +50,7 @@ SIGNATURES = {
     "mgpcg_set_option": (_I, [_P, _I, _I]),
     "mgpcg_get_option": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "mgpcg_info": (_I, [_P, _I, _P, _P]),
+    "mgpcg_pool_bytes": (_I, [_I, _P, _I, _I, _P]),
     "mgpcg_load_level": (_I, [_P, _I, _P, _P, _P]),
     "mgpcg_load_rhs": (_I, [_P, _P]),
     "mgpcg_level_coupling": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float)]),

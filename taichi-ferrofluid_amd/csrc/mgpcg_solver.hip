@@ -95,6 +95,12 @@ struct mgpcg_solver {
   Engine* engine = nullptr;  // private engine of the single-system API
   int device = 0;
   bool full_coupling = false;  // descriptors written now read every Ax
+  // Recorded on an engine's stream when the system retires from it: iterations
+  // queued on that stream before the host saw the system finish still carry it
+  // (they skip it while its state says done).  A later admission, on any
+  // engine, waits for this event before it resets the state.
+  hipEvent_t ev_retired = nullptr;
+  bool retired_pending = false;
 };
 
 namespace {
@@ -477,6 +483,12 @@ int engine_add(Engine* e, mgpcg_solver* s, int64_t tag, const SolveParams& prm) 
   m.tag = tag;
   m.started = e->iteration;
   m.gen += 1;
+  // iterations still queued where the system last ran read its state; the
+  // reset below must come after them
+  if (s->retired_pending) {
+    MG_HIP(hipStreamWaitEvent(e->stream, s->ev_retired, 0));
+    s->retired_pending = false;
+  }
   if (engine_write_desc(e, slot)) return -2;
   if (engine_reset_state(e, slot, prm)) return -2;
   e->dirty = true;
@@ -981,6 +993,11 @@ int engine_collect(Engine* e, int slot, std::vector<Finished>* done) {
     if (M.active && M.gen == g[i].second && st.done) {
       M.active = 0;
       e->dirty = true;
+      // every iteration enqueued so far may still hold the system
+      mgpcg_solver* s = M.s;
+      if (!s->ev_retired) MG_HIP(hipEventCreateWithFlags(&s->ev_retired, hipEventDisableTiming));
+      MG_HIP(hipEventRecord(s->ev_retired, e->stream));
+      s->retired_pending = true;
       if (done) done->push_back({g[i].first, st});
     }
   }
@@ -1082,19 +1099,71 @@ int solve_single(mgpcg_solver* s, int max_iters, double rel_tol, double abs_tol,
   return engine_end(e);
 }
 
-}  // namespace
+// Layout of one system's device pool: every sub-buffer 256-B aligned, in this
+// order per level (both colours of gt, Adiag, Ax per axis, r, z with its zero
+// pads, flag words, residual scratch), then the level's metadata, block flags
+// and work orders; after the levels, level 0's b, x, both p, Ap, the CG state
+// and the reduction partials.
+struct PoolPlan {
+  struct Offs {
+    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta, wet, ord[2], ordtmp;
+  };
+  std::vector<Geo> geo;
+  std::vector<Offs> lo;
+  size_t ob[2], ox[2], op[2][2], oap[2], ost, opart;
+  int nparts;
+  size_t bytes;
+};
 
-// ===========================================================================
-// C-ABI: systems
-// ===========================================================================
-extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, int bottom,
-                            int precision, mgpcg_solver_t** out) {
-  MG_REQUIRE(out != nullptr, "out is NULL");
-  *out = nullptr;
+void plan_pool(int dim, const int32_t* res, int n_levels, size_t esz, size_t xsz, PoolPlan* P) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  P->geo.clear();
+  P->lo.clear();
+  for (int l = 0; l < n_levels; ++l) {
+    int n[3] = {0, 0, 0};
+    for (int a = 0; a < dim; ++a) n[a] = res[a] >> l;
+    P->geo.push_back(make_geo(dim, n));
+    const Geo& g = P->geo.back();
+    PoolPlan::Offs o;
+    for (int c = 0; c < 2; ++c) {
+      o.gt[c] = take(g.size);
+      o.Ad[c] = take(g.size * esz);
+      for (int a = 0; a < 3; ++a) o.Ax[a][c] = (a < dim) ? take(g.size * esz) : (size_t)-1;
+      o.r[c] = take(g.size * esz);
+      o.z[c] = take((size_t)(g.size + 2 * g.zpad) * esz);  // zero pads both sides
+      o.nb[c] = take(g.size * sizeof(uint16_t));
+      o.res[c] = (l + 1 < n_levels) ? take(g.size * esz) : (size_t)-1;
+    }
+    o.meta = take(sizeof(LevelMeta));
+    o.wet = take(nblk(g.size));
+    o.ord[0] = take(sizeof(int32_t) * nblk(g.size));
+    o.ord[1] = take(sizeof(int32_t) * nsweep(g.size));
+    o.ordtmp = take(sizeof(int32_t) * nsweep(g.size));
+    P->lo.push_back(o);
+  }
+  const Geo& g0 = P->geo[0];
+  for (int c = 0; c < 2; ++c) {
+    P->ob[c] = take(g0.size * esz);
+    P->ox[c] = take(g0.size * xsz);
+    P->op[0][c] = take(g0.size * esz);
+    P->op[1][c] = take(g0.size * esz);
+    P->oap[c] = take(g0.size * esz);
+  }
+  P->ost = take(sizeof(CGState));
+  P->nparts = nblk(g0.size);
+  P->opart = take(sizeof(DD) * P->nparts);
+  P->bytes = off;
+}
+
+int check_shape(int dim, const int32_t* res, int n_levels, int precision) {
   MG_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3 (got %d)", dim);
   MG_REQUIRE(res != nullptr, "res is NULL");
   MG_REQUIRE(n_levels >= 1 && n_levels <= 16, "n_levels must be in [1, 16] (got %d)", n_levels);
-  MG_REQUIRE(nu >= 0 && bottom >= 0, "smoothing counts must be >= 0");
   MG_REQUIRE(precision >= 0 && precision <= 2, "precision must be 0, 1 or 2");
   for (int a = 0; a < dim; ++a) {
     MG_REQUIRE(res[a] >= 1, "res[%d] = %d must be positive", a, res[a]);
@@ -1105,6 +1174,20 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
   long long cells = 1;
   for (int a = 0; a < dim; ++a) cells *= res[a];
   MG_REQUIRE(cells < (1ll << 29), "grid too large for 32-bit slot indexing");
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI: systems
+// ===========================================================================
+extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, int bottom,
+                            int precision, mgpcg_solver_t** out) {
+  MG_REQUIRE(out != nullptr, "out is NULL");
+  *out = nullptr;
+  if (check_shape(dim, res, n_levels, precision)) return -1;
+  MG_REQUIRE(nu >= 0 && bottom >= 0, "smoothing counts must be >= 0");
 
   mgpcg_solver* s = new (std::nothrow) mgpcg_solver();
   MG_REQUIRE(s != nullptr, "out of host memory");
@@ -1123,53 +1206,24 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     return -2;
   }
 
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off = align_up(off + bytes, 256);
-    return o;
-  };
-  struct Offs {
-    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta, wet, ord[2], ordtmp;
-  };
-  std::vector<Offs> lo;
+  PoolPlan P;
+  plan_pool(dim, res, n_levels, s->esz, s->xsz, &P);
+  s->geo = P.geo;
   s->coarse_from = n_levels;
-  for (int l = 0; l < n_levels; ++l) {
-    int n[3] = {0, 0, 0};
-    for (int a = 0; a < dim; ++a) n[a] = res[a] >> l;
-    s->geo.push_back(make_geo(dim, n));
-    const Geo& g = s->geo.back();
-    if (s->coarse_from == n_levels && g.size <= g_coarse_max_slots) s->coarse_from = l;
-    Offs o;
-    for (int c = 0; c < 2; ++c) {
-      o.gt[c] = take(g.size);
-      o.Ad[c] = take(g.size * s->esz);
-      for (int a = 0; a < 3; ++a) o.Ax[a][c] = (a < dim) ? take(g.size * s->esz) : (size_t)-1;
-      o.r[c] = take(g.size * s->esz);
-      o.z[c] = take((size_t)(g.size + 2 * g.zpad) * s->esz);  // zero pads both sides
-      o.nb[c] = take(g.size * sizeof(uint16_t));
-      o.res[c] = (l + 1 < n_levels) ? take(g.size * s->esz) : (size_t)-1;
+  for (int l = 0; l < n_levels; ++l)
+    if (s->geo[l].size <= g_coarse_max_slots) {
+      s->coarse_from = l;
+      break;
     }
-    o.meta = take(sizeof(LevelMeta));
-    o.wet = take(nblk(g.size));
-    o.ord[0] = take(sizeof(int32_t) * nblk(g.size));
-    o.ord[1] = take(sizeof(int32_t) * nsweep(g.size));
-    o.ordtmp = take(sizeof(int32_t) * nsweep(g.size));
-    lo.push_back(o);
-  }
-  const Geo& g0 = s->geo[0];
-  size_t ob[2], ox[2], op[2][2], oap[2];
-  for (int c = 0; c < 2; ++c) {
-    ob[c] = take(g0.size * s->esz);
-    ox[c] = take(g0.size * s->xsz);
-    op[0][c] = take(g0.size * s->esz);
-    op[1][c] = take(g0.size * s->esz);
-    oap[c] = take(g0.size * s->esz);
-  }
-  size_t ost = take(sizeof(CGState));
-  s->nparts = nblk(g0.size);
-  size_t opart = take(sizeof(DD) * s->nparts);
-  s->pool_bytes = off;
+  const std::vector<PoolPlan::Offs>& lo = P.lo;
+  const size_t* ob = P.ob;
+  const size_t* ox = P.ox;
+  const size_t(*op)[2] = P.op;
+  const size_t* oap = P.oap;
+  const size_t ost = P.ost;
+  const size_t opart = P.opart;
+  s->nparts = P.nparts;
+  s->pool_bytes = P.bytes;
 
   auto fail = [&](hipError_t err, const char* what) {
     mg_set_error("%s failed: %s", what, hipGetErrorString(err));
@@ -1215,6 +1269,16 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
   return 0;
 }
 
+extern "C" int mgpcg_pool_bytes(int dim, const int32_t* res, int n_levels, int precision,
+                                int64_t* bytes) {
+  MG_REQUIRE(bytes != nullptr, "bytes is NULL");
+  if (check_shape(dim, res, n_levels, precision)) return -1;
+  PoolPlan P;
+  plan_pool(dim, res, n_levels, precision == F64 ? 8 : 4, precision == PLAIN ? 4 : 8, &P);
+  *bytes = (int64_t)P.bytes;
+  return 0;
+}
+
 extern "C" int mgpcg_destroy(mgpcg_solver_t* s) {
   if (!s) return 0;
   if (s->engine) {
@@ -1225,6 +1289,7 @@ extern "C" int mgpcg_destroy(mgpcg_solver_t* s) {
     (void)hipDeviceSynchronize();
     (void)hipFree(s->pool);
   }
+  if (s->ev_retired) (void)hipEventDestroy(s->ev_retired);
   delete s;
   return 0;
 }

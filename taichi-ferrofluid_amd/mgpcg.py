@@ -27,6 +27,33 @@ def level_shape(res, l):
     return [n // (2 ** l) for n in res]
 
 
+def pool_bytes(res, n_levels, precision="plain"):
+    """Device bytes of the library's pool for one hierarchy of this shape,
+    computed by the library from the layout it allocates (mgpcg_pool_bytes)."""
+    n = ctypes.c_int64()
+    _lib.check(_lib.load().mgpcg_pool_bytes(len(res), _lib.int3(list(res)), n_levels,
+                                            PRECISIONS[precision], ctypes.byref(n)))
+    return n.value
+
+
+def resident_bytes(res, n_levels, precision="plain"):
+    """Device bytes one MGPCGPoissonSolver holds: the library's pool plus the
+    canonical fields of this class (b, x, x64 when not plain, and per level
+    grid_type, Adiag and Ax)."""
+    dim = len(res)
+    canon = 0
+    for l in range(n_levels):
+        n = 1
+        for r in level_shape(res, l):
+            n *= r
+        canon += n * (1 + 4 + 4 * dim)
+    n0 = 1
+    for r in res:
+        n0 *= r
+    canon += n0 * (4 + 4 + (8 if precision != "plain" else 0))
+    return pool_bytes(res, n_levels, precision) + canon
+
+
 class MGPCGPoissonSolver:
     """mgpcg.py:5-44.
 

@@ -167,21 +167,3 @@ def c5_request(seed, lo=64, hi=256, L=5):
         side = math.exp(rng.uniform(math.log(lo), math.log(hi)))
         res.append(int(max(lo, min(hi, round(side / 16) * 16))))
     return pool_sphere_system(res, seed, L=L, kind="pressure")
-
-
-def system_bytes(res, L, precision="plain"):
-    """Device bytes of one resident hierarchy (colour-split, padded)."""
-    esz = 8 if precision == "f64" else 4
-    xsz = 4 if precision == "plain" else 8
-    dim = len(res)
-    tot = 0
-    for l in range(L):
-        n = [r >> l for r in res]
-        if dim == 3:
-            slots = (n[0] + 2) * (n[1] + 2) * ((n[2] + 3) // 2)
-        else:
-            slots = (n[0] + 2) * ((n[1] + 3) // 2)
-        tot += 2 * slots * (1 + esz * (3 + dim))
-        if l == 0:
-            tot += 2 * slots * (3 * esz + xsz)
-    return tot

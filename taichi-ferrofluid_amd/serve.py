@@ -67,18 +67,7 @@ class Server:
 
     # -- resident hierarchies -----------------------------------------------------
     def _solver_bytes(self, res):
-        dim = len(res)
-        canon = 0
-        for l in range(self.L):
-            n = 1
-            for r in res:
-                n *= r >> l
-            canon += n * (1 + 4 + 4 * dim)
-        n0 = 1
-        for r in res:
-            n0 *= r
-        canon += n0 * (4 + 4 + (8 if self.precision != "plain" else 0))
-        return scenes.system_bytes(res, self.L, self.precision) + canon
+        return mgpcg.resident_bytes(list(res), self.L, self.precision)
 
     def _acquire(self, res):
         key = tuple(res)

@@ -28,11 +28,10 @@ def dev(request):
     if request.param == "emu":
         if _lib._lib is not None and _lib._path != EMU_LIB:
             pytest.skip("libmgpcg.so already loaded in this process")
-        if not os.path.exists(EMU_LIB):
-            sys.path.insert(0, os.path.join(ROOT, "tests", "cpu_emu"))
-            import build_emu
+        sys.path.insert(0, os.path.join(ROOT, "tests", "cpu_emu"))
+        import build_emu
 
-            build_emu.build()
+        build_emu.build_if_stale()
         _lib.use_library(EMU_LIB)
         return "cpu"
     if _lib._lib is not None and _lib._path == EMU_LIB:

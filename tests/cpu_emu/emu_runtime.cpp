@@ -1,6 +1,7 @@
 // CPU emulation runtime of the HIP subset libmgpcg uses -- TEST INFRASTRUCTURE ONLY.
 //
-// Kernel launches run synchronously.  Each block's threads are ucontext
+// Kernel launches run in stream order (at once, or queued when deferred
+// streams are on; see below).  Each block's threads are ucontext
 // coroutines on one OS thread: a thread runs until it reaches __syncthreads
 // (or a shuffle, which is two barriers) or returns; the barrier opens when
 // every live thread of the block waits at it.  "Device" memory comes from
@@ -13,7 +14,9 @@
 #include <ucontext.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
+#include <deque>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -205,11 +208,29 @@ void guarded_free(void* p) {
 
 }  // namespace hipemu
 
-struct ihipStream_t {
-  int dummy;
-};
+// Streams.  By default every operation runs when it is issued (each stream is
+// trivially in order).  With deferred streams on (hipemu_set_deferred), work
+// issued to a stream the library created is queued and runs only when the
+// host waits (stream/event/device synchronization, blocking copies, frees).
+// The scheduler then either always advances the most recently created stream
+// whose next operation can run, or takes one operation of each stream in turn:
+// orders that are legal but as far from issue order as the events allow, so
+// a missing event dependency between streams shows up as a wrong result.  The NULL stream (the caller's stream
+// on the CPU) always runs at once; waiting on an event from it first runs the
+// queues until the event has passed.
 struct ihipEvent_t {
   std::chrono::steady_clock::time_point t;
+  unsigned long long recorded = 0;   // records issued
+  unsigned long long completed = 0;  // records the stream order has passed
+};
+struct ihipStream_t {
+  struct Op {
+    int kind;  // 0 run, 1 signal event, 2 wait for event
+    std::function<void()> fn;
+    ihipEvent_t* ev;
+    unsigned long long gen;
+  };
+  std::deque<Op> q;
 };
 struct ihipGraph_t {
   std::vector<std::function<void()>> ops;
@@ -222,10 +243,77 @@ namespace hipemu {
 namespace {
 ihipGraph_t* g_capture = nullptr;  // the graph being captured (one at a time)
 hipStream_t g_capture_stream = nullptr;
+bool g_defer = false;
+int g_defer_mode = 1;  // 1: newest stream first, 2: one operation per stream in turn
+size_t g_rr = 0;
+std::vector<ihipStream_t*> g_streams;  // in creation order
+
+bool deferred(hipStream_t s) { return g_defer && s != nullptr; }
+
+// Advance the queues until pred() holds (every queue drained when pred is
+// empty).  Aborts when nothing can run and pred does not hold.
+void run_until(const std::function<bool()>& pred) {
+  for (;;) {
+    if (pred && pred()) return;
+    ihipStream_t* pick = nullptr;
+    const size_t ns = g_streams.size();
+    for (size_t k = 0; k < ns; ++k) {
+      // newest stream first; round robin: the stream after the last one run
+      const size_t i = g_defer_mode == 2 ? (g_rr + 1 + k) % ns : ns - 1 - k;
+      ihipStream_t* st = g_streams[i];
+      if (st->q.empty()) continue;
+      const ihipStream_t::Op& op = st->q.front();
+      if (op.kind == 2 && op.ev->completed < op.gen) continue;
+      pick = st;
+      g_rr = i;
+      break;
+    }
+    if (!pick) {
+      if (!pred) return;
+      fprintf(stderr, "hipemu: deadlock: a wait can never be satisfied\n");
+      abort();
+    }
+    ihipStream_t::Op op = std::move(pick->q.front());
+    pick->q.pop_front();
+    if (getenv("HIPEMU_TRACE")) {
+      int idx = 0;
+      for (size_t i = 0; i < g_streams.size(); ++i) if (g_streams[i] == pick) idx = (int)i;
+      fprintf(stderr, "TRACE stream %d kind %d\n", idx, op.kind);
+    }
+    if (op.kind == 0) {
+      op.fn();
+    } else if (op.kind == 1) {
+      op.ev->t = std::chrono::steady_clock::now();
+      if (op.ev->completed < op.gen) op.ev->completed = op.gen;
+    }
+  }
+}
+void drain_all() { run_until(nullptr); }
 }  // namespace
 bool capturing() { return g_capture != nullptr; }
 void record(std::function<void()> op) { g_capture->ops.push_back(std::move(op)); }
+void submit(hipStream_t s, std::function<void()> op) {
+  if (deferred(s))
+    s->q.push_back({0, std::move(op), nullptr, 0});
+  else
+    op();
+}
 }  // namespace hipemu
+
+// operations queued on the library's streams (deferred streams only)
+extern "C" long hipemu_queued(void) {
+  long n = 0;
+  for (ihipStream_t* st : hipemu::g_streams) n += (long)st->q.size();
+  return n;
+}
+
+// 0: off; 1: run the newest stream with runnable work first; 2: round robin,
+// one operation of each stream in turn (the finest interleaving of streams)
+extern "C" void hipemu_set_deferred(int mode) {
+  hipemu::drain_all();
+  hipemu::g_defer = mode != 0;
+  hipemu::g_defer_mode = mode;
+}
 
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
   if (hipemu::g_capture) return hipErrorInvalidValue;
@@ -245,9 +333,12 @@ hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t g, hipGraphNode_t*,
   *x = new ihipGraphExec_t{g->ops};
   return hipSuccess;
 }
-hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t) {
+hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s) {
   if (hipemu::capturing()) return hipErrorInvalidValue;
-  for (auto& op : x->ops) op();
+  std::vector<std::function<void()>> ops = x->ops;
+  hipemu::submit(s, [ops]() {
+    for (auto& op : ops) op();
+  });
   return hipSuccess;
 }
 hipError_t hipGraphExecDestroy(hipGraphExec_t x) {
@@ -289,73 +380,108 @@ hipError_t hipGetDevice(int* d) {
   return hipSuccess;
 }
 hipError_t hipSetDevice(int) { return hipSuccess; }
-hipError_t hipDeviceSynchronize(void) { return hipSuccess; }
+hipError_t hipDeviceSynchronize(void) {
+  hipemu::drain_all();
+  return hipSuccess;
+}
 hipError_t hipMalloc(void** p, size_t bytes) {
   *p = hipemu::guarded_alloc(bytes);
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 hipError_t hipFree(void* p) {
+  hipemu::drain_all();
   hipemu::guarded_free(p);
   return hipSuccess;
 }
 hipError_t hipMemset(void* p, int v, size_t bytes) {
-  memset(p, v, bytes);
+  memset(p, v, bytes);  // NULL stream: not ordered with non-blocking streams
   return hipSuccess;
 }
-hipError_t hipMemsetAsync(void* p, int v, size_t bytes, hipStream_t) {
+hipError_t hipMemsetAsync(void* p, int v, size_t bytes, hipStream_t s) {
   if (hipemu::capturing()) {
     hipemu::record([p, v, bytes]() { memset(p, v, bytes); });
     return hipSuccess;
   }
-  memset(p, v, bytes);
+  hipemu::submit(s, [p, v, bytes]() { memset(p, v, bytes); });
   return hipSuccess;
 }
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind) {
-  memmove(dst, src, bytes);
+  memmove(dst, src, bytes);  // NULL stream: not ordered with non-blocking streams
   return hipSuccess;
 }
-hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind k,
-                          hipStream_t) {
+hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind,
+                          hipStream_t s) {
   if (hipemu::capturing()) {
     hipemu::record([dst, src, bytes]() { memmove(dst, src, bytes); });
     return hipSuccess;
   }
-  return hipMemcpy(dst, src, bytes, k);
+  hipemu::submit(s, [dst, src, bytes]() { memmove(dst, src, bytes); });
+  return hipSuccess;
 }
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
   *p = hipemu::guarded_alloc(bytes);
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 hipError_t hipHostFree(void* p) {
+  hipemu::drain_all();
   hipemu::guarded_free(p);
   return hipSuccess;
 }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
   *s = new ihipStream_t();
+  hipemu::g_streams.push_back(*s);
   return hipSuccess;
 }
 hipError_t hipStreamDestroy(hipStream_t s) {
+  hipemu::run_until([s]() { return s->q.empty(); });
+  auto& v = hipemu::g_streams;
+  v.erase(std::remove(v.begin(), v.end(), s), v.end());
   delete s;
   return hipSuccess;
 }
-hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
-hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t s) {
+  // the NULL stream's work has run already
+  if (s != nullptr) hipemu::run_until([s]() { return s->q.empty(); });
+  return hipSuccess;
+}
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned) {
+  const unsigned long long gen = e->recorded;
+  if (e->completed >= gen) return hipSuccess;
+  if (hipemu::deferred(s))
+    s->q.push_back({2, nullptr, e, gen});
+  else
+    hipemu::run_until([e, gen]() { return e->completed >= gen; });
+  return hipSuccess;
+}
 hipError_t hipEventCreate(hipEvent_t* e) {
   *e = new ihipEvent_t();
   return hipSuccess;
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
-hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
-  e->t = std::chrono::steady_clock::now();
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
+  const unsigned long long gen = ++e->recorded;
+  if (hipemu::deferred(s)) {
+    s->q.push_back({1, nullptr, e, gen});
+  } else {
+    e->t = std::chrono::steady_clock::now();
+    e->completed = gen;
+  }
   return hipSuccess;
 }
-hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t e) {
+  const unsigned long long gen = e->recorded;
+  hipemu::run_until([e, gen]() { return e->completed >= gen; });
+  return hipSuccess;
+}
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
   *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
   return hipSuccess;
 }
 hipError_t hipEventDestroy(hipEvent_t e) {
+  hipemu::drain_all();  // queued signals and waits may still name it
   delete e;
   return hipSuccess;
 }
-hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventQuery(hipEvent_t e) {
+  return e->completed >= e->recorded ? hipSuccess : hipErrorNotReady;
+}

@@ -1,5 +1,5 @@
-// CPU emulation of hipExtLaunchKernelGGL: the events are recorded around a
-// synchronous launch.
+// CPU emulation of hipExtLaunchKernelGGL: the events are recorded around the
+// launch, in stream order.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -10,6 +10,6 @@ inline void hipExtLaunchKernelGGL(void (*kernel)(KArgs...), const dim3& grid, co
   (void)shmem;
   (void)flags;
   if (start) hipEventRecord(start, stream);
-  ::hipemu::launch(kernel, grid, block, std::forward<Args>(args)...);
+  ::hipemu::launch(stream, kernel, grid, block, std::forward<Args>(args)...);
   if (stop) hipEventRecord(stop, stream);
 }

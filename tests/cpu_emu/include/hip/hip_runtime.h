@@ -89,6 +89,8 @@ void guarded_free(void* p);
 // recorded into the graph instead of running
 bool capturing();
 void record(std::function<void()> op);
+// run `op` in stream order on `s` (at once, unless deferred streams are on)
+void submit(hipStream_t s, std::function<void()> op);
 }  // namespace hipemu
 
 #define threadIdx (::hipemu::threadIdx)
@@ -199,7 +201,7 @@ hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s);
 hipError_t hipGraphExecDestroy(hipGraphExec_t x);
 hipError_t hipGraphDestroy(hipGraph_t g);
 
-// kernel launch: synchronous, every block's threads as coroutines
+// kernel launch: every block's threads as coroutines, in stream order
 namespace hipemu {
 template <typename F, typename Tup>
 struct LaunchCtx {
@@ -212,7 +214,7 @@ void tramp(void* p) {
   std::apply(c->f, c->args);
 }
 template <typename... KArgs, typename... Args>
-void launch(void (*kernel)(KArgs...), dim3 grid, dim3 block, Args&&... args) {
+void launch(hipStream_t stream, void (*kernel)(KArgs...), dim3 grid, dim3 block, Args&&... args) {
   typedef std::tuple<typename std::decay<KArgs>::type...> Tup;
   typedef LaunchCtx<void (*)(KArgs...), Tup> Ctx;
   auto ctx = std::make_shared<Ctx>(Ctx{kernel, Tup(std::forward<Args>(args)...)});
@@ -222,9 +224,9 @@ void launch(void (*kernel)(KArgs...), dim3 grid, dim3 block, Args&&... args) {
   if (capturing())
     record(run);
   else
-    run();
+    submit(stream, run);
 }
 }  // namespace hipemu
 
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
-  ::hipemu::launch(kernel, dim3(grid), dim3(block), ##__VA_ARGS__)
+  ::hipemu::launch((stream), kernel, dim3(grid), dim3(block), ##__VA_ARGS__)

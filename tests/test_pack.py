@@ -156,3 +156,21 @@ def test_flags_of_pressure_scene(dev):
     assert fluid.any() and wet.any()
     # FLUID cells away from the free surface have plain-sum diagonals
     assert (codes[fluid] < 7).mean() > 0.5
+
+
+@pytest.mark.parametrize("precision", ["plain", "mixed", "f64"])
+def test_pool_bytes_is_what_create_allocates(dev, precision):
+    """The admission charge comes from the library's own layout: the size
+    query equals the bytes the created handle holds, for 2D and 3D shapes."""
+    import mgpcg
+
+    for res, L in (([16, 16, 32], 3), ([32, 48, 16], 4), ([64, 32], 4), ([16, 16], 1)):
+        q = mgpcg.pool_bytes(res, L, precision)
+        s = mgpcg.MGPCGPoissonSolver(len(res), res, L, 2, 10, precision=precision, device=dev)
+        assert s.info(0)["device_bytes"] == q, (res, L)
+        canon = sum(t.numel() * t.element_size()
+                    for t in [s.b, s.x] + s.grid_type + s.Adiag + s.Ax)
+        if precision != "plain":
+            canon += s.x.numel() * 8  # x64, allocated at the first readback
+        assert mgpcg.resident_bytes(res, L, precision) == q + canon
+        del s

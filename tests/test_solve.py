@@ -305,3 +305,74 @@ def test_server_continuous_batching_equals_alone(dev, depth, streams):
         xa, ra, _ = gpu_solve(sysd, dev, "mixed")
         assert ra["iters"] == rq["result"]["iters"]
         assert np.array_equal(xa, rq["x"])
+
+
+@pytest.fixture(params=[1, 2], ids=["newest_first", "round_robin"])
+def deferred_streams(dev, request):
+    """On the CPU emulation: the library's streams queue their work and run it
+    only when the host waits, either newest stream first or one operation of
+    each stream in turn (legal orders, as far from issue order as the events
+    allow), so a missing event between streams changes the results.  On the
+    GPU: the streams as they are (the second parameter repeats the test)."""
+    import _lib
+
+    lib = _lib.load()
+    if _emu(dev):
+        lib.hipemu_set_deferred(request.param)
+    yield
+    if _emu(dev):
+        lib.hipemu_set_deferred(0)
+
+
+def test_server_resubmit_on_other_lane_equals_alone(dev, deferred_streams):
+    """A system that retires from one lane while that lane still has
+    iterations queued (server depth 2) and is resubmitted at once lands on the
+    other lane; its second solve must still equal solving it alone.  The
+    queued iterations of the first lane read the system's state, so the second
+    lane's reset of that state must wait for them (the retirement event)."""
+    import serve
+    import systems
+
+    set_knobs(server_depth=2, server_streams=2)
+    try:
+        emu = _emu(dev)
+        f = 1 if emu else 2
+        # A: 7 iterations, B: 1, C: ~36 (oracle, MIXED); shapes differ, so the
+        # server's idle pool hands A's own hierarchy back for its resubmission
+        a = scenes.pool_sphere_system([16 * f, 16 * f, 16 * f], 1, L=3, kind="pressure")
+        b = scenes.pool_sphere_system([16 * f, 16 * f, 32 * f], 1, L=3, kind="surface")
+        c = scenes.pool_sphere_system([16 * f, 32 * f, 16 * f], 1, L=3, kind="psi")
+        da, db, dc = (systems.upload(x, dev) for x in (a, b, c))
+        srv = serve.Server(hbm_budget_bytes=4 << 30, precision="mixed", n_levels=3, device=dev)
+        for d, m in ((da, "a"), (db, "b"), (dc, "c")):  # lanes: a -> 0, b -> 1, c -> 0
+            srv.submit(d, meta=m)
+        first_a = None
+        results = {}
+        lanes_checked = False
+        steps = 0
+        while srv.busy():
+            steps += 1
+            assert steps < 2000
+            for rq in srv.step():
+                m = rq["meta"]
+                rq["x"] = _x(rq["solver"])
+                if m == "a" and first_a is None:
+                    first_a = rq["solver"]
+                    running = sorted(r["meta"] for _, r in srv.running.values())
+                    # c still runs on lane 0 and lane 1 is empty: the
+                    # resubmission goes to lane 1
+                    lanes_checked = running == ["c"] and "b" in results
+                    srv.retire(rq)
+                    srv.submit(da, meta="a2")
+                    continue
+                results[m] = rq
+                srv.retire(rq)
+        assert lanes_checked, "the resubmission did not take the other lane"
+        assert results["a2"]["solver"] is first_a if "solver" in results["a2"] else True
+        for m, sysd in (("a2", a), ("b", b), ("c", c)):
+            print(f"case {m} {sysd['name']}", flush=True)
+            xa, ra, _ = gpu_solve(sysd, dev, "mixed")
+            assert ra["iters"] == results[m]["result"]["iters"], m
+            assert np.array_equal(xa, results[m]["x"]), m
+    finally:
+        set_knobs(server_depth=2, server_streams=2)
