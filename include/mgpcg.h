@@ -208,17 +208,19 @@ int mgpcg_server_pending(const mgpcg_server_t* server, int* n_pending, int* n_ac
                          int* inflight);
 
 /* Measurement (no reference counterpart): enable = 1 starts timing one in
- * MGPCG_OPT_PROBE_EVERY launches of the level-0 red-black half-sweep (the
- * dominant kernel, mg::k_stage<..., 0, 1>) on the device: a timed launch
+ * MGPCG_OPT_PROBE_EVERY launches of the level-0 red-black half-sweep
+ * smooth(0, 0) on the device; a timed launch runs as its own instantiation,
+ * mg::k_stage<..., 0, 2> (the others are mg::k_stage<..., 0, 1>), so a
+ * profiler's statistics of that name cover exactly the timed launches, and
  * records the device clock when its first workgroup starts and when its last
  * one ends.  enable = 0 stops and returns the number of timed launches, their
- * summed device time and the level-0 cells those launches swept (counted on
- * the device: systems already finished are not swept).  A server sums over
- * its engine streams. */
+ * summed device time, and the level-0 cells and FLUID cells those launches
+ * swept (counted on the device: systems already finished are not swept).  A
+ * server sums over its engine streams.  Output pointers may be NULL. */
 int mgpcg_server_probe(mgpcg_server_t* server, int enable, int* n_launches, double* total_ms,
-                       int64_t* total_cells);
+                       int64_t* total_cells, int64_t* total_fluid);
 int mgpcg_batch_probe(mgpcg_batch_t* batch, int enable, int* n_launches, double* total_ms,
-                      int64_t* total_cells);
+                      int64_t* total_cells, int64_t* total_fluid);
 /* Measurement (no reference counterpart): iteration graphs captured and
  * iterations launched from a graph so far by a solver's own engine, a batch
  * or a server (summed over its engine streams).  Exactly one handle is given;

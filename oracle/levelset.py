@@ -92,20 +92,25 @@ def _eikonal(nbs, dx, dim):
     d = (nb[0] + dxf).astype(f32)
     c1 = d > nb[1]
     df = (nb[1] - nb[0]).astype(f32)
-    with np.errstate(invalid="ignore"):
+    # The two- and three-neighbour forms are evaluated on every entry and
+    # selected where they apply; where they do not, their operands may be the
+    # 1e20 "unknown" value, whose squares overflow.  Those entries are
+    # discarded, and the selected result is checked to be finite.
+    with np.errstate(over="ignore", invalid="ignore"):
         d2 = (half * ((nb[0] + nb[1]).astype(f32) +
                       np.sqrt((two_dx2 - (df * df).astype(f32)).astype(f32)))).astype(f32)
-    d = np.where(c1, d2, d)
-    if dim == 3:
-        c2 = c1 & (d > nb[2])
-        s = ((nb[0] + nb[1]).astype(f32) + nb[2]).astype(f32)
-        q = (((nb[0] * nb[0]).astype(f32) + (nb[1] * nb[1]).astype(f32)).astype(f32) +
-             (nb[2] * nb[2]).astype(f32)).astype(f32)
-        q = (q - dx2).astype(f32)
-        inner = np.maximum(f32(0.0), ((s * s).astype(f32) - (f32(3.0) * q).astype(f32))
-                           .astype(f32))
-        d3 = (third * (s + np.sqrt(inner).astype(f32)).astype(f32)).astype(f32)
-        d = np.where(c2, d3, d)
+        d = np.where(c1, d2, d)
+        if dim == 3:
+            c2 = c1 & (d > nb[2])
+            s = ((nb[0] + nb[1]).astype(f32) + nb[2]).astype(f32)
+            q = (((nb[0] * nb[0]).astype(f32) + (nb[1] * nb[1]).astype(f32)).astype(f32) +
+                 (nb[2] * nb[2]).astype(f32)).astype(f32)
+            q = (q - dx2).astype(f32)
+            inner = np.maximum(f32(0.0), ((s * s).astype(f32) - (f32(3.0) * q).astype(f32))
+                               .astype(f32))
+            d3 = (third * (s + np.sqrt(inner).astype(f32)).astype(f32)).astype(f32)
+            d = np.where(c2, d3, d)
+    assert np.all(np.isfinite(d)), "Eikonal update produced a non-finite distance"
     return d.astype(f32)
 
 

@@ -97,7 +97,8 @@ struct Geo {
   int P0, P1, W;   // padded dims of one colour array
   int sx;          // axis-0 stride (P1 * W)
   int size;        // P0 * P1 * W
-  int zpad;        // zero slots before and after each z colour array (>= sx, multiple of 64)
+  int zpad;        // zero slots before and after each z colour array (>= the largest
+                   // neighbour offset: sx in 3D, W in 2D; a multiple of 64)
   int ncells;      // n0 * n1 * n2
   FastDiv divW, divP1;
 };
@@ -120,7 +121,7 @@ inline Geo make_geo(int dim, const int* n) {
   g.W = (g.n2 + 3) / 2;
   g.sx = g.P1 * g.W;
   g.size = g.P0 * g.P1 * g.W;
-  g.zpad = (g.sx + 63) / 64 * 64;
+  g.zpad = ((dim == 3 ? g.sx : g.W) + 63) / 64 * 64;
   g.ncells = g.n0 * g.n1 * g.n2;
   g.divW = FastDiv((uint32_t)g.W);
   g.divP1 = FastDiv((uint32_t)g.P1);

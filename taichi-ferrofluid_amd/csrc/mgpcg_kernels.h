@@ -50,6 +50,7 @@ static_assert(SLOT_LANES >= 1 && SLOT_LANES <= 8, "slots per lane");
 // Ax values (minbits = ~0, maxbits = 0 when there is none).
 struct LevelMeta {
   uint32_t minbits, maxbits;
+  uint32_t nfluid;  // FLUID cells of the level (the launch probe's byte model counts them)
 };
 
 // flag word of a slot: bit a = coupling to the upper neighbour on axis a is
@@ -143,6 +144,7 @@ struct LevelDesc {
   Geo g, gc;          // this level, next coarser level
   LevelPtrs<T> P, C;  // this level, next coarser level
   CGState* st;
+  const LevelMeta* stats;  // the level's metadata, whatever P.meta says
 };
 
 // Level 0 of one system and its CG vectors.
@@ -167,20 +169,15 @@ struct Item {
   int block0;  // first virtual block of this item
 };
 
-constexpr int SPAN_SLOTS = 256;
-
 struct Stage {
   const Item* items;
   int nitems;
   int nblocks;  // total virtual blocks
   // measurement only: when non-NULL, a level-0 smooth(0) launch adds the
   // level-0 cells of every system it sweeps (done systems are skipped) to
-  // *swept, and its span on the device's constant-rate clock: workgroup b
-  // folds its start into t0[b % SPAN_SLOTS] (atomic min) and its end into
-  // t1[b % SPAN_SLOTS] (atomic max), so few workgroups share an address
+  // swept[0] and their FLUID cells to swept[1] (its duration comes from HIP
+  // events recorded around the dispatch)
   unsigned long long* swept;
-  unsigned long long* t0;
-  unsigned long long* t1;
 };
 
 // ---------------------------------------------------------------------------
@@ -1049,18 +1046,16 @@ enum StageOp {
   ST_SMOOTH1_RES = 8,  // smooth(l, 1), and the colour-1 residual into res[1]
 };
 
-// L0 only separates the level-0 instantiation (the dominant launches) from
-// the coarser levels' in profiles; the code is the same.
+// L0 only separates instantiations in profiles; the code is the same: 0 the
+// coarser levels, 1 level 0, 2 level 0's smooth(0) launches that the launch
+// probe records (so a profiler's statistics of that instantiation cover
+// exactly the launches the probe timed).
 template <typename T, int DIM, int OP, int L0>
 __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict__ descs,
                                                  Stage S) {
   ItemCache ic;
   bool done = true;
   Coupling<T> kc{false, (T)0};
-  if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
-    if (S.t0 != nullptr && threadIdx.x == 0)
-      atomicMin(S.t0 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
-  }
   OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
@@ -1072,9 +1067,11 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     }
     if (done) continue;
     const LevelDesc<T>& D = descs[ic.desc];
-    if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
-      if (S.swept != nullptr && it.vb == ic.block0 && threadIdx.x == 0)
+    if constexpr (L0 == 2 && OP == ST_SMOOTH0) {
+      if (S.swept != nullptr && it.vb == ic.block0 && threadIdx.x == 0) {
         atomicAdd(S.swept, (unsigned long long)D.g.ncells);
+        atomicAdd(S.swept + 1, (unsigned long long)ldg(&D.stats->nfluid));
+      }
     }
     if constexpr (OP == ST_SMOOTH0 || OP == ST_SMOOTH1 || OP == ST_SMOOTH1_RES) {
       constexpr int c = OP == ST_SMOOTH0 ? 0 : 1;
@@ -1117,13 +1114,6 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     } else if (OP == ST_PROLONGATE) {
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 0, t);
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 1, t);
-    }
-  }
-  if constexpr (L0 == 1 && OP == ST_SMOOTH0) {
-    if (S.t1 != nullptr) {  // uniform: a kernel argument
-      __syncthreads();      // every wave of the workgroup has finished
-      if (threadIdx.x == 0)
-        atomicMax(S.t1 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
     }
   }
 }
@@ -2301,6 +2291,7 @@ __global__ void k_meta_init(LevelMeta* m) {
   if (threadIdx.x == 0) {
     m->minbits = 0xffffffffu;
     m->maxbits = 0u;
+    m->nfluid = 0u;
   }
 }
 
@@ -2343,7 +2334,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
                                                       const float* __restrict__ Ax,
                                                       LevelPtrs<T> P, LevelMeta* meta) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
-  uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u;
+  uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
   if (t < g.size) {
     // every canonical value of both colours is loaded first (ghost slots and
     // missing lower neighbours read cell 0 and discard it)
@@ -2418,17 +2409,23 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
       P.r[c][t] = (T)0;
       P.z[c][t] = (T)0;
       if (P.res[c] != nullptr) P.res[c][t] = (T)0;
-      if (gvc == MG_FLUID) wet = 1u;
+      if (gvc == MG_FLUID) {
+        wet = 1u;
+        nfl += 1u;
+      }
     }
   }
-  __shared__ uint32_t smn[BLOCK / 64], smx[BLOCK / 64], swet[BLOCK / 64];
+  __shared__ uint32_t smn[BLOCK / 64], smx[BLOCK / 64], swet[BLOCK / 64], snfl[BLOCK / 64];
   mn = wave_min_u32(mn);
   mx = wave_max_u32(mx);
   wet = wave_max_u32(wet);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nfl += (uint32_t)__shfl_xor((int)nfl, o);
   if ((threadIdx.x & 63) == 0) {
     smn[threadIdx.x >> 6] = mn;
     smx[threadIdx.x >> 6] = mx;
     swet[threadIdx.x >> 6] = wet;
+    snfl[threadIdx.x >> 6] = nfl;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -2436,7 +2433,9 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
       mn = smn[w] < mn ? smn[w] : mn;
       mx = smx[w] > mx ? smx[w] : mx;
       wet = swet[w] > wet ? swet[w] : wet;
+      nfl += snfl[w];
     }
+    if (nfl != 0u) atomicAdd(&meta->nfluid, nfl);
     if (P.wet != nullptr) ((uint8_t*)P.wet)[blockIdx.x] = (uint8_t)wet;
     if (P.ord[0] != nullptr)  // per-block work order: natural, with the wet bit
       ((int32_t*)P.ord[0])[blockIdx.x] = (int32_t)blockIdx.x | (wet ? ORD_WET : 0);

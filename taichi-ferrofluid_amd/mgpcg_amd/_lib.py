@@ -10,7 +10,7 @@ import os
 import torch  # noqa: F401  (must precede loading the HIP library)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmgpcg.so")
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libmgpcg.so")  # taichi-ferrofluid_amd/
 
 PLAIN, MIXED, F64 = 0, 1, 2
 RUNNING, CONVERGED, MAX_ITERS, BREAKDOWN = 0, 1, 2, 3
@@ -96,9 +96,9 @@ SIGNATURES = {
     "mgpcg_server_pending": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                   ctypes.POINTER(_I)]),
     "mgpcg_server_probe": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_D),
-                                ctypes.POINTER(ctypes.c_int64)]),
+                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "mgpcg_batch_probe": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_D),
-                               ctypes.POINTER(ctypes.c_int64)]),
+                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "mgpcg_graph_counts": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(ctypes.c_int64)]),
     "mgpcg_last_error": (ctypes.c_char_p, []),

@@ -28,7 +28,7 @@ import json
 import os
 import time
 
-import _lib
+from . import _lib
 
 OPTIONS = {
     "max_grid": _lib.OPT_MAX_GRID,
@@ -78,7 +78,7 @@ def _served(make_scene, n_levels, precision, concurrency, requests, warmup, devi
     (`warmup` + `requests`)-th.  Returns (solves/s, {request index: x of the
     timed requests}); x stays on the device (a copy in stream order), so
     keeping it does not stall the host's submissions."""
-    import serve
+    from . import serve
 
     srv = serve.Server(64 << 30, precision=precision, n_levels=n_levels, device=device)
     xs = {}

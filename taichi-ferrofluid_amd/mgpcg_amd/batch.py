@@ -9,7 +9,7 @@ alone (the per-system arithmetic does not depend on the batch).
 """
 import ctypes
 
-import _lib
+from . import _lib
 
 
 class BatchSolver:

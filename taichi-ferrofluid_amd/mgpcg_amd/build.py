@@ -9,9 +9,10 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CSRC = os.path.join(HERE, "csrc")
-INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-OUT = os.path.join(HERE, "libmgpcg.so")
+PKG_ROOT = os.path.dirname(HERE)  # taichi-ferrofluid_amd/
+CSRC = os.path.join(PKG_ROOT, "csrc")
+INCLUDE = os.path.join(os.path.dirname(PKG_ROOT), "include")
+OUT = os.path.join(PKG_ROOT, "libmgpcg.so")
 
 SOURCES = ["mgpcg_solver.hip", "mgpcg_strategies.hip", "mgpcg_levelset.hip"]
 

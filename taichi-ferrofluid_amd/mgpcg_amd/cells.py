@@ -6,7 +6,7 @@ init_boundary follows FluidSimulator.init_boundary (fluid_simulator.py:337-341):
 the one-cell border becomes SOLID.  ``cell_type`` is an int8 device tensor
 (FLUID 0, AIR 1, SOLID 2), updated in place.
 """
-import _lib
+from . import _lib
 
 
 def apply_markers(cell_type, phi):

@@ -12,7 +12,7 @@ import math
 
 import torch
 
-import _lib
+from . import _lib
 
 
 class LevelSet:

@@ -5,8 +5,8 @@ p0 + 1/2 k mu0 |H|^2, with H interpolated at the face (H_interp,
 ferrofluid_simulator.py:66-70) and no ghost-fluid weight in b; A is inherited.
 ``H`` is a list of dim float32 face fields (shape res + e_k).
 """
-import _lib
-from pressure_project import PressureProjectStrategy
+from . import _lib
+from .pressure_project import PressureProjectStrategy
 
 
 class MagneticForceStrategy(PressureProjectStrategy):

@@ -17,8 +17,8 @@ import time
 
 import torch
 
-import _lib
-import utils
+from . import _lib
+from . import utils
 
 PRECISIONS = {"plain": _lib.PLAIN, "mixed": _lib.MIXED, "f64": _lib.F64}
 

@@ -6,7 +6,7 @@ the reference point psi(0) = 0.  A is negative definite apart from the
 reference row, so nothing in the solver assumes Adiag > 0 or pAp > 0.
 ``chi`` is float32 [*res]; ``H_ext`` a list of dim face fields (res + e_k).
 """
-import _lib
+from . import _lib
 
 
 class PotentialFunctionStrategy:
@@ -18,10 +18,6 @@ class PotentialFunctionStrategy:
         self.dx = dx
         self.H_ext = H_ext
         self.chi = chi
-
-    def in_domain(self, I):
-        """potential_function.py:15-17 (on the fine res at every level)."""
-        return all(i >= 0 for i in I) and all(i + 1 <= n for i, n in zip(I, self.res))
 
     def build_b(self, solver):
         """build_b_kernel (potential_function.py:19-34)."""

@@ -9,7 +9,7 @@ tensors (shape res + e_k), ``phi`` the float32 level set [*res].
 ``c_max`` is the ghost-fluid clamp ``min(c, 1e3)`` of pressure_project.py:37,62
 (1e3 in the reference; a clamp theta >= theta_min is c_max = 1 / theta_min).
 """
-import _lib
+from . import _lib
 
 
 class PressureProjectStrategy:

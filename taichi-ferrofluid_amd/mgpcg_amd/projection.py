@@ -10,7 +10,7 @@ component k of shape res + e_k) in place; ``pressure`` is the solver's x.
 """
 import torch
 
-import _lib
+from . import _lib
 
 
 def _vel_ptrs(velocity):

@@ -21,10 +21,10 @@ import time
 
 import torch
 
-import _lib
-import mgpcg
-import scenes
-import systems
+from . import _lib
+from . import mgpcg
+from . import scenes
+from . import systems
 
 
 class Server:
@@ -165,12 +165,16 @@ class Server:
             req["x"] = s.x.clone()
         self._release(s)
 
-    def probe(self, enable):
+    def probe(self, enable, fluid=False):
         """Start (True) / stop (False) timing the level-0 half-sweep launches;
-        stopping returns (launches, device ms, level-0 cells swept)."""
-        n, ms, cells = ctypes.c_int(), ctypes.c_double(), ctypes.c_int64()
+        stopping returns (launches, device ms, level-0 cells swept), plus the
+        FLUID cells among them when ``fluid``."""
+        n, ms, cells, fl = ctypes.c_int(), ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
         _lib.check(self._lib.mgpcg_server_probe(self._h, int(bool(enable)), ctypes.byref(n),
-                                                ctypes.byref(ms), ctypes.byref(cells)))
+                                                ctypes.byref(ms), ctypes.byref(cells),
+                                                ctypes.byref(fl)))
+        if fluid:
+            return n.value, ms.value, cells.value, fl.value
         return n.value, ms.value, cells.value
 
     def graph_counts(self):

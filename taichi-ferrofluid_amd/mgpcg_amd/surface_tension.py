@@ -13,8 +13,8 @@ import time
 
 import torch
 
-import _lib
-import utils
+from . import _lib
+from . import utils
 
 
 class SurfaceTensionStrategy:
@@ -97,7 +97,7 @@ class SurfaceTension:
             end2 = time.perf_counter()
 
             if verbose:
-                print(f"\033[33msolve surface tension ({k}), init cost {end1 - start1}s, "
-                      f"solve cost {end2 - start2}s\033[0m")
+                print(f"surface tension, axis {k}: assembly {1e3 * (end1 - start1):.3f} ms, "
+                      f"solve {1e3 * (end2 - start2):.3f} ms")
             utils.copy(self.poisson_solver.x, self.simulator.velocity[k][
                 tuple(slice(0, n) for n in self.res)])

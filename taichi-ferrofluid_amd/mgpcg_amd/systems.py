@@ -9,12 +9,12 @@ import types
 import numpy as np
 import torch
 
-import _lib
-from level_set import LevelSet
-from potential_function import PotentialFunctionStrategy
-from pressure_project import PressureProjectStrategy
-from surface_tension import SurfaceTension
-from volume_control import PressureProjectWithVolumeControlStrategy
+from . import _lib
+from .level_set import LevelSet
+from .potential_function import PotentialFunctionStrategy
+from .pressure_project import PressureProjectStrategy
+from .surface_tension import SurfaceTension
+from .volume_control import PressureProjectWithVolumeControlStrategy
 
 
 def _t(a, device):

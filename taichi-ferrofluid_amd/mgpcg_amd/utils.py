@@ -10,11 +10,6 @@ AIR = 1
 SOLID = 2
 
 
-def clamp(x, a, b):
-    """utils.py:8-10."""
-    return max(a, min(b, x))
-
-
 def copy(src, dst):
     """utils.py:59-62: dst[I] = src[I] (torch tensors of the same shape)."""
     dst.copy_(src)

@@ -7,8 +7,8 @@ of the per-cell FP32 volumes, so the sum does not depend on launch order).
 """
 import torch
 
-import _lib
-from pressure_project import PressureProjectStrategy
+from . import _lib
+from .pressure_project import PressureProjectStrategy
 
 
 class PressureProjectWithVolumeControlStrategy(PressureProjectStrategy):
@@ -19,14 +19,17 @@ class PressureProjectWithVolumeControlStrategy(PressureProjectStrategy):
         self.dt = dt
         self.verbose = verbose
 
+        # Controller state (volume_control.py:13-20): the first call's volume
+        # is the target; a PI controller with the reference's gains, k_p =
+        # 2.3 / (n_p dt) with n_p = 25 steps, and k_i = (k_p / (2 zeta))^2
+        # with damping ratio zeta = 2.
         self.step = 0
-        self.vol_0 = 0.0  # the desired value
-        # proportional-integral (PI) controller (volume_control.py:17-21)
-        self.y = 0.0
+        self.vol_0 = 0.0
+        self.y = 0.0  # time integral of the relative volume error
         self.n_p = 25
-        self.k_p = 2.3 / (self.n_p * self.dt)  # proportional gain
-        self.zeta = 2  # suppress the noise coming from the volume computation
-        self.k_i = (self.k_p / (2 * self.zeta)) ** 2  # PI gain
+        self.k_p = 2.3 / (self.n_p * self.dt)
+        self.zeta = 2
+        self.k_i = (self.k_p / (2 * self.zeta)) ** 2
         self.last_c = None
         self._partials = None
         self._out = None
@@ -63,10 +66,12 @@ class PressureProjectWithVolumeControlStrategy(PressureProjectStrategy):
             self.last_c = None
             return
 
-        x = (vol - self.vol_0) / self.vol_0  # normalised volume error
-        self.y += x * self.dt  # integral of the error
-        c = (-self.k_p * x - self.k_i * self.y) / (x + 1)  # the required divergence
+        # relative error x, its integral y, and the divergence c the fluid
+        # must gain per unit time to steer the volume back (volume_control.py:47-49)
+        x = (vol - self.vol_0) / self.vol_0
+        self.y += x * self.dt
+        c = (-self.k_p * x - self.k_i * self.y) / (x + 1)
         self.last_c = c
         if self.verbose:
-            print(f"\033[31mvolume = {vol}, c = {c}\033[0m")
+            print(f"volume control: step {self.step}, volume {vol:.6g} cells, c = {c:.6g}")
         self.add_c(solver, c)

@@ -23,7 +23,7 @@ def dev(request):
     gpu), or the CPU emulation build of the same HIP sources ("emu" -> torch
     CPU tensors), which runs every kernel and the engine bounds-checked on the
     host.  One process loads one of the two libraries."""
-    import _lib
+    from mgpcg_amd import _lib
 
     if request.param == "emu":
         if _lib._lib is not None and _lib._path != EMU_LIB:

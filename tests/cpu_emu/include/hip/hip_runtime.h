@@ -118,6 +118,11 @@ inline T __shfl(T v, int src, int width = 64) {
 }
 inline void __threadfence() {}
 // the emulator runs every lane on one host thread, so a plain update is atomic
+inline unsigned atomicAdd(unsigned* p, unsigned v) {
+  unsigned o = *p;
+  *p = o + v;
+  return o;
+}
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
   const unsigned long long old = *p;
   *p = old + v;

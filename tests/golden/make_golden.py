@@ -32,7 +32,7 @@ CASES = {
 
 
 def build(name):
-    import scenes
+    from mgpcg_amd import scenes
     from oracle import core
     from oracle import systems as osys
 

@@ -41,7 +41,7 @@ def lib():
 
 
 def test_header_declares_the_binding_table():
-    import _lib
+    from mgpcg_amd import _lib
 
     assert declared_functions() == sorted(_lib.SIGNATURES)
 
@@ -63,7 +63,7 @@ def test_version_matches_header(lib):
 
 def test_constants_match_header():
     """The Python binding's constants are the header's #defines."""
-    import _lib
+    from mgpcg_amd import _lib
 
     defs = dict(re.findall(r"#define (MGPCG_[A-Z0-9_]+)\s+(-?\d+)", open(HEADER).read()))
     pairs = {"PLAIN": "MGPCG_PLAIN", "MIXED": "MGPCG_MIXED", "F64": "MGPCG_F64",
@@ -91,7 +91,7 @@ def test_constants_match_header():
 
 
 def test_result_struct_layout():
-    import _lib
+    from mgpcg_amd import _lib
 
     assert ctypes.sizeof(_lib.MgpcgResult) == 8 + 6 * 8
 

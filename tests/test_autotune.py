@@ -7,13 +7,13 @@ emulation build.  The last two tests are host logic only.
 import pytest
 import torch
 
-import scenes
+from mgpcg_amd import scenes
 from test_solve import KNOB_DEFAULTS, _emu, set_knobs
 
 
 def test_tune_small_space(dev, tmp_path):
-    import autotune
-    import systems
+    from mgpcg_amd import autotune
+    from mgpcg_amd import systems
 
     emu = _emu(dev)
     lo, hi = (16, 24) if emu else (32, 64)
@@ -51,7 +51,7 @@ def test_tune_small_space(dev, tmp_path):
 def test_tune_rejects_a_setting_that_changes_results(monkeypatch):
     """Host logic: a candidate whose results differ from the first
     measurement's stops the tuner, and the starting settings come back."""
-    import autotune
+    from mgpcg_amd import autotune
 
     state = {"n": 0, "applied": []}
     monkeypatch.setattr(autotune, "current", lambda names=None: {"fuse_p": 0})
@@ -70,7 +70,7 @@ def test_tune_rejects_a_setting_that_changes_results(monkeypatch):
 def test_tune_keeps_only_clear_gains(monkeypatch):
     """Host logic: a value is kept only when it beats the best by min_gain;
     a pass that changes nothing ends the descent."""
-    import autotune
+    from mgpcg_amd import autotune
 
     rates = {(2048, 0): 100.0, (64, 0): 100.5, (4096, 0): 120.0, (4096, 1): 119.0,
              (64, 1): 99.0, (2048, 1): 98.0}

@@ -39,7 +39,7 @@ def test_oracle_reproduces_golden(name):
 
 
 def _solver_from_golden(g, dev, precision):
-    import mgpcg
+    from mgpcg_amd import mgpcg
 
     L = int(g["L"])
     res = list(g["gt0"].shape)
@@ -73,9 +73,9 @@ def test_library_reproduces_golden_mixed(dev, name):
 
 @pytest.mark.parametrize("name", NAMES)
 def test_device_assembly_reproduces_golden(dev, name):
-    import mgpcg
-    import scenes
-    import systems
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import scenes
+    from mgpcg_amd import systems
 
     g = load(name)
     sysd = make_golden.CASES[name](scenes)

@@ -15,19 +15,19 @@ import numpy as np
 import pytest
 import torch
 
-import scenes
+from mgpcg_amd import scenes
 from test_solve import SYSTEMS, _emu, _x, set_knobs
 
 
 def _graph(on):
-    import _lib
+    from mgpcg_amd import _lib
 
     _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_GRAPH, int(on)))
 
 
 def _solver(sysd, dev, precision):
-    import mgpcg
-    import systems
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import systems
 
     s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], sysd["nu"], sysd["bottom"],
                                  precision=precision, device=dev)
@@ -45,7 +45,7 @@ def _eager_work(dev):
 
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
 def test_single_solve_replays_equal_eager(dev, precision):
-    import _lib
+    from mgpcg_amd import _lib
 
     if _emu(dev) and precision == "mixed":
         pytest.skip("the emulator covers PLAIN; the server and batch cases cover MIXED")
@@ -84,8 +84,8 @@ def test_server_replays_equal_alone(dev):
     """Continuous batching with graphs: the member set changes at every
     admission and retirement (a new capture), and stays unchanged in between
     (replays)."""
-    import serve
-    import systems
+    from mgpcg_amd import serve
+    from mgpcg_amd import systems
 
     emu = _emu(dev)
     nreq = 3 if emu else 6
@@ -123,7 +123,7 @@ def test_server_replays_equal_alone(dev):
 
 
 def test_batch_replays_equal_alone(dev):
-    import batch
+    from mgpcg_amd import batch
 
     emu = _emu(dev)
     count, div, L = (2, 4, 3) if emu else (6, 2, 4)

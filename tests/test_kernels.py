@@ -23,7 +23,7 @@ CASES = [
 
 
 def _solver(h, dev, precision="plain"):
-    import mgpcg
+    from mgpcg_amd import mgpcg
 
     res = list(h["gt"][0].shape)
     dim = len(res)
@@ -47,7 +47,7 @@ def _case_id(c):
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 def test_coarsen_bit_exact(dev, case):
-    import _lib
+    from mgpcg_amd import _lib
 
     res, _ = case
     rng = np.random.default_rng(1)
@@ -65,7 +65,7 @@ def test_coarsen_bit_exact(dev, case):
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 @pytest.mark.parametrize("phase", [0, 1])
 def test_smooth_bit_exact(dev, case, phase, coupling):
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=2, coupling=coupling)
@@ -87,7 +87,7 @@ def test_smooth_bit_exact(dev, case, phase, coupling):
 def test_smooth_pair_from_zero(dev, case):
     """z.fill(0); smooth(l, 0); smooth(l, 1) in one kernel (colour 0 recomputed
     on the fly inside the colour-1 stencil)."""
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=4)
@@ -109,7 +109,7 @@ def test_smooth_pair_from_zero(dev, case):
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 def test_prolong_smooth1_bit_exact(dev, case):
     """prolongate(l); smooth(l, 1) fused: every colour-1 FLUID value matches."""
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=16)
@@ -142,7 +142,7 @@ def test_restrict_bit_exact(dev, case, coupling, rows):
     gather skips blocks of coarse cells none of which is FLUID (their r is
     never read), so it is compared on the FLUID coarse cells; the other two
     write every coarse cell and are compared everywhere."""
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=6, coupling=coupling)
@@ -178,7 +178,7 @@ def test_restrict_bit_exact(dev, case, coupling, rows):
 
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 def test_prolongate_bit_exact(dev, case):
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=8)
@@ -201,7 +201,7 @@ def test_prolongate_bit_exact(dev, case):
 @pytest.mark.parametrize("coupling", [None, -0.75], ids=["random", "uniform"])
 def test_update_p_compute_Ap_bit_exact(dev, case, coupling):
     """p_next = z + 0 * p (update_p with beta = 0), Ap = A p_next."""
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=10, coupling=coupling)
@@ -225,7 +225,7 @@ def test_update_p_compute_Ap_bit_exact(dev, case, coupling):
 @pytest.mark.parametrize("case", CASES, ids=_case_id)
 def test_vcycle_bit_exact(dev, case):
     """Every FLUID value of z = M r; non-FLUID z is inert (unprolonged)."""
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     h = random_hierarchy(res, L, seed=12)
@@ -244,7 +244,7 @@ def test_vcycle_bit_exact(dev, case):
 def test_vcycle_schedule_variants(dev, case, coarse_max):
     """All levels on batched kernels (coarse_max 0) or all in the
     one-workgroup coarse kernel: the same FLUID values."""
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     lib = _lib.load()
@@ -265,7 +265,7 @@ def test_vcycle_schedule_variants(dev, case, coarse_max):
 
 @pytest.mark.parametrize("case", CASES[:2], ids=_case_id)
 def test_vcycle_f64_matches_oracle(dev, case):
-    import _lib
+    from mgpcg_amd import _lib
 
     res, L = case
     h = core.to_f64(random_hierarchy(res, L, seed=14))

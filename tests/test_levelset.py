@@ -72,7 +72,7 @@ def test_oracle_markers_and_boundary():
 # -- library against the oracle ---------------------------------------------------
 @pytest.mark.parametrize("res", [(24, 20), (14, 12, 16)], ids=["2d", "3d"])
 def test_markers_and_boundary_match_oracle(dev, res):
-    import cells
+    from mgpcg_amd import cells
 
     rng = np.random.default_rng(1)
     ct = rng.integers(0, 3, res).astype(np.int8)
@@ -89,7 +89,7 @@ def test_markers_and_boundary_match_oracle(dev, res):
                          ids=["2d", "3d", "3d_cube"])
 @pytest.mark.parametrize("repeat", [2, 3])
 def test_redistance_matches_oracle(dev, res, repeat):
-    from level_set import FastSweepingLevelSet
+    from mgpcg_amd.level_set import FastSweepingLevelSet
 
     dx = 1.0 / max(res)
     c = [0.5 * n * dx for n in res]

@@ -966,3 +966,43 @@ def test_core_pin_rejects_planted_c_oracle(tmp_path_factory, pin, old, new):
     lib = planted_c(tmp_path_factory, [(old, new)])
     with oracle_lib(lib):
         assert not CORE_PINS[pin]()
+
+
+# ----------------------------------------------------------------------------
+# PressureProjectStrategy.build_A (pressure_project.py:45-64), GFM off
+# ----------------------------------------------------------------------------
+def _check_pressure_dirichlet(build_A, res=(10, 8, 12), modes=(2, 1, 3)):
+    """All-FLUID interior in an AIR ring, GFM off: A = sA times the Dirichlet
+    Laplacian, so A v = lambda v for v = prod sin(pi k_a i_a / (m_a + 1))."""
+    sA = 0.375
+    ct = np.full(res, AIR, np.int8)
+    ct[tuple(slice(1, n - 1) for n in res)] = FLUID
+    Ad, Ax = build_A(ct, 0, sA, ghost_fluid_method=False)
+    Ad, Ax = Ad.astype(np.float64), Ax.astype(np.float64)
+    G = _grid(res)
+    v = np.ones(res)
+    lam = 0.0
+    for a, (n, k) in enumerate(zip(res, modes)):
+        m = n - 2
+        v = v * np.sin(np.pi * k * G[a] / (m + 1))
+        lam += 2 - 2 * np.cos(np.pi * k / (m + 1))
+    lam *= sA
+    v = np.where(ct == FLUID, v, 0.0)
+    A = assemble_matrix(ct, Ad, Ax)
+    Av = (A @ v.ravel()).reshape(res)
+    fl = ct == FLUID
+    return rel_l2(Av[fl], lam * v[fl])
+
+
+def test_pressure_A_dirichlet_eigenpair():
+    assert _check_pressure_dirichlet(S.pressure_build_A) < 1e-12
+
+
+@pytest.mark.parametrize("old,new", [
+    ("Adiag = np.where(m_a, (Adiag + sA).astype(f32), Adiag)", "Adiag = Adiag"),
+    ("Ax[..., k] = np.where(m_f, -sA, Ax[..., k])", "Ax[..., k] = np.where(m_f, sA, Ax[..., k])"),
+    ("if s > 0:\n                Ax[..., k]", "if s < 0:\n                Ax[..., k]"),
+])
+def test_pressure_A_check_rejects_planted(old, new):
+    bad = planted(S.pressure_build_A, old, new)
+    assert _check_pressure_dirichlet(bad) > 1e-3

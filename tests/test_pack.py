@@ -122,7 +122,7 @@ def _check(s, gts, Ads, Axs):
 @pytest.mark.parametrize("coupling", [None, -0.75], ids=["random", "uniform"])
 @pytest.mark.parametrize("res,L", [((32, 24), 3), ((16, 12, 10), 2)], ids=["2d", "3d"])
 def test_flags_of_random_hierarchy(dev, res, L, coupling):
-    import mgpcg
+    from mgpcg_amd import mgpcg
 
     h = random_hierarchy(list(res), L, seed=21, coupling=coupling)
     s = mgpcg.MGPCGPoissonSolver(len(res), list(res), L, 2, 10, precision="plain", device=dev)
@@ -138,9 +138,9 @@ def test_flags_of_random_hierarchy(dev, res, L, coupling):
 def test_flags_of_pressure_scene(dev):
     """A ghost-fluid pool+sphere pressure system: the level's operator as the
     strategy assembled it, and most FLUID cells get a decoded diagonal."""
-    import mgpcg
-    import scenes
-    import systems
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import scenes
+    from mgpcg_amd import systems
 
     sysd = scenes.c2_3d(seed=1, n=16, L=3)
     s = mgpcg.MGPCGPoissonSolver(3, sysd["res"], 3, 2, 10, precision="mixed", device=dev)
@@ -162,7 +162,7 @@ def test_flags_of_pressure_scene(dev):
 def test_pool_bytes_is_what_create_allocates(dev, precision):
     """The admission charge comes from the library's own layout: the size
     query equals the bytes the created handle holds, for 2D and 3D shapes."""
-    import mgpcg
+    from mgpcg_amd import mgpcg
 
     for res, L in (([16, 16, 32], 3), ([32, 48, 16], 4), ([64, 32], 4), ([16, 16], 1)):
         q = mgpcg.pool_bytes(res, L, precision)

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 import torch
 
-import scenes
+from mgpcg_amd import scenes
 from helpers import rel_l2
 from oracle import strategies as S
 
@@ -44,7 +44,7 @@ def _t(a, dev):
 @pytest.mark.parametrize("res", [(24, 20), (12, 10, 14)], ids=["2d", "3d"])
 @pytest.mark.parametrize("gfm", [False, True], ids=["plain", "gfm"])
 def test_apply_pressure_matches_oracle(dev, res, gfm):
-    import projection
+    from mgpcg_amd import projection
 
     ct, phi, p, vel, _ = _fields(res, 3, dev)
     dt, rho, dx, p0, c_max = 1.25e-2, 1000.0, 1.0 / 32, 0.7, 50.0
@@ -63,7 +63,7 @@ def test_apply_pressure_matches_oracle(dev, res, gfm):
 
 @pytest.mark.parametrize("res", [(24, 20), (12, 10, 14)], ids=["2d", "3d"])
 def test_apply_magnetic_force_matches_oracle(dev, res):
-    import projection
+    from mgpcg_amd import projection
 
     ct, _, p, vel, H = _fields(res, 4, dev)
     dt, rho, dx, p0, mu0, kk = 1.25e-2, 1000.0, 1.0 / 32, 0.3, 1.25663706e-6, 2.0
@@ -76,7 +76,7 @@ def test_apply_magnetic_force_matches_oracle(dev, res):
 
 
 def _pressure_b(sysd, dev, velocity, ct, phi):
-    import _lib
+    from mgpcg_amd import _lib
 
     lib = _lib.load()
     b = torch.empty(sysd["res"], dtype=torch.float32, device=dev)
@@ -90,9 +90,9 @@ def _pressure_b(sysd, dev, velocity, ct, phi):
 
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
 def test_projection_leaves_residual_divergence(dev, precision):
-    import mgpcg
-    import projection
-    import systems
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import projection
+    from mgpcg_amd import systems
 
     emu = dev == "cpu"
     sysd = scenes.c2_3d(seed=5, n=24 if emu else 64, L=3 if emu else 4)
@@ -125,7 +125,7 @@ def test_projection_leaves_residual_divergence(dev, precision):
 @pytest.mark.parametrize("res", [(20, 18), (12, 10, 14)], ids=["2d", "3d"])
 @pytest.mark.parametrize("iters", [1, 10])
 def test_extrap_velocity_matches_oracle(dev, res, iters):
-    import projection
+    from mgpcg_amd import projection
     from oracle import levelset as OL
 
     rng = np.random.default_rng(7)

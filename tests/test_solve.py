@@ -13,7 +13,7 @@ reduced sizes.
 import numpy as np
 import pytest
 
-import scenes
+from mgpcg_amd import scenes
 from helpers import rel_l2
 from oracle import core
 from oracle import systems as osys
@@ -22,9 +22,9 @@ PREC = {"plain": core.PLAIN, "mixed": core.MIXED, "f64": core.F64}
 
 
 def gpu_solve(sysd, dev, precision="plain", hostsync=False, max_iters=500, rel_tol=1e-12):
-    import _lib
-    import mgpcg
-    import systems
+    from mgpcg_amd import _lib
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import systems
 
     d = systems.upload(sysd, dev)
     s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], sysd["nu"], sysd["bottom"],
@@ -84,9 +84,9 @@ def test_hostsync_loop_equals_device_loop(dev):
 def test_reference_structured_equals_device_loop(dev):
     """The plain baseline (one launch per reference kernel, scalars on the
     host) computes exactly what the batched engine computes in PLAIN."""
-    import _lib
-    import mgpcg
-    import systems
+    from mgpcg_amd import _lib
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import systems
 
     sysd = SYSTEMS["c2_3d_small"](_emu(dev))
     xa, outa, _ = gpu_solve(sysd, dev, "plain")
@@ -105,7 +105,7 @@ KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 0, "fuse_pr
 
 
 def set_knobs(**kw):
-    import _lib
+    from mgpcg_amd import _lib
 
     lib = _lib.load()
     opt = {"coarse_max": _lib.OPT_COARSE_MAX_SLOTS, "max_grid": _lib.OPT_MAX_GRID,
@@ -199,8 +199,8 @@ def test_max_iters_and_zero_rhs(dev):
 
 
 def _assembled(sysd, dev, precision):
-    import mgpcg
-    import systems
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import systems
 
     d = systems.upload(sysd, dev)
     s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], sysd["nu"], sysd["bottom"],
@@ -234,7 +234,7 @@ def test_level_coupling_detection(dev, kind):
 def test_batch_uneven_equals_alone(dev, precision):
     """A batch mixing the four system kinds at uneven sizes: every result is
     bit-identical to the same system solved alone, and matches the oracle."""
-    import batch
+    from mgpcg_amd import batch
 
     emu = _emu(dev)
     count, div, L = (3, 4, 3) if emu else (8, 2, 4)
@@ -262,8 +262,8 @@ def test_server_continuous_batching_equals_alone(dev, depth, streams):
     same bits as solving alone, however many iterations the server keeps
     queued ahead of the host and however many streams it spreads its members
     over; slots and hierarchies are reused."""
-    import serve
-    import systems
+    from mgpcg_amd import serve
+    from mgpcg_amd import systems
 
     if _emu(dev) and depth == 3:
         pytest.skip("the emulator covers depths 1 and 2")
@@ -289,16 +289,21 @@ def test_server_continuous_batching_equals_alone(dev, depth, streams):
             rq["coarse_from"] = rq["solver"].info(0)["coarse_from"]
             srv.retire(rq)
             done.append(rq)
-    n_launch, ms, cells = srv.probe(False)
+    n_launch, ms, cells, fluid = srv.probe(False, fluid=True)
     set_knobs(probe_every=4, server_depth=2, server_streams=2)
     assert srv.max_members >= 2  # they really overlapped
     # with every launch timed, the probe counts exactly the level-0 cells swept
     # by smooth(0, 0): two launches per V-cycle (nu = 2), iters + 1 V-cycles
     # per solve, none for members that finished on the device but were not
-    # yet retired, and none for systems run entirely by the coarse kernel
+    # yet retired, and none for systems run entirely by the coarse kernel;
+    # likewise their FLUID cells (counted by the packing kernel)
+    swept = [rq for rq in done if rq["coarse_from"] > 0]
     expect = sum((rq["result"]["iters"] + 1) * 2 * int(np.prod(reqs[rq["meta"]]["res"]))
-                 for rq in done if rq["coarse_from"] > 0)
+                 for rq in swept)
+    expect_fluid = sum((rq["result"]["iters"] + 1) * 2 *
+                       int(np.count_nonzero(reqs[rq["meta"]]["cell_type"] == 0)) for rq in swept)
     assert cells == expect
+    assert fluid == expect_fluid
     assert n_launch > 0 and ms >= 0.0
     for rq in done:
         sysd = reqs[rq["meta"]]
@@ -314,7 +319,7 @@ def deferred_streams(dev, request):
     each stream in turn (legal orders, as far from issue order as the events
     allow), so a missing event between streams changes the results.  On the
     GPU: the streams as they are (the second parameter repeats the test)."""
-    import _lib
+    from mgpcg_amd import _lib
 
     lib = _lib.load()
     if _emu(dev):
@@ -330,8 +335,8 @@ def test_server_resubmit_on_other_lane_equals_alone(dev, deferred_streams):
     other lane; its second solve must still equal solving it alone.  The
     queued iterations of the first lane read the system's state, so the second
     lane's reset of that state must wait for them (the retirement event)."""
-    import serve
-    import systems
+    from mgpcg_amd import serve
+    from mgpcg_amd import systems
 
     set_knobs(server_depth=2, server_streams=2)
     try:

@@ -11,15 +11,15 @@ import numpy as np
 import pytest
 import torch
 
-import scenes
+from mgpcg_amd import scenes
 from oracle import systems as osys
 from oracle import strategies as S
 
 
 def _assemble(sysd, dev):
-    import _lib
-    import mgpcg
-    import systems
+    from mgpcg_amd import _lib
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import systems
 
     d = systems.upload(sysd, dev)
     s = mgpcg.MGPCGPoissonSolver(sysd["dim"], sysd["res"], sysd["L"], 2, 10, device=dev)
@@ -70,9 +70,9 @@ def test_surface_tension_assembly(dev, d):
 
 
 def test_magnetic_b_bit_exact(dev):
-    import magnetic_force
-    import mgpcg
-    import systems
+    from mgpcg_amd import magnetic_force
+    from mgpcg_amd import mgpcg
+    from mgpcg_amd import systems
 
     sysd = scenes.c2_3d(seed=3, n=32, L=3)
     rng = np.random.default_rng(4)
@@ -93,8 +93,8 @@ def test_magnetic_b_bit_exact(dev):
 
 
 def test_calc_volume(dev):
-    import volume_control
-    from level_set import LevelSet
+    from mgpcg_amd import volume_control
+    from mgpcg_amd.level_set import LevelSet
 
     sysd = scenes.c2_3d(seed=1, n=32, L=3)
     phi = torch.from_numpy(sysd["phi"]).to(dev)
