@@ -86,7 +86,7 @@ def parse():
                     help="admissions per server step (0: no limit)")
     ap.add_argument("--server-depth", type=int, default=2,
                     help="iterations the server keeps queued on the device ahead of the host")
-    ap.add_argument("--server-streams", type=int, default=2,
+    ap.add_argument("--server-streams", type=int, default=1,
                     help="engine streams the server spreads the requests in flight over")
     ap.add_argument("--baseline-steps", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -226,8 +226,8 @@ def smooth_roofline(nl, ms_tot, cells_tot, fluid_tot, tpc):
     """Roofline of the probed level-0 half-sweep launches: algorithmic bytes
     of the reference's half-sweep over the cells the launches swept (20 B per
     FLUID cell + the type byte of the colour's cells, counted on the device)
-    over the launch duration from HIP events the dispatch records around the
-    kernel; traffic from the committed counters."""
+    over the launch's span on the device clock (first workgroup start to last
+    workgroup end); traffic from the committed counters."""
     cells = cells_tot / nl
     fluid = fluid_tot / nl
     alg_bytes = SMOOTH_FLUID_BYTES_PER_CELL_3D * fluid + SMOOTH_TYPE_BYTES_PER_CELL * cells
@@ -462,7 +462,11 @@ def profile_summary(db_path, bench_path, command):
     w(f"  launches the bench timed         {rf['launches']} (first {len(head)} dispatches of "
       f"that name in the trace)")
     w(f"  rocprofv3 average duration       {avg_us:.3f} us")
-    w(f"  bench HIP-event average          {rf['launch_us']} us")
+    w(f"  bench device-clock span average  {rf['launch_us']} us (first workgroup start to "
+      f"last workgroup end)")
+    w(f"  difference                       {avg_us - rf['launch_us']:.3f} us per launch: "
+      f"the profiler's interval adds the dispatch before the first workgroup and the "
+      f"completion signal after the last")
     w(f"  level-0 cells / FLUID per launch {rf['cells_per_launch']} / {rf['fluid_per_launch']}")
     w(f"  algorithmic bytes per launch     {rf['alg_bytes_per_launch']} ({rf['alg_model']})")
     w(f"  frac of 8 TB/s, rocprofv3 time   {frac_prof:.4f}")
@@ -678,6 +682,16 @@ def main():
             iso.pop("scenes")
             line["roofline_isolated"] = dict(iso["roofline"] or {}, server_streams=1,
                                              solves_per_s=round(iso["value"], 3))
+        else:
+            # the same requests spread over two engine streams: one lane's
+            # latency-bound kernels beside the other's level-0 kernels
+            two = measure_served(args, wl, n, L, args.seeds, args.concurrency,
+                                 max(16, args.steps // 2), 4, dev, streams=2)
+            two.pop("scenes")
+            line["two_lanes"] = {"solves_per_s": round(two["value"], 3),
+                                 "p50_ms": round(two["lat"]["p50_ms"], 3),
+                                 "p99_ms": round(two["lat"]["p99_ms"], 3),
+                                 "roofline": two["roofline"]}
         # single request in flight: the latency-bound rate
         one = measure_served(args, wl, n, L, args.seeds, 1, max(4, args.steps // 8), 2, dev)
         one.pop("scenes")

@@ -169,15 +169,21 @@ struct Item {
   int block0;  // first virtual block of this item
 };
 
+constexpr int SPAN_SLOTS = 256;
+
 struct Stage {
   const Item* items;
   int nitems;
   int nblocks;  // total virtual blocks
   // measurement only: when non-NULL, a level-0 smooth(0) launch adds the
   // level-0 cells of every system it sweeps (done systems are skipped) to
-  // swept[0] and their FLUID cells to swept[1] (its duration comes from HIP
-  // events recorded around the dispatch)
+  // swept[0] and their FLUID cells to swept[1], and its span on the device's
+  // constant-rate clock: workgroup b folds its start into t0[b % SPAN_SLOTS]
+  // (atomic min) and its end into t1[b % SPAN_SLOTS] (atomic max), so few
+  // workgroups share an address
   unsigned long long* swept;
+  unsigned long long* t0;
+  unsigned long long* t1;
 };
 
 // ---------------------------------------------------------------------------
@@ -1056,6 +1062,10 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
   ItemCache ic;
   bool done = true;
   Coupling<T> kc{false, (T)0};
+  if constexpr (L0 == 2 && OP == ST_SMOOTH0) {
+    if (S.t0 != nullptr && threadIdx.x == 0)
+      atomicMin(S.t0 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
+  }
   OrdScan ws;
   int iter = 0;
   for (VBIter it = vb_begin(S.nblocks); it.vb < it.end; it.vb += it.step, ++iter) {
@@ -1114,6 +1124,13 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
     } else if (OP == ST_PROLONGATE) {
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 0, t);
       prolongate_slot<T>(D.g, D.gc, D.P.z, D.C.z[0], D.C.z[1], 1, t);
+    }
+  }
+  if constexpr (L0 == 2 && OP == ST_SMOOTH0) {
+    if (S.t1 != nullptr) {  // uniform: a kernel argument
+      __syncthreads();      // every wave of the workgroup has finished
+      if (threadIdx.x == 0)
+        atomicMax(S.t1 + blockIdx.x % SPAN_SLOTS, (unsigned long long)wall_clock64());
     }
   }
 }

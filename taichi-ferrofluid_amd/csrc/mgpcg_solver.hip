@@ -12,7 +12,6 @@
 //               reads back, and retires / admits systems at iteration
 //               boundaries -- which is also how the server does continuous
 //               batching.
-#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -242,13 +241,14 @@ struct Engine {
   char* h_desc = nullptr;
   CGState* h_stinit = nullptr;
 
-  // launch-duration probe of the level-0 half-sweeps: HIP events recorded by
-  // the dispatch itself around each probed launch
+  // launch-duration probe of the level-0 half-sweeps, timed on the device
   bool probe_on = false;
   int probe_used = 0;  // launches recorded
   int probe_seen = 0;  // eligible launches since the probe started
   unsigned long long* d_swept = nullptr;  // [2]: level-0 cells, FLUID cells swept by probed launches
-  std::vector<hipEvent_t> probe_ev;       // [2 i], [2 i + 1]: start, end of recorded launch i
+  // per recorded launch i: workgroup starts folded into SPAN_SLOTS minima at
+  // [i * SPAN_SLOTS], ends into maxima at [(PROBE_POOL + i) * SPAN_SLOTS]
+  unsigned long long* d_span = nullptr;
 
   // hipGraph of one iteration of the current member set (g_graph): captured
   // on the second iteration of an item layout, replayed until the layout or
@@ -296,8 +296,7 @@ struct Engine {
       if (h_states[i]) (void)hipHostFree(h_states[i]);
       if (ev_states[i]) (void)hipEventDestroy(ev_states[i]);
     }
-    for (hipEvent_t ev : probe_ev) (void)hipEventDestroy(ev);
-    probe_ev.clear();
+    if (d_span) (void)hipFree(d_span);
     if (d_swept) (void)hipFree(d_swept);
     d_swept = nullptr;
     if (ev_user) (void)hipEventDestroy(ev_user);
@@ -617,17 +616,16 @@ int engine_build_items(Engine* e) {
 }
 
 // Probe: launches beyond the pool are not recorded.
-const int PROBE_POOL = 2048;
+const int PROBE_POOL = 4096;
 // Returns whether the launch is recorded.  Every g_probe_every-th eligible
 // launch is; it counts the cells it sweeps into d_swept (on the device, so
 // systems that finished on the device but are not retired yet are not
-// counted), and the dispatch records a pair of HIP events around its
-// execution (hipExtLaunchKernelGGL), the interval a profiler reports as the
-// kernel's duration.
+// counted) and records its span from the first workgroup's start to the last
+// workgroup's end on the device clock, i.e. the kernel's execution, without
+// the dispatch before it and the completion signal after it.
 bool probe_begin(Engine* e) {
   if (e->probe_seen++ % g_probe_every != 0) return false;
-  return e->probe_used < PROBE_POOL && e->d_swept != nullptr &&
-         (int)e->probe_ev.size() == 2 * PROBE_POOL;
+  return e->probe_used < PROBE_POOL && e->d_swept != nullptr && e->d_span != nullptr;
 }
 void probe_end(Engine* e) { e->probe_used++; }
 
@@ -637,6 +635,8 @@ inline Stage stage_of(const Engine* e, const Engine::StageRef& r) {
   s.nitems = r.n;
   s.nblocks = r.nblocks;
   s.swept = nullptr;
+  s.t0 = nullptr;
+  s.t1 = nullptr;
   return s;
 }
 inline int grid_of(const Engine::StageRef& r) { return std::min(r.nblocks, g_max_grid); }
@@ -657,9 +657,10 @@ struct Sched {
       Stage st = stage_of(e, r);
       if (probe) {
         st.swept = e->d_swept;
-        hipExtLaunchKernelGGL((k_stage<T, DIM, OP, 2>), dim3(grid_of(r)), dim3(BLOCK), 0,
-                              e->stream, e->probe_ev[2 * e->probe_used],
-                              e->probe_ev[2 * e->probe_used + 1], 0, levels(e), st);
+        st.t0 = e->d_span + (size_t)e->probe_used * SPAN_SLOTS;
+        st.t1 = e->d_span + (size_t)(PROBE_POOL + e->probe_used) * SPAN_SLOTS;
+        hipLaunchKernelGGL((k_stage<T, DIM, OP, 2>), dim3(grid_of(r)), dim3(BLOCK), 0,
+                           e->stream, levels(e), st);
       } else
         hipLaunchKernelGGL((k_stage<T, DIM, OP, 1>), dim3(grid_of(r)), dim3(BLOCK), 0,
                            e->stream, levels(e), st);
@@ -2141,12 +2142,13 @@ int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells
                  int64_t* fluid) {
   if (enable) {
     if (e->d_swept == nullptr) MG_HIP(hipMalloc(&e->d_swept, 2 * sizeof(unsigned long long)));
-    while ((int)e->probe_ev.size() < 2 * PROBE_POOL) {
-      hipEvent_t ev = nullptr;
-      MG_HIP(hipEventCreate(&ev));  // timing events
-      e->probe_ev.push_back(ev);
-    }
+    const size_t half = (size_t)PROBE_POOL * SPAN_SLOTS;
+    if (e->d_span == nullptr)
+      MG_HIP(hipMalloc(&e->d_span, 2 * half * sizeof(unsigned long long)));
     MG_HIP(hipMemsetAsync(e->d_swept, 0, 2 * sizeof(unsigned long long), e->stream));
+    // starts to the largest value (atomic min), ends to 0 (atomic max)
+    MG_HIP(hipMemsetAsync(e->d_span, 0xff, half * sizeof(unsigned long long), e->stream));
+    MG_HIP(hipMemsetAsync(e->d_span + half, 0, half * sizeof(unsigned long long), e->stream));
     e->probe_on = true;
     e->probe_used = 0;
     e->probe_seen = 0;
@@ -2155,10 +2157,25 @@ int engine_probe(Engine* e, int enable, int* n, double* total_ms, int64_t* cells
   e->probe_on = false;
   MG_HIP(hipStreamSynchronize(e->stream));
   double tot = 0.0;
-  for (int i = 0; i < e->probe_used; ++i) {
-    float ms = 0.0f;
-    MG_HIP(hipEventElapsedTime(&ms, e->probe_ev[2 * i], e->probe_ev[2 * i + 1]));
-    tot += ms;
+  if (e->probe_used > 0) {
+    const size_t n = (size_t)e->probe_used * SPAN_SLOTS;
+    std::vector<unsigned long long> t0(n), t1(n);
+    MG_HIP(hipMemcpy(t0.data(), e->d_span, n * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+    MG_HIP(hipMemcpy(t1.data(), e->d_span + (size_t)PROBE_POOL * SPAN_SLOTS,
+                     n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    int dev = 0, khz = 0;
+    MG_HIP(hipGetDevice(&dev));
+    MG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    MG_REQUIRE(khz > 0, "device clock rate unknown");
+    for (int i = 0; i < e->probe_used; ++i) {
+      unsigned long long a = ~0ull, b = 0ull;
+      for (int k = 0; k < SPAN_SLOTS; ++k) {
+        a = std::min(a, t0[(size_t)i * SPAN_SLOTS + k]);
+        b = std::max(b, t1[(size_t)i * SPAN_SLOTS + k]);
+      }
+      if (b > a) tot += (double)(b - a) / (double)khz;  // ms
+    }
   }
   unsigned long long c[2] = {0, 0};
   if (e->d_swept) MG_HIP(hipMemcpy(c, e->d_swept, sizeof(c), hipMemcpyDeviceToHost));
