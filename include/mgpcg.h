@@ -279,6 +279,14 @@ int mgpcg_magnetic_build_b(void* stream, int dim, const int32_t* res, const int8
  * *out_device; partials needs ceil(cells / 256) doubles of device memory. */
 int mgpcg_calc_volume(void* stream, int dim, const int32_t* res, const float* phi, double dx,
                       double* partials, int partials_cap, double* out_device);
+/* No counterpart in the reference: mgpcg_calc_volume without a partials
+ * buffer and without any allocation, for callers that cannot allocate
+ * (inside HIP graph capture, tight serving loops).  The per-block partial
+ * sums live in a file-scope device array of 2^18 doubles (2 MiB per GPU, held
+ * until the process ends), so grids of at most 2^26 cells; larger grids are
+ * rejected.  Calls that overlap on one GPU share that array and race. */
+int mgpcg_calc_volume_static(void* stream, int dim, const int32_t* res, const float* phi,
+                             double dx, double* out_device);
 /* add_c (volume_control.py:33-37): b[FLUID] += f32(c). */
 int mgpcg_add_c(void* stream, int dim, const int32_t* res, const int8_t* cell_type, double c,
                 float* b);

@@ -13,6 +13,7 @@
 #include <math.h>
 
 #include <cstring>
+#include <mutex>
 
 #include "../../include/mgpcg.h"
 #include "mgpcg_common.h"
@@ -836,6 +837,58 @@ extern "C" int mgpcg_calc_volume(void* stream, int dim, const int32_t* res, cons
   Shape s = make_shape(dim, res);
   const unsigned nb = nb_for(s.cells());
   MG_REQUIRE((long)partials_cap >= (long)nb, "partials buffer needs %u entries", nb);
+  Field f{phi, s};
+  hipLaunchKernelGGL(k_volume, dim3(nb), dim3(SB), 0, (hipStream_t)stream, s, f, make_ls(dx),
+                     partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(SB), 0, (hipStream_t)stream, partials,
+                     (int)nb, out_device);
+  MG_HIP(hipGetLastError());
+  return 0;
+}
+
+// The allocation-free variant of mgpcg_calc_volume keeps its per-block
+// partial sums in this file-scope array (one per 256 cells, so grids of up to
+// VOLUME_SCRATCH_PARTIALS * 256 cells).  Each GPU holds its own copy until the
+// process ends.  Host code reaches it only through the device address that
+// hipGetSymbolAddress returns, once per device.  Every entry a call reads was
+// written by the same call, so nothing needs clearing; calls that overlap on
+// one GPU share the array and race.
+constexpr int VOLUME_SCRATCH_PARTIALS = 1 << 18;
+static __device__ __align__(256) double scratch[VOLUME_SCRATCH_PARTIALS];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up on the device's first call.
+int scratch_ptr(double** out) {
+  int dev = 0;
+  MG_HIP(hipGetDevice(&dev));
+  MG_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    MG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MG_REQUIRE(p != nullptr, "no device address for the scratch array");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" int mgpcg_calc_volume_static(void* stream, int dim, const int32_t* res,
+                                        const float* phi, double dx, double* out_device) {
+  if (check_common(dim, res)) return -1;
+  MG_REQUIRE(phi && out_device, "NULL argument");
+  Shape s = make_shape(dim, res);
+  const unsigned nb = nb_for(s.cells());
+  MG_REQUIRE((long)nb <= (long)VOLUME_SCRATCH_PARTIALS,
+             "grid of %ld cells exceeds the allocation-free limit of %ld cells", (long)s.cells(),
+             (long)VOLUME_SCRATCH_PARTIALS * SB);
+  double* partials = nullptr;
+  if (scratch_ptr(&partials)) return -2;
   Field f{phi, s};
   hipLaunchKernelGGL(k_volume, dim3(nb), dim3(SB), 0, (hipStream_t)stream, s, f, make_ls(dx),
                      partials);

@@ -70,6 +70,7 @@ SIGNATURES = {
     "mgpcg_pressure_build_A": (_I, [_P, _I, _P, _I, _P, _P, _D, _I, _D, _P, _P]),
     "mgpcg_magnetic_build_b": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _D, _D, _D, _P]),
     "mgpcg_calc_volume": (_I, [_P, _I, _P, _P, _D, _P, _I, _P]),
+    "mgpcg_calc_volume_static": (_I, [_P, _I, _P, _P, _D, _P]),
     "mgpcg_add_c": (_I, [_P, _I, _P, _P, _D, _P]),
     "mgpcg_potential_build_b": (_I, [_P, _I, _P, _P, _P, _P]),
     "mgpcg_potential_build_A": (_I, [_P, _I, _P, _I, _D, _P, _P, _P]),

@@ -393,6 +393,10 @@ hipError_t hipFree(void* p) {
   hipemu::guarded_free(p);
   return hipSuccess;
 }
+hipError_t hipGetSymbolAddress(void** devPtr, const void* symbol) {
+  *devPtr = const_cast<void*>(symbol);
+  return symbol ? hipSuccess : hipErrorInvalidValue;
+}
 hipError_t hipMemset(void* p, int v, size_t bytes) {
   memset(p, v, bytes);  // NULL stream: not ordered with non-blocking streams
   return hipSuccess;

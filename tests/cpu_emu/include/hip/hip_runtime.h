@@ -25,6 +25,9 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+// a __device__ variable is a host variable here: its address is its "device" address
+#define HIP_SYMBOL(x) ((const void*)&(x))
 #define MG_GLOBAL
 #define MG_READLANE(v, l) (::hipemu::readlane_i((v), (l)))
 #define MG_KEEP(x) ((void)(x))
@@ -181,6 +184,7 @@ inline hipError_t hipMalloc(T** p, size_t bytes) {
 }
 hipError_t hipFree(void* p);
 hipError_t hipMemset(void* p, int v, size_t bytes);
+hipError_t hipGetSymbolAddress(void** devPtr, const void* symbol);
 hipError_t hipMemsetAsync(void* p, int v, size_t bytes, hipStream_t s);
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind k);
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind k,
