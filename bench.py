@@ -88,6 +88,10 @@ def parse():
                     help="iterations the server keeps queued on the device ahead of the host")
     ap.add_argument("--server-streams", type=int, default=1,
                     help="engine streams the server spreads the requests in flight over")
+    ap.add_argument("--canonical-assembly", action="store_true",
+                    help="assemble pressure systems through the canonical fields and load "
+                         "them (the reference's route) instead of straight into the packed "
+                         "hierarchy")
     ap.add_argument("--baseline-steps", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -199,7 +203,8 @@ def measure_served(args, workload, n, L, seeds, concurrency, steps, warmup, dev,
     if streams is not None:
         _lib.check(lib.mgpcg_set_option(None, _lib.OPT_SERVER_STREAMS, streams))
     srv = serve.Server(64 << 30, precision=args.precision, n_levels=L, rel_tol=rel_tol,
-                       admit_per_step=args.admit_per_step or None, device=dev)
+                       admit_per_step=args.admit_per_step or None,
+                       packed=not args.canonical_assembly, device=dev)
     if streams is not None:
         _lib.check(lib.mgpcg_set_option(None, _lib.OPT_SERVER_STREAMS, args.server_streams))
     mk = (lambda i: scenes_dev[i % len(scenes_dev)])
@@ -661,6 +666,7 @@ def main():
                    "n": n, "levels": L, "seeds": args.seeds, "concurrency": args.concurrency,
                    "admit_per_step": args.admit_per_step, "server_depth": args.server_depth,
                    "server_streams": args.server_streams,
+                   "assembly": "canonical" if args.canonical_assembly else "packed",
                    **({"copy_calibration_mib": args.copy_calibration_mib}
                       if args.copy_calibration_mib else {})},
         **({"autotune": {"knobs": tuned["knobs"], "solves_per_s": tuned["solves_per_s"],

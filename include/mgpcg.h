@@ -115,6 +115,21 @@ int mgpcg_pool_bytes(int dim, const int32_t* res, int n_levels, int precision, i
  * full_reinitialize (mgpcg.py:79-94). */
 int mgpcg_load_level(mgpcg_solver_t* solver, int level, const int8_t* grid_type,
                      const float* Adiag, const float* Ax);
+/* PressureProjectStrategy's reinitialize (pressure_project.py:15-64 via
+ * mgpcg.py:79-87) straight into the device hierarchy: b and every level's
+ * operator are computed in the packed layout, bit for bit what
+ * mgpcg_pressure_build_b / _A, mgpcg_coarsen, mgpcg_load_level and
+ * mgpcg_load_rhs produce, without the canonical b, Adiag and Ax.
+ * grid_type[l]: canonical int8 cell types per level; level 0 is read,
+ * levels >= 1 are written (coarsened, mgpcg.py:46-60).  velocity[k]: MAC
+ * component k (res + e_k); phi: the level set (ghost-fluid weights at level 0
+ * when ghost_fluid_method).  add_c != 0 adds f32(c) to b on FLUID cells
+ * (PressureProjectWithVolumeControlStrategy, volume_control.py:33-37).
+ * Ordered on the handle's stream. */
+int mgpcg_assemble_pressure(mgpcg_solver_t* solver, int8_t* const* grid_type,
+                            const float* const* velocity, const float* phi, double scale_A,
+                            double scale_b, double p0, int ghost_fluid_method, double c_max,
+                            int add_c, double c);
 /* Whether every nonzero off-diagonal coefficient of a loaded level has one
  * value (then *coef is it, or 0 when the level has no coupling); waits for
  * the handle's stream.  No reference counterpart: inspection only. */

@@ -2304,6 +2304,11 @@ __device__ __forceinline__ long canon_index(const Geo& g, int i, int j, int k) {
   return ((long)i * g.n1 + j) * g.n2 + k;
 }
 
+// the MAC velocity components of a level-0 grid (component k: res + e_k)
+struct MacVel {
+  const float* u[3];
+};
+
 __global__ void k_meta_init(LevelMeta* m) {
   if (threadIdx.x == 0) {
     m->minbits = 0xffffffffu;
@@ -2339,99 +2344,82 @@ __device__ __forceinline__ bool same_bits(T a, T b) {
   return true;
 }
 
-// Packs one level from the canonical arrays, computes every slot's flag word
-// (coupling flags and diagonal code), and folds the level's nonzero coupling
-// bit patterns into *meta.  A slot's diagonal code is checked against the
+// One slot of one colour in the packed layout: its cell type, diagonal and
+// couplings (upper and, where the lower neighbour exists, lower per axis), as
+// the level's canonical arrays hold them; a ghost slot is SOLID with zeros.
+template <int DIM>
+struct SlotCoefs {
+  bool inn;
+  bool hdn[DIM];
+  int8_t gv;
+  float ad, up[DIM], dn[DIM];
+};
+
+// Writes slot t of colour c from its coefficients: cell type, diagonal,
+// couplings, the flag word (coupling flags and diagonal code), zero r, z and
+// residual scratch; folds its nonzero upper couplings into mn / mx, and counts
+// a FLUID cell into wet / nfl.  A slot's diagonal code is checked against the
 // first nonzero coupling of the slot itself: on a level whose couplings turn
 // out uniform that is the level's value, and the kernels read codes only on
 // such levels.
 template <typename T, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __restrict__ gt,
-                                                      const float* __restrict__ Adiag,
-                                                      const float* __restrict__ Ax,
-                                                      LevelPtrs<T> P, LevelMeta* meta) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
-  uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
-  if (t < g.size) {
-    // every canonical value of both colours is loaded first (ghost slots and
-    // missing lower neighbours read cell 0 and discard it)
-    bool inn[2], hdn[2][DIM];
-    int8_t gv[2];
-    float adv[2], up[2][DIM], dn[2][DIM];
+__device__ __forceinline__ void pack_slot(const SlotCoefs<DIM>& v, int c, int t,
+                                          const LevelPtrs<T>& P, uint32_t& mn, uint32_t& mx,
+                                          uint32_t& wet, uint32_t& nfl) {
+  int8_t gvc = MG_SOLID;
+  T ad = (T)0, a[3] = {(T)0, (T)0, (T)0};
+  T cl = (T)0;  // the slot's first nonzero coupling
+  bool have = false;
+  uint32_t f = 0u;
+  if (v.inn) {
+    gvc = v.gv;
+    ad = (T)v.ad;
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const Slot sl = decode(g, t, c);
-      inn[c] = sl.interior;
-      const long ci = sl.interior ? canon_index(g, sl.i, sl.j, sl.k) : 0;
-      const int v[3] = {sl.i, sl.j, sl.k};
-      long wi[DIM];
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        // lower neighbour on original axis k (view axis k + 3 - DIM)
-        const int va = k + 3 - DIM;
-        hdn[c][k] = sl.interior && v[va] > 0;
-        int w[3] = {sl.i, sl.j, sl.k};
-        w[va] -= 1;
-        wi[k] = hdn[c][k] ? canon_index(g, w[0], w[1], w[2]) : 0;
+    for (int k = 0; k < DIM; ++k) {
+      a[k] = (T)v.up[k];
+      const uint32_t ub = __float_as_uint(v.up[k]);
+      if (ub != 0u) {
+        f |= 1u << k;
+        mn = ub < mn ? ub : mn;
+        mx = ub > mx ? ub : mx;
+        if (!have) cl = (T)v.up[k];
+        have = true;
       }
-      gv[c] = gt[ci];
-      adv[c] = Adiag[ci];
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        up[c][k] = Ax[ci * DIM + k];
-        dn[c][k] = Ax[wi[k] * DIM + k];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      int8_t gvc = MG_SOLID;
-      T ad = (T)0, a[3] = {(T)0, (T)0, (T)0};
-      T cl = (T)0;  // the slot's first nonzero coupling
-      bool have = false;
-      uint32_t f = 0u;
-      if (inn[c]) {
-        gvc = gv[c];
-        ad = (T)adv[c];
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          a[k] = (T)up[c][k];
-          const uint32_t ub = __float_as_uint(up[c][k]);
-          if (ub != 0u) {
-            f |= 1u << k;
-            mn = ub < mn ? ub : mn;
-            mx = ub > mx ? ub : mx;
-            if (!have) cl = (T)up[c][k];
-            have = true;
-          }
-          if (hdn[c][k] && __float_as_uint(dn[c][k]) != 0u) {
-            f |= 1u << (NB_DN + k);
-            if (!have) cl = (T)dn[c][k];
-            have = true;
-          }
-        }
-      }
-      ((int8_t*)P.gt[c])[t] = gvc;
-      ((T*)P.Adiag[c])[t] = ad;
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) ((T*)P.Ax[k][c])[t] = a[k];
-      uint32_t code = NB_AD_STORED;
-      for (uint32_t n = 0; n < NB_AD_STORED; ++n) {
-        if (n > 0 && !have) break;
-        if (same_bits(diag_of_code<T>(n, cl), ad)) {
-          code = n;
-          break;
-        }
-      }
-      ((uint16_t*)P.nb[c])[t] = (uint16_t)(f | (code << NB_AD));
-      P.r[c][t] = (T)0;
-      P.z[c][t] = (T)0;
-      if (P.res[c] != nullptr) P.res[c][t] = (T)0;
-      if (gvc == MG_FLUID) {
-        wet = 1u;
-        nfl += 1u;
+      if (v.hdn[k] && __float_as_uint(v.dn[k]) != 0u) {
+        f |= 1u << (NB_DN + k);
+        if (!have) cl = (T)v.dn[k];
+        have = true;
       }
     }
   }
+  ((int8_t*)P.gt[c])[t] = gvc;
+  ((T*)P.Adiag[c])[t] = ad;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) ((T*)P.Ax[k][c])[t] = a[k];
+  uint32_t code = NB_AD_STORED;
+  for (uint32_t n = 0; n < NB_AD_STORED; ++n) {
+    if (n > 0 && !have) break;
+    if (same_bits(diag_of_code<T>(n, cl), ad)) {
+      code = n;
+      break;
+    }
+  }
+  ((uint16_t*)P.nb[c])[t] = (uint16_t)(f | (code << NB_AD));
+  P.r[c][t] = (T)0;
+  P.z[c][t] = (T)0;
+  if (P.res[c] != nullptr) P.res[c][t] = (T)0;
+  if (gvc == MG_FLUID) {
+    wet = 1u;
+    nfl += 1u;
+  }
+}
+
+// End of a packing workgroup: its block's FLUID flag and natural-order work
+// entry, and the level's coupling extremes and FLUID count.  Called by the
+// whole workgroup.
+__device__ __forceinline__ void pack_block_tail(uint32_t mn, uint32_t mx, uint32_t wet,
+                                                uint32_t nfl, uint8_t* wet_out, int32_t* ord0,
+                                                LevelMeta* meta) {
   __shared__ uint32_t smn[BLOCK / 64], smx[BLOCK / 64], swet[BLOCK / 64], snfl[BLOCK / 64];
   mn = wave_min_u32(mn);
   mx = wave_max_u32(mx);
@@ -2453,9 +2441,9 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
       nfl += snfl[w];
     }
     if (nfl != 0u) atomicAdd(&meta->nfluid, nfl);
-    if (P.wet != nullptr) ((uint8_t*)P.wet)[blockIdx.x] = (uint8_t)wet;
-    if (P.ord[0] != nullptr)  // per-block work order: natural, with the wet bit
-      ((int32_t*)P.ord[0])[blockIdx.x] = (int32_t)blockIdx.x | (wet ? ORD_WET : 0);
+    if (wet_out != nullptr) wet_out[blockIdx.x] = (uint8_t)wet;
+    if (ord0 != nullptr)  // per-block work order: natural, with the wet bit
+      ord0[blockIdx.x] = (int32_t)blockIdx.x | (wet ? ORD_WET : 0);
     // a value that would not move the level's current extreme skips the
     // atomic: with one coupling value on the level (the usual case) only the
     // first workgroups update it, instead of every workgroup queueing on the
@@ -2466,6 +2454,243 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
       if (mn < cur_mn) atomicMin(&meta->minbits, mn);
       if (mx > cur_mx) atomicMax(&meta->maxbits, mx);
     }
+  }
+}
+
+// Packs one level from the canonical arrays (the strategies' output).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __restrict__ gt,
+                                                      const float* __restrict__ Adiag,
+                                                      const float* __restrict__ Ax,
+                                                      LevelPtrs<T> P, LevelMeta* meta) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
+  if (t < g.size) {
+    // every canonical value of both colours is loaded first (ghost slots and
+    // missing lower neighbours read cell 0 and discard it)
+    SlotCoefs<DIM> v[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const Slot sl = decode(g, t, c);
+      v[c].inn = sl.interior;
+      const long ci = sl.interior ? canon_index(g, sl.i, sl.j, sl.k) : 0;
+      const int x[3] = {sl.i, sl.j, sl.k};
+      long wi[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        // lower neighbour on original axis k (view axis k + 3 - DIM)
+        const int va = k + 3 - DIM;
+        v[c].hdn[k] = sl.interior && x[va] > 0;
+        int w[3] = {sl.i, sl.j, sl.k};
+        w[va] -= 1;
+        wi[k] = v[c].hdn[k] ? canon_index(g, w[0], w[1], w[2]) : 0;
+      }
+      v[c].gv = gt[ci];
+      v[c].ad = Adiag[ci];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        v[c].up[k] = Ax[ci * DIM + k];
+        v[c].dn[k] = Ax[wi[k] * DIM + k];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) pack_slot<T, DIM>(v[c], c, t, P, mn, mx, wet, nfl);
+  }
+  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], meta);
+}
+
+// The pressure operator of one level assembled straight into the packed
+// layout: per cell, PressureProjectStrategy.build_A_kernel
+// (pressure_project.py:45-64) -- for each axis, lower then upper neighbour:
+// FLUID adds sA to the diagonal (and sets the upper coupling to -sA), AIR adds
+// sA, or sA * c with the ghost-fluid weight c at level 0, SOLID or out of
+// range adds nothing -- in the same operation order as the canonical
+// assembly, so the packed values equal k_pack_level's of that assembly bit for
+// bit.  A cell's lower coupling on axis k is the lower neighbour's upper one:
+// -sA when both are FLUID.  gt: the level's canonical cell types; phi: the
+// level set (ghost-fluid levels only, else NULL).
+__device__ __forceinline__ float gfm_weight(float phi_i, float phi_j, float c_max) {
+  const float c = (phi_i - phi_j) / phi_i;
+  return phi_i == 0.0f ? c_max : fminf(c, c_max);
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_pack_pressure(Geo g, const int8_t* __restrict__ gt,
+                                                         const float* __restrict__ phi,
+                                                         float sA, float c_max, LevelPtrs<T> P,
+                                                         LevelMeta* meta) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
+  if (t < g.size) {
+    SlotCoefs<DIM> v[2];
+    int8_t nbt[2][DIM][2];
+    float nbp[2][DIM][2], ph[2];
+    // every load of both colours first (out-of-range neighbours read the
+    // cell itself and count as SOLID)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const Slot sl = decode(g, t, c);
+      v[c].inn = sl.interior;
+      const long ci = sl.interior ? canon_index(g, sl.i, sl.j, sl.k) : 0;
+      const int x[3] = {sl.i, sl.j, sl.k};
+      const int nn[3] = {g.n0, g.n1, g.n2};
+      v[c].gv = gt[ci];
+      ph[c] = phi ? phi[ci] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        const int va = k + 3 - DIM;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          int w[3] = {sl.i, sl.j, sl.k};
+          w[va] += h ? 1 : -1;
+          const bool in = sl.interior && w[va] >= 0 && w[va] < nn[va];
+          const long wi = in ? canon_index(g, w[0], w[1], w[2]) : ci;
+          const int8_t tv = gt[wi];
+          nbt[c][k][h] = in ? tv : (int8_t)MG_SOLID;
+          nbp[c][k][h] = phi ? phi[wi] : 0.0f;
+        }
+        v[c].hdn[k] = sl.interior && x[va] > 0;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      float ad = 0.0f, up[3] = {0.0f, 0.0f, 0.0f};
+      const bool fl = v[c].inn && v[c].gv == MG_FLUID;
+      if (fl) {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {  // sgn = -1, then +1
+            const int nb = nbt[c][k][h];
+            if (nb == MG_FLUID) {
+              ad = ad + sA;
+              if (h) up[k] = -sA;
+            } else if (nb == MG_AIR) {
+              if (phi != nullptr) {
+                const float cw = gfm_weight(ph[c], nbp[c][k][h], c_max);
+                ad = ad + sA * cw;
+              } else {
+                ad = ad + sA;
+              }
+            }
+          }
+        }
+      }
+      v[c].ad = ad;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        v[c].up[k] = up[k];
+        v[c].dn[k] = (fl && nbt[c][k][0] == MG_FLUID) ? -sA : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) pack_slot<T, DIM>(v[c], c, t, P, mn, mx, wet, nfl);
+  }
+  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], meta);
+}
+
+// The pressure right-hand side straight into the packed b (both colours of a
+// slot per thread): PressureProjectStrategy.build_b_kernel
+// (pressure_project.py:15-39) in the operation order of the canonical
+// assembly -- the velocity divergence times sb, SOLID faces removing their
+// velocity term, AIR faces adding sA * p0 (times the ghost-fluid weight when
+// phi is given) -- then, when add_c, the volume-control constant c added on
+// FLUID cells (volume_control.py:33-37).  u[k]: MAC component k, shape
+// res + e_k (canonical).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_pack_pressure_b(Geo g, const int8_t* __restrict__ gt,
+                                                           MacVel V,
+                                                           const float* __restrict__ phi,
+                                                           float sA, float sb, float p0,
+                                                           float c_max, int add_c, float cadd,
+                                                           T* b0, T* b1) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= g.size) return;
+  float out[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const Slot sl = decode(g, t, c);
+    out[c] = 0.0f;
+    if (!sl.interior) continue;
+    const long ci = canon_index(g, sl.i, sl.j, sl.k);
+    if (gt[ci] != MG_FLUID) continue;
+    const int x[3] = {sl.i, sl.j, sl.k};
+    const int nn[3] = {g.n0, g.n1, g.n2};
+    float ulo[DIM], uhi[DIM];
+    int nbt[DIM][2];
+    float nbp[DIM][2];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const int va = k + 3 - DIM;
+      // MAC component k has one more entry on axis k: index I and I + e_k
+      int e[3] = {g.n0, g.n1, g.n2};
+      e[va] += 1;
+      const long base = ((long)(DIM == 3 ? x[0] : 0) * e[1] + x[1]) * e[2] + x[2];
+      const long step = va == 2 ? 1 : (va == 1 ? (long)e[2] : (long)e[1] * e[2]);
+      ulo[k] = V.u[k][base];
+      uhi[k] = V.u[k][base + step];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        int w[3] = {sl.i, sl.j, sl.k};
+        w[va] += h ? 1 : -1;
+        const bool in = w[va] >= 0 && w[va] < nn[va];
+        const long wi = in ? canon_index(g, w[0], w[1], w[2]) : ci;
+        const int8_t tv = gt[wi];
+        nbt[k][h] = in ? tv : MG_SOLID;
+        nbp[k][h] = phi ? phi[wi] : 0.0f;
+      }
+    }
+    const float phi_t = phi ? phi[ci] : 0.0f;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      float d = ulo[k] - uhi[k];
+      acc = acc + d;
+    }
+    float bv = acc * sb;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {  // sgn = -1, then +1
+        const int nb = nbt[k][h];
+        if (nb == MG_SOLID) {
+          if (!h) {
+            float u = ulo[k] - 0.0f;
+            bv = bv - sb * u;
+          } else {
+            float u = uhi[k] - 0.0f;
+            bv = bv + sb * u;
+          }
+        } else if (nb == MG_AIR) {
+          float term;
+          if (phi != nullptr) {
+            float cw = gfm_weight(phi_t, nbp[k][h], c_max);
+            term = (sA * cw) * p0;
+          } else {
+            term = sA * p0;
+          }
+          bv = bv + term;
+        }
+      }
+    }
+    if (add_c) bv = bv + cadd;
+    out[c] = bv;
+  }
+  b0[t] = (T)out[0];
+  b1[t] = (T)out[1];
+}
+
+// every level's metadata reset (one launch before a hierarchy is packed)
+struct MetaList {
+  LevelMeta* m[16];
+  int n;
+};
+__global__ void k_meta_init_all(MetaList ml) {
+  const int l = threadIdx.x;
+  if (l < ml.n) {
+    ml.m[l]->minbits = 0xffffffffu;
+    ml.m[l]->maxbits = 0u;
+    ml.m[l]->nfluid = 0u;
   }
 }
 

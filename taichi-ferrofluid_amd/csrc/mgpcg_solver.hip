@@ -1501,6 +1501,71 @@ extern "C" int mgpcg_load_rhs(mgpcg_solver_t* s, const float* b) {
   return 0;
 }
 
+namespace {
+template <typename T>
+int assemble_pressure_t(mgpcg_solver* s, int8_t* const* gt, const float* const* velocity,
+                        const float* phi, float sA, float sb, float p0, int gfm, float c_max,
+                        int add_c, float cadd) {
+  hipStream_t st = s->user_stream;
+  MetaList ml;
+  ml.n = s->L;
+  for (int l = 0; l < s->L; ++l) ml.m[l] = s->lv[l].meta;
+  hipLaunchKernelGGL(k_meta_init_all, dim3(1), dim3(64), 0, st, ml);
+  MacVel V;
+  for (int k = 0; k < 3; ++k) V.u[k] = k < s->dim ? velocity[k] : nullptr;
+  const Geo& g0 = s->geo[0];
+  const float* ph = gfm ? phi : nullptr;
+  if (s->dim == 3)
+    hipLaunchKernelGGL((k_pack_pressure_b<T, 3>), dim3(nblk(g0.size)), dim3(BLOCK), 0, st, g0,
+                       gt[0], V, ph, sA, sb, p0, c_max, add_c, cadd, (T*)s->b[0], (T*)s->b[1]);
+  else
+    hipLaunchKernelGGL((k_pack_pressure_b<T, 2>), dim3(nblk(g0.size)), dim3(BLOCK), 0, st, g0,
+                       gt[0], V, ph, sA, sb, p0, c_max, add_c, cadd, (T*)s->b[0], (T*)s->b[1]);
+  for (int l = 0; l < s->L; ++l) {
+    const Geo& g = s->geo[l];
+    if (l > 0) {
+      const Geo& gf = s->geo[l - 1];
+      const int nc = (s->dim == 3 ? g.n0 : 1) * g.n1 * g.n2;
+      hipLaunchKernelGGL(k_coarsen, dim3(nblk(nc)), dim3(BLOCK), 0, st, s->dim,
+                         s->dim == 3 ? gf.n0 : 1, gf.n1, gf.n2, gt[l - 1], gt[l], nc);
+    }
+    const float* phl = l == 0 ? ph : nullptr;  // ghost-fluid weights at level 0 only
+    if (s->dim == 3)
+      hipLaunchKernelGGL((k_pack_pressure<T, 3>), dim3(nblk(g.size)), dim3(BLOCK), 0, st, g,
+                         gt[l], phl, sA, c_max, level_ptrs<T>(s, l), s->lv[l].meta);
+    else
+      hipLaunchKernelGGL((k_pack_pressure<T, 2>), dim3(nblk(g.size)), dim3(BLOCK), 0, st, g,
+                         gt[l], phl, sA, c_max, level_ptrs<T>(s, l), s->lv[l].meta);
+  }
+  if (s->coarse_from > 0) {
+    const LevelBuf& B = s->lv[0];
+    hipLaunchKernelGGL(k_level_order, dim3(1), dim3(ORDER_THREADS), 0, st, B.wet,
+                       nblk(g0.size), B.ord[1], B.ordtmp);
+  }
+  MG_HIP(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+extern "C" int mgpcg_assemble_pressure(mgpcg_solver_t* s, int8_t* const* grid_type,
+                                       const float* const* velocity, const float* phi,
+                                       double scale_A, double scale_b, double p0,
+                                       int ghost_fluid_method, double c_max, int add_c,
+                                       double c) {
+  MG_REQUIRE(s != nullptr, "solver is NULL");
+  MG_REQUIRE(grid_type != nullptr && velocity != nullptr, "NULL argument");
+  for (int l = 0; l < s->L; ++l) MG_REQUIRE(grid_type[l] != nullptr, "grid_type[%d] is NULL", l);
+  for (int k = 0; k < s->dim; ++k) MG_REQUIRE(velocity[k], "velocity[%d] is NULL", k);
+  MG_REQUIRE(!ghost_fluid_method || phi, "ghost-fluid method needs phi");
+  MG_HIP(hipSetDevice(s->device));
+  const float sA = (float)scale_A, sb = (float)scale_b, fp0 = (float)p0, cm = (float)c_max;
+  if (s->precision == F64)
+    return assemble_pressure_t<double>(s, grid_type, velocity, phi, sA, sb, fp0,
+                                       ghost_fluid_method, cm, add_c, (float)c);
+  return assemble_pressure_t<float>(s, grid_type, velocity, phi, sA, sb, fp0, ghost_fluid_method,
+                                    cm, add_c, (float)c);
+}
+
 extern "C" int mgpcg_solve(mgpcg_solver_t* s, int max_iters, double rel_tol, double abs_tol,
                            double eps, int refine_every, mgpcg_result_t* out) {
   MG_REQUIRE(s != nullptr, "solver is NULL");

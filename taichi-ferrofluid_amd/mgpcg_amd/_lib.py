@@ -50,6 +50,7 @@ SIGNATURES = {
     "mgpcg_set_option": (_I, [_P, _I, _I]),
     "mgpcg_get_option": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "mgpcg_info": (_I, [_P, _I, _P, _P]),
+    "mgpcg_assemble_pressure": (_I, [_P, _P, _P, _P, _D, _D, _D, _I, _D, _I, _D]),
     "mgpcg_pool_bytes": (_I, [_I, _P, _I, _I, _P]),
     "mgpcg_load_level": (_I, [_P, _I, _P, _P, _P]),
     "mgpcg_load_rhs": (_I, [_P, _P]),
@@ -157,7 +158,9 @@ def ptr(t):
 
 
 def ptr_array(tensors):
-    return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+    """A C array of the tensors' addresses (None -> NULL)."""
+    return (ctypes.c_void_p * len(tensors))(*[None if t is None else t.data_ptr()
+                                              for t in tensors])
 
 
 def int3(vals):

@@ -103,6 +103,7 @@ class MGPCGPoissonSolver:
                                           pre_and_post_smoothing, bottom_smoothing,
                                           PRECISIONS[precision], ctypes.byref(h)))
         self._h = h
+        self._packed = False
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -132,6 +133,7 @@ class MGPCGPoissonSolver:
 
     def reinitialize(self, cell_type, strategy):
         """mgpcg.py:79-87.  ``cell_type`` is an int tensor of shape res."""
+        self._packed = False
         if not getattr(strategy, "writes_all_cells", False):
             self.initialize()
         self.grid_type[0].copy_(cell_type.to(self.grid_type[0].device))  # i32 -> i8
@@ -141,8 +143,20 @@ class MGPCGPoissonSolver:
             self._coarsen(l)
             strategy.build_A(self, l)
 
+    def reinitialize_packed(self, cell_type, strategy):
+        """reinitialize for a strategy with ``assemble_packed`` (the pressure
+        systems): b and every level's operator go straight into the device
+        hierarchy, the same bits as reinitialize followed by load.  The
+        canonical ``b``, ``Adiag`` and ``Ax`` are left as they were; the
+        coarse levels' ``grid_type`` is written.  A later reinitialize or
+        full_reinitialize returns to the canonical route."""
+        self.grid_type[0].copy_(cell_type.to(self.grid_type[0].device))  # i32 -> i8
+        strategy.assemble_packed(self)
+        self._packed = True
+
     def full_reinitialize(self, strategy):
         """mgpcg.py:89-94: every level all FLUID."""
+        self._packed = False
         if not getattr(strategy, "writes_all_cells", False):
             self.initialize()
         strategy.build_b(self)
@@ -151,7 +165,10 @@ class MGPCGPoissonSolver:
             strategy.build_A(self, l)
 
     def load(self):
-        """Hand the canonical fields to the device hierarchy."""
+        """Hand the canonical fields to the device hierarchy (nothing to do
+        after reinitialize_packed, which wrote the hierarchy itself)."""
+        if self._packed:
+            return
         h = self._h
         _lib.check(self._lib.mgpcg_set_stream(h, _lib.stream_handle(self.device)))
         for l in range(self.n_mg_levels):

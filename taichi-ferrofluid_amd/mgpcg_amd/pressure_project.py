@@ -8,6 +8,11 @@ tensors (shape res + e_k), ``phi`` the float32 level set [*res].
 
 ``c_max`` is the ghost-fluid clamp ``min(c, 1e3)`` of pressure_project.py:37,62
 (1e3 in the reference; a clamp theta >= theta_min is c_max = 1 / theta_min).
+
+``assemble_packed`` is the serving path's route to the same system: b and
+every level's operator computed straight into the solver's device hierarchy
+(mgpcg_assemble_pressure), bit for bit what reinitialize + load produce,
+without writing the canonical ``b``, ``Adiag`` and ``Ax`` fields.
 """
 from . import _lib
 
@@ -33,6 +38,24 @@ class PressureProjectStrategy:
             _lib.ptr(solver.grid_type[0]), _lib.ptr_array(self.velocity), _lib.ptr(self.phi),
             float(self.scale_A), float(self.scale_b), float(self.p0),
             int(bool(self.ghost_fluid_method)), float(self.c_max), _lib.ptr(solver.b)))
+
+    def packed_c(self):
+        """The divergence constant added to b on FLUID cells, or None (none
+        for the plain pressure system)."""
+        return None
+
+    def assemble_packed(self, solver):
+        """b and every level's operator straight into the device hierarchy
+        (solver.grid_type[0] holds the cell types; coarser levels' are
+        written here)."""
+        c = self.packed_c()
+        lib = _lib.load()
+        _lib.check(lib.mgpcg_set_stream(solver._h, _lib.stream_handle(solver.device)))
+        _lib.check(lib.mgpcg_assemble_pressure(
+            solver._h, _lib.ptr_array(solver.grid_type), _lib.ptr_array(self.velocity),
+            _lib.ptr(self.phi) if self.phi is not None else None, float(self.scale_A),
+            float(self.scale_b), float(self.p0), int(bool(self.ghost_fluid_method)),
+            float(self.c_max), int(c is not None), float(c or 0.0)))
 
     def build_A(self, solver, level):
         """build_A_kernel (pressure_project.py:45-64)."""

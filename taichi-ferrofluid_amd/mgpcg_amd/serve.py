@@ -4,9 +4,10 @@ A request is a scene (scenes.py) to assemble and solve.  The loop:
   * admits queued requests while the HBM budget allows, onto a resident
     hierarchy of the request's shape (hierarchies are kept on the device and
     reused; a shape that does not fit evicts idle hierarchies of other shapes);
-  * assembles the admitted system on the caller's stream (strategy kernels)
-    while the engine's stream iterates the systems already running, so the
-    two overlap on the device;
+  * assembles the admitted system on the caller's stream (strategy kernels;
+    pressure systems straight into the packed hierarchy unless
+    ``packed=False``) while the engine's stream iterates the systems already
+    running, so the two overlap on the device;
   * calls mgpcg_server_step: the new systems join at the next iteration
     boundary, one iteration of every member is enqueued, and members the
     device reports done (one iteration behind) retire;
@@ -30,7 +31,7 @@ from . import systems
 class Server:
     def __init__(self, hbm_budget_bytes, precision="mixed", n_levels=5, nu=2, bottom=10,
                  max_iters=500, rel_tol=1e-12, abs_tol=1e-14, eps=1e-12, refine_every=10,
-                 max_active=None, admit_per_step=None, device="cuda:0"):
+                 max_active=None, admit_per_step=None, packed=True, device="cuda:0"):
         self.device = torch.device(device)
         self.budget = int(hbm_budget_bytes)
         self.precision = precision
@@ -41,6 +42,7 @@ class Server:
         # fits in the iteration the device is running, and a cohort that
         # finished together re-enters staggered
         self.admit_per_step = admit_per_step
+        self.packed = packed
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         _lib.check(self._lib.mgpcg_server_create(int(max_iters), float(rel_tol), float(abs_tol),
@@ -120,7 +122,7 @@ class Server:
                     raise RuntimeError("request does not fit the HBM budget even alone")
                 break
             self.queue.popleft()
-            systems.assemble(s, req["scene"])
+            systems.assemble(s, req["scene"], packed=self.packed)
             s.load()
             _lib.check(self._lib.mgpcg_set_stream(s._h, _lib.stream_handle(self.device)))
             _lib.check(self._lib.mgpcg_server_submit(self._h, s._h, req["tag"]))

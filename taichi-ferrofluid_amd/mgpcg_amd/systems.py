@@ -33,6 +33,9 @@ class SeededVolumeControl(PressureProjectWithVolumeControlStrategy):
         PressureProjectStrategy.build_b(self, solver)
         self.add_c(solver, self.c)
 
+    def packed_c(self):
+        return self.c
+
 
 def upload(sysd, device):
     """Device copies of every array field of a scene."""
@@ -47,8 +50,12 @@ def upload(sysd, device):
     return out
 
 
-def assemble(solver, dev):
-    """Build b and the operator hierarchy of scene ``dev`` (uploaded) in ``solver``."""
+def assemble(solver, dev, packed=False):
+    """Build b and the operator hierarchy of scene ``dev`` (uploaded) in ``solver``.
+
+    ``packed`` assembles the pressure kinds straight into the device hierarchy
+    (reinitialize_packed); the other kinds take the canonical route either way.
+    """
     kind = dev["kind"]
     dim, res = dev["dim"], dev["res"]
     if kind in ("pressure", "volume"):
@@ -61,7 +68,10 @@ def assemble(solver, dev):
                                      dev["dt"], c_max=dev["c_max"], c=dev["c"])
         st.scale_A = dev["scale_A"]
         st.scale_b = dev["scale_b"]
-        solver.reinitialize(dev["cell_type"], st)
+        if packed:
+            solver.reinitialize_packed(dev["cell_type"], st)
+        else:
+            solver.reinitialize(dev["cell_type"], st)
         return st
     if kind == "psi":
         st = PotentialFunctionStrategy(dim, res, dev["dx"], dev["H_ext"], dev["chi"])

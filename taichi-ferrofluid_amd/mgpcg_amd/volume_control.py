@@ -56,10 +56,10 @@ class PressureProjectWithVolumeControlStrategy(PressureProjectStrategy):
                                    _lib.int3(solver.res), _lib.ptr(solver.grid_type[0]), float(c),
                                    _lib.ptr(solver.b)))
 
-    def build_b(self, solver):
-        """volume_control.py:39-51."""
+    def _controller_c(self):
+        """One controller step (volume_control.py:39-50): the constant to add,
+        or None on the first call, which records the target volume."""
         self.step += 1
-        super().build_b(solver)
         vol = self.calc_volume()
         if self.step == 1:
             self.vol_0 = vol
@@ -74,4 +74,14 @@ class PressureProjectWithVolumeControlStrategy(PressureProjectStrategy):
         self.last_c = c
         if self.verbose:
             print(f"volume control: step {self.step}, volume {vol:.6g} cells, c = {c:.6g}")
-        self.add_c(solver, c)
+        return c
+
+    def build_b(self, solver):
+        """volume_control.py:39-51."""
+        super().build_b(solver)
+        c = self._controller_c()
+        if c is not None:
+            self.add_c(solver, c)
+
+    def packed_c(self):
+        return self._controller_c()
