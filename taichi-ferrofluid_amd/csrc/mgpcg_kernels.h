@@ -2309,14 +2309,6 @@ struct MacVel {
   const float* u[3];
 };
 
-__global__ void k_meta_init(LevelMeta* m) {
-  if (threadIdx.x == 0) {
-    m->minbits = 0xffffffffu;
-    m->maxbits = 0u;
-    m->nfluid = 0u;
-  }
-}
-
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -2415,11 +2407,16 @@ __device__ __forceinline__ void pack_slot(const SlotCoefs<DIM>& v, int c, int t,
 }
 
 // End of a packing workgroup: its block's FLUID flag and natural-order work
-// entry, and the level's coupling extremes and FLUID count.  Called by the
-// whole workgroup.
+// entry, and its share of the level's summary in `part`, a structure of
+// arrays over the nb = gridDim.x workgroups: [0, nb) the smallest and
+// [nb, 2 nb) the largest nonzero upper-coupling bit pattern, [2 nb, 3 nb) the
+// FLUID cells.  k_meta_reduce folds them into the level's LevelMeta.  (One
+// device-scope atomic per workgroup on the level's counters instead costs a
+// level-0 packing over 100 us: atomics on one address from thousands of
+// workgroups serialise.)  Called by the whole workgroup.
 __device__ __forceinline__ void pack_block_tail(uint32_t mn, uint32_t mx, uint32_t wet,
                                                 uint32_t nfl, uint8_t* wet_out, int32_t* ord0,
-                                                LevelMeta* meta) {
+                                                uint32_t* part) {
   __shared__ uint32_t smn[BLOCK / 64], smx[BLOCK / 64], swet[BLOCK / 64], snfl[BLOCK / 64];
   mn = wave_min_u32(mn);
   mx = wave_max_u32(mx);
@@ -2440,20 +2437,55 @@ __device__ __forceinline__ void pack_block_tail(uint32_t mn, uint32_t mx, uint32
       wet = swet[w] > wet ? swet[w] : wet;
       nfl += snfl[w];
     }
-    if (nfl != 0u) atomicAdd(&meta->nfluid, nfl);
-    if (wet_out != nullptr) wet_out[blockIdx.x] = (uint8_t)wet;
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    part[b] = mn;
+    part[nb + b] = mx;
+    part[2 * nb + b] = nfl;
+    if (wet_out != nullptr) wet_out[b] = (uint8_t)wet;
     if (ord0 != nullptr)  // per-block work order: natural, with the wet bit
-      ord0[blockIdx.x] = (int32_t)blockIdx.x | (wet ? ORD_WET : 0);
-    // a value that would not move the level's current extreme skips the
-    // atomic: with one coupling value on the level (the usual case) only the
-    // first workgroups update it, instead of every workgroup queueing on the
-    // same address (a stale read only costs a redundant atomic)
-    if (mx != 0u) {
-      const uint32_t cur_mn = *(volatile const uint32_t*)&meta->minbits;
-      const uint32_t cur_mx = *(volatile const uint32_t*)&meta->maxbits;
-      if (mn < cur_mn) atomicMin(&meta->minbits, mn);
-      if (mx > cur_mx) atomicMax(&meta->maxbits, mx);
+      ord0[b] = (int32_t)b | (wet ? ORD_WET : 0);
+  }
+}
+
+// The packing partials of up to 16 levels folded into their LevelMeta: one
+// workgroup per level (blockIdx.x), after the levels' packing kernels.
+struct MetaReduceList {
+  LevelMeta* meta[16];
+  const uint32_t* part[16];
+  int nblocks[16];
+};
+constexpr int META_THREADS = 1024;
+__global__ void __launch_bounds__(META_THREADS) k_meta_reduce(MetaReduceList ml) {
+  __shared__ uint32_t smn[META_THREADS / 64], smx[META_THREADS / 64], snf[META_THREADS / 64];
+  const int l = blockIdx.x;
+  const int nb = ml.nblocks[l];
+  const uint32_t* p = ml.part[l];
+  uint32_t mn = 0xffffffffu, mx = 0u, nf = 0u;
+  for (int b = threadIdx.x; b < nb; b += META_THREADS) {
+    const uint32_t a = p[b], c = p[nb + b];
+    mn = a < mn ? a : mn;
+    mx = c > mx ? c : mx;
+    nf += p[2 * nb + b];
+  }
+  mn = wave_min_u32(mn);
+  mx = wave_max_u32(mx);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nf += (uint32_t)__shfl_xor((int)nf, o);
+  if ((threadIdx.x & 63) == 0) {
+    smn[threadIdx.x >> 6] = mn;
+    smx[threadIdx.x >> 6] = mx;
+    snf[threadIdx.x >> 6] = nf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < META_THREADS / 64; ++w) {
+      mn = smn[w] < mn ? smn[w] : mn;
+      mx = smx[w] > mx ? smx[w] : mx;
+      nf += snf[w];
     }
+    ml.meta[l]->minbits = mn;
+    ml.meta[l]->maxbits = mx;
+    ml.meta[l]->nfluid = nf;
   }
 }
 
@@ -2462,7 +2494,7 @@ template <typename T, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __restrict__ gt,
                                                       const float* __restrict__ Adiag,
                                                       const float* __restrict__ Ax,
-                                                      LevelPtrs<T> P, LevelMeta* meta) {
+                                                      LevelPtrs<T> P, uint32_t* part) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
   if (t < g.size) {
@@ -2496,7 +2528,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
 #pragma unroll
     for (int c = 0; c < 2; ++c) pack_slot<T, DIM>(v[c], c, t, P, mn, mx, wet, nfl);
   }
-  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], meta);
+  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], part);
 }
 
 // The pressure operator of one level assembled straight into the packed
@@ -2518,7 +2550,7 @@ template <typename T, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_pack_pressure(Geo g, const int8_t* __restrict__ gt,
                                                          const float* __restrict__ phi,
                                                          float sA, float c_max, LevelPtrs<T> P,
-                                                         LevelMeta* meta) {
+                                                         uint32_t* part) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
   if (t < g.size) {
@@ -2586,7 +2618,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack_pressure(Geo g, const int8_t* __
 #pragma unroll
     for (int c = 0; c < 2; ++c) pack_slot<T, DIM>(v[c], c, t, P, mn, mx, wet, nfl);
   }
-  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], meta);
+  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], part);
 }
 
 // The pressure right-hand side straight into the packed b (both colours of a
@@ -2678,20 +2710,6 @@ __global__ void __launch_bounds__(BLOCK) k_pack_pressure_b(Geo g, const int8_t* 
   }
   b0[t] = (T)out[0];
   b1[t] = (T)out[1];
-}
-
-// every level's metadata reset (one launch before a hierarchy is packed)
-struct MetaList {
-  LevelMeta* m[16];
-  int n;
-};
-__global__ void k_meta_init_all(MetaList ml) {
-  const int l = threadIdx.x;
-  if (l < ml.n) {
-    ml.m[l]->minbits = 0xffffffffu;
-    ml.m[l]->maxbits = 0u;
-    ml.m[l]->nfluid = 0u;
-  }
 }
 
 template <typename S, typename D>

@@ -52,6 +52,7 @@ struct LevelBuf {
   int32_t* ord[2];   // work order per block / per half-sweep virtual block
   int32_t* ordtmp;   // scratch of the order kernel (one int per block)
   LevelMeta* meta;   // nonzero-coupling summary, written when the level is loaded
+  uint32_t* part;    // the packing kernel's per-block share of meta (3 ints per block)
   void* res[2];      // restriction scratch (NULL on the coarsest level)
 };
 
@@ -1104,12 +1105,12 @@ int solve_single(mgpcg_solver* s, int max_iters, double rel_tol, double abs_tol,
 
 // Layout of one system's device pool: every sub-buffer 256-B aligned, in this
 // order per level (both colours of gt, Adiag, Ax per axis, r, z with its zero
-// pads, flag words, residual scratch), then the level's metadata, block flags
-// and work orders; after the levels, level 0's b, x, both p, Ap, the CG state
+// pads, flag words, residual scratch), then the level's metadata, block flags,
+// work orders and packing partials; after the levels, level 0's b, x, both p, Ap, the CG state
 // and the reduction partials.
 struct PoolPlan {
   struct Offs {
-    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta, wet, ord[2], ordtmp;
+    size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta, wet, ord[2], ordtmp, part;
   };
   std::vector<Geo> geo;
   std::vector<Offs> lo;
@@ -1147,6 +1148,7 @@ void plan_pool(int dim, const int32_t* res, int n_levels, size_t esz, size_t xsz
     o.ord[0] = take(sizeof(int32_t) * nblk(g.size));
     o.ord[1] = take(sizeof(int32_t) * nsweep(g.size));
     o.ordtmp = take(sizeof(int32_t) * nsweep(g.size));
+    o.part = take(sizeof(uint32_t) * 3 * nblk(g.size));
     P->lo.push_back(o);
   }
   const Geo& g0 = P->geo[0];
@@ -1252,6 +1254,7 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
     B.ord[0] = (int32_t*)(s->pool + lo[l].ord[0]);
     B.ord[1] = (int32_t*)(s->pool + lo[l].ord[1]);
     B.ordtmp = (int32_t*)(s->pool + lo[l].ordtmp);
+    B.part = (uint32_t*)(s->pool + lo[l].part);
     s->lv.push_back(B);
     const Geo& g = s->geo[l];
     // ghost cells are SOLID (the zero fill made them FLUID)
@@ -1410,24 +1413,28 @@ extern "C" int mgpcg_load_level(mgpcg_solver_t* s, int level, const int8_t* grid
   MG_REQUIRE(grid_type && Adiag && Ax, "NULL field pointer");
   MG_HIP(hipSetDevice(s->device));
   const Geo& g = s->geo[level];
-  LevelMeta* meta = s->lv[level].meta;
-  hipLaunchKernelGGL(k_meta_init, dim3(1), dim3(64), 0, s->user_stream, meta);
+  uint32_t* part = s->lv[level].part;
   const dim3 grid(nblk(g.size)), block(BLOCK);
   if (s->precision == F64) {
     if (s->dim == 3)
       hipLaunchKernelGGL((k_pack_level<double, 3>), grid, block, 0, s->user_stream, g,
-                         grid_type, Adiag, Ax, level_ptrs<double>(s, level), meta);
+                         grid_type, Adiag, Ax, level_ptrs<double>(s, level), part);
     else
       hipLaunchKernelGGL((k_pack_level<double, 2>), grid, block, 0, s->user_stream, g,
-                         grid_type, Adiag, Ax, level_ptrs<double>(s, level), meta);
+                         grid_type, Adiag, Ax, level_ptrs<double>(s, level), part);
   } else {
     if (s->dim == 3)
       hipLaunchKernelGGL((k_pack_level<float, 3>), grid, block, 0, s->user_stream, g,
-                         grid_type, Adiag, Ax, level_ptrs<float>(s, level), meta);
+                         grid_type, Adiag, Ax, level_ptrs<float>(s, level), part);
     else
       hipLaunchKernelGGL((k_pack_level<float, 2>), grid, block, 0, s->user_stream, g,
-                         grid_type, Adiag, Ax, level_ptrs<float>(s, level), meta);
+                         grid_type, Adiag, Ax, level_ptrs<float>(s, level), part);
   }
+  MetaReduceList ml;
+  ml.meta[0] = s->lv[level].meta;
+  ml.part[0] = part;
+  ml.nblocks[0] = nblk(g.size);
+  hipLaunchKernelGGL(k_meta_reduce, dim3(1), dim3(META_THREADS), 0, s->user_stream, ml);
   if (level == 0 && s->coarse_from > 0) {
     const LevelBuf& B = s->lv[level];
     hipLaunchKernelGGL(k_level_order, dim3(1), dim3(ORDER_THREADS), 0, s->user_stream, B.wet,
@@ -1507,10 +1514,6 @@ int assemble_pressure_t(mgpcg_solver* s, int8_t* const* gt, const float* const* 
                         const float* phi, float sA, float sb, float p0, int gfm, float c_max,
                         int add_c, float cadd) {
   hipStream_t st = s->user_stream;
-  MetaList ml;
-  ml.n = s->L;
-  for (int l = 0; l < s->L; ++l) ml.m[l] = s->lv[l].meta;
-  hipLaunchKernelGGL(k_meta_init_all, dim3(1), dim3(64), 0, st, ml);
   MacVel V;
   for (int k = 0; k < 3; ++k) V.u[k] = k < s->dim ? velocity[k] : nullptr;
   const Geo& g0 = s->geo[0];
@@ -1532,11 +1535,18 @@ int assemble_pressure_t(mgpcg_solver* s, int8_t* const* gt, const float* const* 
     const float* phl = l == 0 ? ph : nullptr;  // ghost-fluid weights at level 0 only
     if (s->dim == 3)
       hipLaunchKernelGGL((k_pack_pressure<T, 3>), dim3(nblk(g.size)), dim3(BLOCK), 0, st, g,
-                         gt[l], phl, sA, c_max, level_ptrs<T>(s, l), s->lv[l].meta);
+                         gt[l], phl, sA, c_max, level_ptrs<T>(s, l), s->lv[l].part);
     else
       hipLaunchKernelGGL((k_pack_pressure<T, 2>), dim3(nblk(g.size)), dim3(BLOCK), 0, st, g,
-                         gt[l], phl, sA, c_max, level_ptrs<T>(s, l), s->lv[l].meta);
+                         gt[l], phl, sA, c_max, level_ptrs<T>(s, l), s->lv[l].part);
   }
+  MetaReduceList ml;
+  for (int l = 0; l < s->L; ++l) {
+    ml.meta[l] = s->lv[l].meta;
+    ml.part[l] = s->lv[l].part;
+    ml.nblocks[l] = nblk(s->geo[l].size);
+  }
+  hipLaunchKernelGGL(k_meta_reduce, dim3(s->L), dim3(META_THREADS), 0, st, ml);
   if (s->coarse_from > 0) {
     const LevelBuf& B = s->lv[0];
     hipLaunchKernelGGL(k_level_order, dim3(1), dim3(ORDER_THREADS), 0, st, B.wet,
