@@ -1492,6 +1492,323 @@ __device__ void coarse_bottom(const Geo& g, const LevelPtrs<T>& P, int bottom) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The coarse V-cycle on its LDS image.  Every phase gives each lane up to
+// CZ_SLOTS slots and gathers the operands of all of them before computing any
+// (within a phase no slot reads a value the phase writes), then stores: a lane
+// keeps all its slots' LDS reads in flight together instead of one slot's
+// chain of reads at a time.  The arithmetic is the global-memory form's,
+// operation for operation (smooth_slot, restrict_slot, prolong_smooth1_slot).
+// ---------------------------------------------------------------------------
+constexpr int CZ_SLOTS = 4;
+
+// operands of a half-sweep, or of a residual (own = the slot's z), at one slot
+template <typename T, int DIM>
+struct CzOps {
+  int gt;
+  T rv, ad, own;
+  T al[DIM], ah[DIM], zl[DIM], zh[DIM];
+};
+
+// couplings of colour-c slot t: from its flag word on a level with uniform
+// couplings, else the stored Ax (global memory; indices clamped, so a ghost
+// slot's reads stay in bounds and are discarded)
+template <typename T, int DIM>
+__device__ __forceinline__ void cz_couplings(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
+                                             int t, const Nbr<DIM>& n, T* al, T* ah) {
+  if (k.uniform) {
+    const uint32_t f = P.nb[c][t];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      al[a] = ((f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
+      ah[a] = ((f >> a) & 1u) ? k.coef : (T)0;
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      al[a] = P.Ax[a][c ^ 1][n.lo[a]];
+      ah[a] = P.Ax[a][c][t];
+    }
+  }
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ void cz_gather(const Geo& g, const LevelPtrs<T>& P,
+                                          const Coupling<T>& k, int c, int t, bool own,
+                                          CzOps<T, DIM>& v) {
+  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, row_q(g, t, c));
+  v.gt = P.gt[c][t];
+  v.rv = P.r[c][t];
+  v.ad = P.Adiag[c][t];
+  v.own = own ? P.z[c][t] : (T)0;
+  const T* zo = P.z[c ^ 1];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    v.zl[a] = zo[n.lo[a]];
+    v.zh[a] = zo[n.hi[a]];
+  }
+  cz_couplings<T, DIM>(P, k, c, t, n, v.al, v.ah);
+}
+
+// smooth(l, c) over the level (mgpcg.py:104-109)
+template <typename T, int DIM>
+__device__ void cz_half(const Geo& g, const LevelPtrs<T>& P, const Coupling<T>& k, int c) {
+  const int n = g.size;
+  for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
+    CzOps<T, DIM> v[CZ_SLOTS];
+#pragma unroll
+    for (int s = 0; s < CZ_SLOTS; ++s) {
+      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+      v[s].gt = MG_SOLID;
+      if (t < n) cz_gather<T, DIM>(g, P, k, c, t, false, v[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < CZ_SLOTS; ++s) {
+      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+      if (v[s].gt != MG_FLUID) continue;
+      T sm = nsum_vals<T, DIM>(v[s].al, v[s].ah, v[s].zl, v[s].zh);
+      T num = v[s].rv - sm;
+      P.z[c][t] = num / v[s].ad;
+    }
+  }
+  __syncthreads();
+}
+
+// z.fill(0); smooth(l, 0): z0 = r0 / Adiag0 on FLUID, 0 elsewhere, and z1 = 0
+// (the smooth(l, 1) that follows rewrites every colour-1 FLUID value)
+template <typename T>
+__device__ void cz_first(const Geo& g, const LevelPtrs<T>& P) {
+  const int n = g.size;
+  for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
+    int gt[CZ_SLOTS];
+    T rv[CZ_SLOTS], ad[CZ_SLOTS];
+#pragma unroll
+    for (int s = 0; s < CZ_SLOTS; ++s) {
+      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+      gt[s] = MG_SOLID;
+      if (t < n) {
+        gt[s] = P.gt[0][t];
+        rv[s] = P.r[0][t];
+        ad[s] = P.Adiag[0][t];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < CZ_SLOTS; ++s) {
+      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+      if (t >= n) continue;
+      P.z[0][t] = gt[s] == MG_FLUID ? rv[s] / ad[s] : (T)0;
+      P.z[1][t] = (T)0;
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void cz_zero(const Geo& g, const LevelPtrs<T>& P) {
+  for (int t = threadIdx.x; t < g.size; t += COARSE_THREADS) zero_slot<T>(P, t);
+  __syncthreads();
+}
+
+// restrict(l) (mgpcg.py:111-118) into the coarser level's r: the residual of
+// every fine cell (0 off FLUID) into `res` (two colour arrays of the fine
+// level's size), then every interior coarse cell sums its eight children in
+// lexicographic order (adding a 0 leaves the sum unchanged, as skipping a
+// non-FLUID child does: the sum starts at +0 and never becomes -0)
+template <typename T, int DIM>
+__device__ void cz_restrict(const Geo& gf, const LevelPtrs<T>& F, const Coupling<T>& k,
+                            const Geo& gc, T* rc0, T* rc1, T* const* res) {
+  const int n = gf.size;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
+      CzOps<T, DIM> v[CZ_SLOTS];
+#pragma unroll
+      for (int s = 0; s < CZ_SLOTS; ++s) {
+        const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+        v[s].gt = MG_SOLID;
+        if (t < n) cz_gather<T, DIM>(gf, F, k, c, t, true, v[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < CZ_SLOTS; ++s) {
+        const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+        if (t >= n) continue;
+        T rr = (T)0;
+        if (v[s].gt == MG_FLUID) {
+          T Az = v[s].ad * v[s].own;
+          T ns = nsum_vals<T, DIM>(v[s].al, v[s].ah, v[s].zl, v[s].zh);
+          Az = Az + ns;
+          rr = v[s].rv - Az;
+        }
+        res[c][t] = rr;
+      }
+    }
+  }
+  __syncthreads();
+  const int nc = gc.size;
+  for (int tc = threadIdx.x; tc < nc; tc += COARSE_THREADS) {
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const Slot s = decode(gc, tc, cc);
+      if (!s.interior) continue;
+      T v[8];
+      int nch = 0;
+#pragma unroll
+      for (int o0 = 0; o0 < (DIM == 3 ? 2 : 1); ++o0)
+#pragma unroll
+        for (int o1 = 0; o1 < 2; ++o1)
+#pragma unroll
+          for (int o2 = 0; o2 < 2; ++o2) {
+            const int i = (DIM == 3) ? 2 * s.i + o0 : 0;
+            const int j = 2 * s.j + o1, kk = 2 * s.k + o2;
+            v[nch++] = res[colour_of(i, j, kk)][slot_of(gf, i, j, kk)];
+          }
+      T sum = (T)0;
+#pragma unroll
+      for (int o = 0; o < (DIM == 3 ? 8 : 4); ++o) sum = sum + v[o];
+      (cc ? rc1 : rc0)[tc] = sum;
+    }
+  }
+  __syncthreads();
+}
+
+// prolongate(l); smooth(l, 1) (mgpcg.py:120-123, 146-148) at every colour-1
+// slot: the colour-0 neighbours read z0 + z_coarse[parent] (the parent logic
+// of prolong_smooth1_slot_g)
+template <typename T, int DIM>
+struct CzProl {
+  int gt;
+  T rv, ad, vp;
+  T zl[DIM], zh[DIM], vq[DIM], al[DIM], ah[DIM];
+  bool up[DIM], inq[DIM];
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ void cz_prolong_gather(const Geo& g, const Geo& gc,
+                                                  const LevelPtrs<T>& P, const Coupling<T>& k,
+                                                  const T* zc0, const T* zc1, int t,
+                                                  CzProl<T, DIM>& v) {
+  const Slot c = decode(g, t, 1);
+  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, c.q);
+  v.gt = P.gt[1][t];
+  v.rv = P.r[1][t];
+  v.ad = P.Adiag[1][t];
+  const T* z0 = P.z[0];
+  const int pi = (DIM == 3 && c.i > 0) ? c.i >> 1 : 0;
+  const int pj = c.j > 0 ? c.j >> 1 : 0, pk = c.k > 0 ? c.k >> 1 : 0;
+  const int cp = colour_of(pi, pj, pk);
+  const int sp = min(slot_of(gc, pi, pj, pk), gc.size - 1);
+  const T* zcp = cp ? zc1 : zc0;
+  const T* zcq = cp ? zc0 : zc1;
+  v.vp = zcp[sp];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    const int va = a + 3 - DIM;  // view axis of original axis a
+    const int x = va == 0 ? c.i : (va == 1 ? c.j : c.k);
+    const int nx = va == 0 ? g.n0 : (va == 1 ? g.n1 : g.n2);
+    v.up[a] = (x & 1) != 0;  // the other parent lies above (odd x) or below (even x)
+    const int d = v.up[a] ? 1 : -1;
+    v.inq[a] = x + d >= 0 && x + d < nx;
+    int sq;
+    if (va == 0) sq = sp + d * gc.sx;
+    else if (va == 1) sq = sp + d * gc.W;
+    else sq = sp - ((pk + 1) >> 1) + ((pk + d + 1) >> 1);
+    sq = (v.inq[a] && sq >= 0 && sq < gc.size) ? sq : sp;
+    v.vq[a] = zcq[sq];
+    v.zl[a] = z0[n.lo[a]];
+    v.zh[a] = z0[n.hi[a]];
+  }
+  cz_couplings<T, DIM>(P, k, 1, t, n, v.al, v.ah);
+}
+
+template <typename T, int DIM>
+__device__ void cz_prolong_smooth1(const Geo& g, const Geo& gc, const LevelPtrs<T>& P,
+                                   const Coupling<T>& k, const T* zc0, const T* zc1) {
+  const int n = g.size;
+  for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
+    CzProl<T, DIM> v[CZ_SLOTS];
+#pragma unroll
+    for (int s = 0; s < CZ_SLOTS; ++s) {
+      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+      v[s].gt = MG_SOLID;
+      if (t < n) cz_prolong_gather<T, DIM>(g, gc, P, k, zc0, zc1, t, v[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < CZ_SLOTS; ++s) {
+      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+      if (v[s].gt != MG_FLUID) continue;
+      T zl[DIM], zh[DIM];
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        zl[a] = v[s].zl[a];
+        zh[a] = v[s].zh[a];
+        if (v[s].up[a]) {
+          zl[a] = zl[a] + v[s].vp;
+          if (v[s].inq[a]) zh[a] = zh[a] + v[s].vq[a];
+        } else {
+          if (v[s].inq[a]) zl[a] = zl[a] + v[s].vq[a];
+          zh[a] = zh[a] + v[s].vp;
+        }
+      }
+      T ret = nsum_vals<T, DIM>(v[s].al, v[s].ah, zl, zh);
+      T num = v[s].rv - ret;
+      P.z[1][t] = num / v[s].ad;
+    }
+  }
+  __syncthreads();
+}
+
+// coarse_pre / coarse_bottom / coarse_post on the LDS image
+template <typename T, int DIM>
+__device__ void cz_pre(const Geo& g, const LevelPtrs<T>& P, int nu) {
+  if (nu <= 0) {
+    cz_zero<T>(g, P);
+    return;
+  }
+  const Coupling<T> k = coupling(P);
+  cz_first<T>(g, P);
+  cz_half<T, DIM>(g, P, k, 1);
+  for (int i = 1; i < nu; ++i) {
+    cz_half<T, DIM>(g, P, k, 0);
+    cz_half<T, DIM>(g, P, k, 1);
+  }
+}
+
+template <typename T, int DIM>
+__device__ void cz_bottom(const Geo& g, const LevelPtrs<T>& P, int bottom) {
+  const int half = bottom / 2;
+  if (half <= 0) {
+    cz_zero<T>(g, P);
+    return;
+  }
+  const Coupling<T> k = coupling(P);
+  cz_first<T>(g, P);
+  cz_half<T, DIM>(g, P, k, 1);
+  for (int i = 1; i < half; ++i) {
+    cz_half<T, DIM>(g, P, k, 0);
+    cz_half<T, DIM>(g, P, k, 1);
+  }
+  for (int i = 0; i < half; ++i) {
+    cz_half<T, DIM>(g, P, k, 1);
+    cz_half<T, DIM>(g, P, k, 0);
+  }
+}
+
+template <typename T, int DIM>
+__device__ void cz_post(const Geo& g, const Geo& gc, const LevelPtrs<T>& P, T* const* zc,
+                        int nu) {
+  if (nu <= 0) {
+    coarse_post<T, DIM>(g, gc, P, zc, nu);
+    return;
+  }
+  const Coupling<T> k = coupling(P);
+  cz_prolong_smooth1<T, DIM>(g, gc, P, k, zc[0], zc[1]);
+  cz_half<T, DIM>(g, P, k, 0);
+  for (int i = 1; i < nu; ++i) {
+    cz_half<T, DIM>(g, P, k, 1);
+    cz_half<T, DIM>(g, P, k, 0);
+  }
+}
+
 // LDS image of one system's coarse levels: per level the z, r and Adiag
 // colour arrays, the cell types, the coupling flags and the coupling range.
 // The stored couplings (Ax) of a level without uniform couplings stay in
@@ -1502,15 +1819,21 @@ __host__ __device__ constexpr int coarse_image_bytes(int size, int esz) {
   return 6 * lds_align(size * esz) + 2 * lds_align(size) + 2 * lds_align(2 * size) +
          lds_align((int)sizeof(LevelMeta));
 }
+// the restriction's fine residuals (two colour arrays of the largest level
+// that restricts), after the levels' images
+__host__ __device__ constexpr int coarse_scratch_bytes(int size, int esz) {
+  return 2 * lds_align(size * esz);
+}
 
 // One workgroup per item: item.desc = the system's LevelDesc of its first
 // coarse level, item.block0 = number of levels from there to the bottom
 // (a system's LevelDescs are contiguous).  With USE_LDS every per-slot array
 // the V-cycle reads is copied into LDS once (the host checks that the
-// levels' coarse_image_bytes fit COARSE_LDS_BYTES), the V-cycle runs on the
-// copy, and z of every level and the restricted r of the levels below the
-// first are copied back at the end.  The slot functions are the ones the
-// global-memory form uses, on pointers into LDS.
+// levels' coarse_image_bytes and the restriction's coarse_scratch_bytes fit
+// COARSE_LDS_BYTES), the V-cycle runs on the copy in the multi-slot phases
+// above, and z of every level and the restricted r of the levels below the
+// first are copied back at the end.  Without, the slot functions of the
+// global-memory form run on the levels in place.
 template <typename T, int DIM, bool USE_LDS>
 __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* __restrict__ descs,
                                                            const Item* __restrict__ items,
@@ -1561,22 +1884,43 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
     }
     __syncthreads();
   }
-  for (int k = 0; k < nlev - 1; ++k) {
-    const LevelDesc<T>& D = descs[d0 + k];
-    const LevelPtrs<T> P = lv(k);
-    const LevelPtrs<T> C = lv(k + 1);
-    coarse_pre<T, DIM>(D.g, P, nu);
-    for (int tc = threadIdx.x; tc < D.gc.size; tc += COARSE_THREADS) {
-      restrict_slot<T, DIM>(D.g, P, P.z, D.gc, C.r[0], C.r[1], 0, tc);
-      restrict_slot<T, DIM>(D.g, P, P.z, D.gc, C.r[0], C.r[1], 1, tc);
+  if constexpr (USE_LDS) {
+    int off = 0;
+    for (int j = 0; j < nlev; ++j) off += coarse_image_bytes(descs[d0 + j].g.size, (int)sizeof(T));
+    T* res[2];
+    res[0] = (T*)(img + off);
+    res[1] = (T*)(img + off + lds_align(descs[d0].g.size * (int)sizeof(T)));
+    for (int k = 0; k < nlev - 1; ++k) {
+      const LevelDesc<T>& D = descs[d0 + k];
+      const LevelPtrs<T> P = lv(k);
+      const LevelPtrs<T> C = lv(k + 1);
+      cz_pre<T, DIM>(D.g, P, nu);
+      cz_restrict<T, DIM>(D.g, P, coupling(P), D.gc, C.r[0], C.r[1], res);
     }
-    __syncthreads();
-  }
-  coarse_bottom<T, DIM>(descs[d0 + nlev - 1].g, lv(nlev - 1), bottom);
-  for (int k = nlev - 2; k >= 0; --k) {
-    const LevelDesc<T>& D = descs[d0 + k];
-    const LevelPtrs<T> C = lv(k + 1);
-    coarse_post<T, DIM>(D.g, D.gc, lv(k), C.z, nu);
+    cz_bottom<T, DIM>(descs[d0 + nlev - 1].g, lv(nlev - 1), bottom);
+    for (int k = nlev - 2; k >= 0; --k) {
+      const LevelDesc<T>& D = descs[d0 + k];
+      const LevelPtrs<T> C = lv(k + 1);
+      cz_post<T, DIM>(D.g, D.gc, lv(k), C.z, nu);
+    }
+  } else {
+    for (int k = 0; k < nlev - 1; ++k) {
+      const LevelDesc<T>& D = descs[d0 + k];
+      const LevelPtrs<T> P = lv(k);
+      const LevelPtrs<T> C = lv(k + 1);
+      coarse_pre<T, DIM>(D.g, P, nu);
+      for (int tc = threadIdx.x; tc < D.gc.size; tc += COARSE_THREADS) {
+        restrict_slot<T, DIM>(D.g, P, P.z, D.gc, C.r[0], C.r[1], 0, tc);
+        restrict_slot<T, DIM>(D.g, P, P.z, D.gc, C.r[0], C.r[1], 1, tc);
+      }
+      __syncthreads();
+    }
+    coarse_bottom<T, DIM>(descs[d0 + nlev - 1].g, lv(nlev - 1), bottom);
+    for (int k = nlev - 2; k >= 0; --k) {
+      const LevelDesc<T>& D = descs[d0 + k];
+      const LevelPtrs<T> C = lv(k + 1);
+      coarse_post<T, DIM>(D.g, D.gc, lv(k), C.z, nu);
+    }
   }
   if constexpr (USE_LDS) {
     // the finer level's prolongation reads the first coarse level's z; the

@@ -571,7 +571,7 @@ int engine_build_items(Engine* e) {
     x.desc = m * L + M.s->coarse_from;
     x.block0 = L - M.s->coarse_from;
     it.push_back(x);
-    long bytes = 0;
+    long bytes = coarse_scratch_bytes(M.s->geo[M.s->coarse_from].size, (int)M.s->esz);
     for (int l = M.s->coarse_from; l < L; ++l)
       bytes += coarse_image_bytes(M.s->geo[l].size, (int)M.s->esz);
     if (bytes > COARSE_LDS_BYTES) e->coarse_lds = false;
