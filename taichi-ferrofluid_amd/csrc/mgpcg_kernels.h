@@ -1500,42 +1500,30 @@ __device__ void coarse_bottom(const Geo& g, const LevelPtrs<T>& P, int bottom) {
 // chain of reads at a time.  The arithmetic is the global-memory form's,
 // operation for operation (smooth_slot, restrict_slot, prolong_smooth1_slot).
 // ---------------------------------------------------------------------------
-constexpr int CZ_SLOTS = 4;
+constexpr int CZ_SLOTS = 3;
+// slots per lane of a phase that also gathers stored couplings (more
+// operands per slot: fewer slots keep them in registers)
+template <bool COMPACT>
+__host__ __device__ constexpr int cz_slots() {
+  return COMPACT ? CZ_SLOTS : 2;
+}
 
-// operands of a half-sweep, or of a residual (own = the slot's z), at one slot
-template <typename T, int DIM>
+// operands of a half-sweep, or of a residual (own = the slot's z), at one
+// slot: on a level with uniform couplings (COMPACT) the flag word, else the
+// stored couplings (global memory, at clamped indices: a ghost slot's reads
+// stay in bounds and are discarded).  Only loads: the couplings are decoded
+// when the value is formed, so no read waits for another.
+template <typename T, int DIM, bool COMPACT>
 struct CzOps {
   int gt;
+  uint32_t f;
   T rv, ad, own;
   T al[DIM], ah[DIM], zl[DIM], zh[DIM];
 };
 
-// couplings of colour-c slot t: from its flag word on a level with uniform
-// couplings, else the stored Ax (global memory; indices clamped, so a ghost
-// slot's reads stay in bounds and are discarded)
-template <typename T, int DIM>
-__device__ __forceinline__ void cz_couplings(const LevelPtrs<T>& P, const Coupling<T>& k, int c,
-                                             int t, const Nbr<DIM>& n, T* al, T* ah) {
-  if (k.uniform) {
-    const uint32_t f = P.nb[c][t];
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      al[a] = ((f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
-      ah[a] = ((f >> a) & 1u) ? k.coef : (T)0;
-    }
-  } else {
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      al[a] = P.Ax[a][c ^ 1][n.lo[a]];
-      ah[a] = P.Ax[a][c][t];
-    }
-  }
-}
-
-template <typename T, int DIM>
-__device__ __forceinline__ void cz_gather(const Geo& g, const LevelPtrs<T>& P,
-                                          const Coupling<T>& k, int c, int t, bool own,
-                                          CzOps<T, DIM>& v) {
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void cz_gather(const Geo& g, const LevelPtrs<T>& P, int c, int t,
+                                          bool own, CzOps<T, DIM, COMPACT>& v) {
   const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, row_q(g, t, c));
   v.gt = P.gt[c][t];
   v.rv = P.r[c][t];
@@ -1547,31 +1535,63 @@ __device__ __forceinline__ void cz_gather(const Geo& g, const LevelPtrs<T>& P,
     v.zl[a] = zo[n.lo[a]];
     v.zh[a] = zo[n.hi[a]];
   }
-  cz_couplings<T, DIM>(P, k, c, t, n, v.al, v.ah);
+  if constexpr (COMPACT) {
+    v.f = P.nb[c][t];
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      v.al[a] = P.Ax[a][c ^ 1][n.lo[a]];
+      v.ah[a] = P.Ax[a][c][t];
+    }
+  }
+}
+
+// the slot's couplings (decoded from the flag word on COMPACT levels)
+template <typename T, int DIM, bool COMPACT>
+__device__ __forceinline__ void cz_coefs(const Coupling<T>& k, uint32_t f, const T* sal,
+                                         const T* sah, T* al, T* ah) {
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    if constexpr (COMPACT) {
+      al[a] = ((f >> (NB_DN + a)) & 1u) ? k.coef : (T)0;
+      ah[a] = ((f >> a) & 1u) ? k.coef : (T)0;
+    } else {
+      al[a] = sal[a];
+      ah[a] = sah[a];
+    }
+  }
 }
 
 // smooth(l, c) over the level (mgpcg.py:104-109)
-template <typename T, int DIM>
-__device__ void cz_half(const Geo& g, const LevelPtrs<T>& P, const Coupling<T>& k, int c) {
+template <typename T, int DIM, bool COMPACT>
+__device__ void cz_half_t(const Geo& g, const LevelPtrs<T>& P, const Coupling<T>& k, int c) {
   const int n = g.size;
-  for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
-    CzOps<T, DIM> v[CZ_SLOTS];
+  constexpr int NS = cz_slots<COMPACT>();
+  for (int b = 0; b < n; b += NS * COARSE_THREADS) {
+    CzOps<T, DIM, COMPACT> v[NS];
 #pragma unroll
-    for (int s = 0; s < CZ_SLOTS; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
       v[s].gt = MG_SOLID;
-      if (t < n) cz_gather<T, DIM>(g, P, k, c, t, false, v[s]);
+      if (t < n) cz_gather<T, DIM, COMPACT>(g, P, c, t, false, v[s]);
     }
 #pragma unroll
-    for (int s = 0; s < CZ_SLOTS; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
       if (v[s].gt != MG_FLUID) continue;
-      T sm = nsum_vals<T, DIM>(v[s].al, v[s].ah, v[s].zl, v[s].zh);
+      T al[DIM], ah[DIM];
+      cz_coefs<T, DIM, COMPACT>(k, v[s].f, v[s].al, v[s].ah, al, ah);
+      T sm = nsum_vals<T, DIM>(al, ah, v[s].zl, v[s].zh);
       T num = v[s].rv - sm;
       P.z[c][t] = num / v[s].ad;
     }
   }
   __syncthreads();
+}
+template <typename T, int DIM>
+__device__ void cz_half(const Geo& g, const LevelPtrs<T>& P, const Coupling<T>& k, int c) {
+  if (k.uniform) cz_half_t<T, DIM, true>(g, P, k, c);
+  else cz_half_t<T, DIM, false>(g, P, k, c);
 }
 
 // z.fill(0); smooth(l, 0): z0 = r0 / Adiag0 on FLUID, 0 elsewhere, and z1 = 0
@@ -1614,28 +1634,31 @@ __device__ void cz_zero(const Geo& g, const LevelPtrs<T>& P) {
 // level's size), then every interior coarse cell sums its eight children in
 // lexicographic order (adding a 0 leaves the sum unchanged, as skipping a
 // non-FLUID child does: the sum starts at +0 and never becomes -0)
-template <typename T, int DIM>
-__device__ void cz_restrict(const Geo& gf, const LevelPtrs<T>& F, const Coupling<T>& k,
-                            const Geo& gc, T* rc0, T* rc1, T* const* res) {
+template <typename T, int DIM, bool COMPACT>
+__device__ void cz_residuals(const Geo& gf, const LevelPtrs<T>& F, const Coupling<T>& k,
+                             T* const* res) {
   const int n = gf.size;
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
-      CzOps<T, DIM> v[CZ_SLOTS];
+    constexpr int NS = cz_slots<COMPACT>();
+    for (int b = 0; b < n; b += NS * COARSE_THREADS) {
+    CzOps<T, DIM, COMPACT> v[NS];
 #pragma unroll
-      for (int s = 0; s < CZ_SLOTS; ++s) {
+      for (int s = 0; s < NS; ++s) {
         const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
         v[s].gt = MG_SOLID;
-        if (t < n) cz_gather<T, DIM>(gf, F, k, c, t, true, v[s]);
+        if (t < n) cz_gather<T, DIM, COMPACT>(gf, F, c, t, true, v[s]);
       }
 #pragma unroll
-      for (int s = 0; s < CZ_SLOTS; ++s) {
+      for (int s = 0; s < NS; ++s) {
         const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
         if (t >= n) continue;
         T rr = (T)0;
         if (v[s].gt == MG_FLUID) {
+          T al[DIM], ah[DIM];
+          cz_coefs<T, DIM, COMPACT>(k, v[s].f, v[s].al, v[s].ah, al, ah);
           T Az = v[s].ad * v[s].own;
-          T ns = nsum_vals<T, DIM>(v[s].al, v[s].ah, v[s].zl, v[s].zh);
+          T ns = nsum_vals<T, DIM>(al, ah, v[s].zl, v[s].zh);
           Az = Az + ns;
           rr = v[s].rv - Az;
         }
@@ -1644,6 +1667,13 @@ __device__ void cz_restrict(const Geo& gf, const LevelPtrs<T>& F, const Coupling
     }
   }
   __syncthreads();
+}
+
+template <typename T, int DIM>
+__device__ void cz_restrict(const Geo& gf, const LevelPtrs<T>& F, const Coupling<T>& k,
+                            const Geo& gc, T* rc0, T* rc1, T* const* res) {
+  if (k.uniform) cz_residuals<T, DIM, true>(gf, F, k, res);
+  else cz_residuals<T, DIM, false>(gf, F, k, res);
   const int nc = gc.size;
   for (int tc = threadIdx.x; tc < nc; tc += COARSE_THREADS) {
 #pragma unroll
@@ -1674,19 +1704,20 @@ __device__ void cz_restrict(const Geo& gf, const LevelPtrs<T>& F, const Coupling
 // prolongate(l); smooth(l, 1) (mgpcg.py:120-123, 146-148) at every colour-1
 // slot: the colour-0 neighbours read z0 + z_coarse[parent] (the parent logic
 // of prolong_smooth1_slot_g)
-template <typename T, int DIM>
+template <typename T, int DIM, bool COMPACT>
 struct CzProl {
   int gt;
+  uint32_t f;
   T rv, ad, vp;
   T zl[DIM], zh[DIM], vq[DIM], al[DIM], ah[DIM];
   bool up[DIM], inq[DIM];
 };
 
-template <typename T, int DIM>
+template <typename T, int DIM, bool COMPACT>
 __device__ __forceinline__ void cz_prolong_gather(const Geo& g, const Geo& gc,
-                                                  const LevelPtrs<T>& P, const Coupling<T>& k,
-                                                  const T* zc0, const T* zc1, int t,
-                                                  CzProl<T, DIM>& v) {
+                                                  const LevelPtrs<T>& P, const T* zc0,
+                                                  const T* zc1, int t,
+                                                  CzProl<T, DIM, COMPACT>& v) {
   const Slot c = decode(g, t, 1);
   const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, c.q);
   v.gt = P.gt[1][t];
@@ -1717,26 +1748,36 @@ __device__ __forceinline__ void cz_prolong_gather(const Geo& g, const Geo& gc,
     v.zl[a] = z0[n.lo[a]];
     v.zh[a] = z0[n.hi[a]];
   }
-  cz_couplings<T, DIM>(P, k, 1, t, n, v.al, v.ah);
+  if constexpr (COMPACT) {
+    v.f = P.nb[1][t];
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      v.al[a] = P.Ax[a][0][n.lo[a]];
+      v.ah[a] = P.Ax[a][1][t];
+    }
+  }
 }
 
-template <typename T, int DIM>
-__device__ void cz_prolong_smooth1(const Geo& g, const Geo& gc, const LevelPtrs<T>& P,
-                                   const Coupling<T>& k, const T* zc0, const T* zc1) {
+template <typename T, int DIM, bool COMPACT>
+__device__ void cz_prolong_smooth1_t(const Geo& g, const Geo& gc, const LevelPtrs<T>& P,
+                                     const Coupling<T>& k, const T* zc0, const T* zc1) {
   const int n = g.size;
-  for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
-    CzProl<T, DIM> v[CZ_SLOTS];
+  constexpr int NS = cz_slots<COMPACT>();
+  for (int b = 0; b < n; b += NS * COARSE_THREADS) {
+    CzProl<T, DIM, COMPACT> v[NS];
 #pragma unroll
-    for (int s = 0; s < CZ_SLOTS; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
       v[s].gt = MG_SOLID;
-      if (t < n) cz_prolong_gather<T, DIM>(g, gc, P, k, zc0, zc1, t, v[s]);
+      if (t < n) cz_prolong_gather<T, DIM, COMPACT>(g, gc, P, zc0, zc1, t, v[s]);
     }
 #pragma unroll
-    for (int s = 0; s < CZ_SLOTS; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
       if (v[s].gt != MG_FLUID) continue;
-      T zl[DIM], zh[DIM];
+      T zl[DIM], zh[DIM], al[DIM], ah[DIM];
+      cz_coefs<T, DIM, COMPACT>(k, v[s].f, v[s].al, v[s].ah, al, ah);
 #pragma unroll
       for (int a = 0; a < DIM; ++a) {
         zl[a] = v[s].zl[a];
@@ -1749,12 +1790,18 @@ __device__ void cz_prolong_smooth1(const Geo& g, const Geo& gc, const LevelPtrs<
           zh[a] = zh[a] + v[s].vp;
         }
       }
-      T ret = nsum_vals<T, DIM>(v[s].al, v[s].ah, zl, zh);
+      T ret = nsum_vals<T, DIM>(al, ah, zl, zh);
       T num = v[s].rv - ret;
       P.z[1][t] = num / v[s].ad;
     }
   }
   __syncthreads();
+}
+template <typename T, int DIM>
+__device__ void cz_prolong_smooth1(const Geo& g, const Geo& gc, const LevelPtrs<T>& P,
+                                   const Coupling<T>& k, const T* zc0, const T* zc1) {
+  if (k.uniform) cz_prolong_smooth1_t<T, DIM, true>(g, gc, P, k, zc0, zc1);
+  else cz_prolong_smooth1_t<T, DIM, false>(g, gc, P, k, zc0, zc1);
 }
 
 // coarse_pre / coarse_bottom / coarse_post on the LDS image
@@ -1868,19 +1915,41 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
   };
   if constexpr (USE_LDS) {
     for (int k = 0; k < nlev; ++k) {
-      const LevelPtrs<T>& G = descs[d0 + k].P;
+      const LevelPtrs<T> G = descs[d0 + k].P;
       const LevelPtrs<T> P = lv(k);
       const int n = descs[d0 + k].g.size;
-      for (int t = threadIdx.x; t < n; t += COARSE_THREADS) {
+      const bool flags = G.meta != nullptr;
+      // all loads of a lane's slots in flight before the first LDS store
+      for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
+        T rv[CZ_SLOTS][2], ad[CZ_SLOTS][2];
+        int8_t gt[CZ_SLOTS][2];
+        uint16_t f[CZ_SLOTS][2];
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          P.r[c][t] = ldgi(G.r[c], t);
-          const_cast<T*>(P.Adiag[c])[t] = ldgi(G.Adiag[c], t);
-          const_cast<int8_t*>(P.gt[c])[t] = ldgi(G.gt[c], t);
-          if (G.meta != nullptr) const_cast<uint16_t*>(P.nb[c])[t] = ldgi(G.nb[c], t);
+        for (int s = 0; s < CZ_SLOTS; ++s) {
+          const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+          if (t >= n) continue;
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            rv[s][c] = ldgi(G.r[c], t);
+            ad[s][c] = ldgi(G.Adiag[c], t);
+            gt[s][c] = ldgi(G.gt[c], t);
+            f[s][c] = flags ? ldgi(G.nb[c], t) : (uint16_t)0;
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < CZ_SLOTS; ++s) {
+          const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+          if (t >= n) continue;
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            P.r[c][t] = rv[s][c];
+            const_cast<T*>(P.Adiag[c])[t] = ad[s][c];
+            const_cast<int8_t*>(P.gt[c])[t] = gt[s][c];
+            if (flags) const_cast<uint16_t*>(P.nb[c])[t] = f[s][c];
+          }
         }
       }
-      if (G.meta != nullptr && threadIdx.x == 0) *const_cast<LevelMeta*>(P.meta) = *G.meta;
+      if (flags && threadIdx.x == 0) *const_cast<LevelMeta*>(P.meta) = *G.meta;
     }
     __syncthreads();
   }
@@ -1890,18 +1959,24 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const LevelDesc<T>* _
     T* res[2];
     res[0] = (T*)(img + off);
     res[1] = (T*)(img + off + lds_align(descs[d0].g.size * (int)sizeof(T)));
+    // the geometry is read once per level into registers: a barrier orders
+    // memory, so fields read through the descriptor would be fetched again
+    // after every phase
     for (int k = 0; k < nlev - 1; ++k) {
-      const LevelDesc<T>& D = descs[d0 + k];
+      const Geo g = descs[d0 + k].g, gc = descs[d0 + k].gc;
       const LevelPtrs<T> P = lv(k);
       const LevelPtrs<T> C = lv(k + 1);
-      cz_pre<T, DIM>(D.g, P, nu);
-      cz_restrict<T, DIM>(D.g, P, coupling(P), D.gc, C.r[0], C.r[1], res);
+      cz_pre<T, DIM>(g, P, nu);
+      cz_restrict<T, DIM>(g, P, coupling(P), gc, C.r[0], C.r[1], res);
     }
-    cz_bottom<T, DIM>(descs[d0 + nlev - 1].g, lv(nlev - 1), bottom);
+    {
+      const Geo gb = descs[d0 + nlev - 1].g;
+      cz_bottom<T, DIM>(gb, lv(nlev - 1), bottom);
+    }
     for (int k = nlev - 2; k >= 0; --k) {
-      const LevelDesc<T>& D = descs[d0 + k];
+      const Geo g = descs[d0 + k].g, gc = descs[d0 + k].gc;
       const LevelPtrs<T> C = lv(k + 1);
-      cz_post<T, DIM>(D.g, D.gc, lv(k), C.z, nu);
+      cz_post<T, DIM>(g, gc, lv(k), C.z, nu);
     }
   } else {
     for (int k = 0; k < nlev - 1; ++k) {
