@@ -1701,109 +1701,6 @@ __device__ void cz_restrict(const Geo& gf, const LevelPtrs<T>& F, const Coupling
   __syncthreads();
 }
 
-// prolongate(l); smooth(l, 1) (mgpcg.py:120-123, 146-148) at every colour-1
-// slot: the colour-0 neighbours read z0 + z_coarse[parent] (the parent logic
-// of prolong_smooth1_slot_g)
-template <typename T, int DIM, bool COMPACT>
-struct CzProl {
-  int gt;
-  uint32_t f;
-  T rv, ad, vp;
-  T zl[DIM], zh[DIM], vq[DIM], al[DIM], ah[DIM];
-  bool up[DIM], inq[DIM];
-};
-
-template <typename T, int DIM, bool COMPACT>
-__device__ __forceinline__ void cz_prolong_gather(const Geo& g, const Geo& gc,
-                                                  const LevelPtrs<T>& P, const T* zc0,
-                                                  const T* zc1, int t,
-                                                  CzProl<T, DIM, COMPACT>& v) {
-  const Slot c = decode(g, t, 1);
-  const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, c.q);
-  v.gt = P.gt[1][t];
-  v.rv = P.r[1][t];
-  v.ad = P.Adiag[1][t];
-  const T* z0 = P.z[0];
-  const int pi = (DIM == 3 && c.i > 0) ? c.i >> 1 : 0;
-  const int pj = c.j > 0 ? c.j >> 1 : 0, pk = c.k > 0 ? c.k >> 1 : 0;
-  const int cp = colour_of(pi, pj, pk);
-  const int sp = min(slot_of(gc, pi, pj, pk), gc.size - 1);
-  const T* zcp = cp ? zc1 : zc0;
-  const T* zcq = cp ? zc0 : zc1;
-  v.vp = zcp[sp];
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    const int va = a + 3 - DIM;  // view axis of original axis a
-    const int x = va == 0 ? c.i : (va == 1 ? c.j : c.k);
-    const int nx = va == 0 ? g.n0 : (va == 1 ? g.n1 : g.n2);
-    v.up[a] = (x & 1) != 0;  // the other parent lies above (odd x) or below (even x)
-    const int d = v.up[a] ? 1 : -1;
-    v.inq[a] = x + d >= 0 && x + d < nx;
-    int sq;
-    if (va == 0) sq = sp + d * gc.sx;
-    else if (va == 1) sq = sp + d * gc.W;
-    else sq = sp - ((pk + 1) >> 1) + ((pk + d + 1) >> 1);
-    sq = (v.inq[a] && sq >= 0 && sq < gc.size) ? sq : sp;
-    v.vq[a] = zcq[sq];
-    v.zl[a] = z0[n.lo[a]];
-    v.zh[a] = z0[n.hi[a]];
-  }
-  if constexpr (COMPACT) {
-    v.f = P.nb[1][t];
-  } else {
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      v.al[a] = P.Ax[a][0][n.lo[a]];
-      v.ah[a] = P.Ax[a][1][t];
-    }
-  }
-}
-
-template <typename T, int DIM, bool COMPACT>
-__device__ void cz_prolong_smooth1_t(const Geo& g, const Geo& gc, const LevelPtrs<T>& P,
-                                     const Coupling<T>& k, const T* zc0, const T* zc1) {
-  const int n = g.size;
-  constexpr int NS = cz_slots<COMPACT>();
-  for (int b = 0; b < n; b += NS * COARSE_THREADS) {
-    CzProl<T, DIM, COMPACT> v[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
-      v[s].gt = MG_SOLID;
-      if (t < n) cz_prolong_gather<T, DIM, COMPACT>(g, gc, P, zc0, zc1, t, v[s]);
-    }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
-      if (v[s].gt != MG_FLUID) continue;
-      T zl[DIM], zh[DIM], al[DIM], ah[DIM];
-      cz_coefs<T, DIM, COMPACT>(k, v[s].f, v[s].al, v[s].ah, al, ah);
-#pragma unroll
-      for (int a = 0; a < DIM; ++a) {
-        zl[a] = v[s].zl[a];
-        zh[a] = v[s].zh[a];
-        if (v[s].up[a]) {
-          zl[a] = zl[a] + v[s].vp;
-          if (v[s].inq[a]) zh[a] = zh[a] + v[s].vq[a];
-        } else {
-          if (v[s].inq[a]) zl[a] = zl[a] + v[s].vq[a];
-          zh[a] = zh[a] + v[s].vp;
-        }
-      }
-      T ret = nsum_vals<T, DIM>(al, ah, zl, zh);
-      T num = v[s].rv - ret;
-      P.z[1][t] = num / v[s].ad;
-    }
-  }
-  __syncthreads();
-}
-template <typename T, int DIM>
-__device__ void cz_prolong_smooth1(const Geo& g, const Geo& gc, const LevelPtrs<T>& P,
-                                   const Coupling<T>& k, const T* zc0, const T* zc1) {
-  if (k.uniform) cz_prolong_smooth1_t<T, DIM, true>(g, gc, P, k, zc0, zc1);
-  else cz_prolong_smooth1_t<T, DIM, false>(g, gc, P, k, zc0, zc1);
-}
-
 // coarse_pre / coarse_bottom / coarse_post on the LDS image
 template <typename T, int DIM>
 __device__ void cz_pre(const Geo& g, const LevelPtrs<T>& P, int nu) {
@@ -1840,6 +1737,41 @@ __device__ void cz_bottom(const Geo& g, const LevelPtrs<T>& P, int bottom) {
   }
 }
 
+// prolongate(l) (mgpcg.py:120-123) on colour 0: the smooth(l, 1) that follows
+// reads these values, each z0 + z_coarse[parent] formed as the reference
+// forms it, and rewrites every colour-1 FLUID value without reading it.
+// (Colour-1 cells that are not FLUID keep z unprolongated, as do those of
+// the global-memory form: no FLUID result reads them -- their couplings are
+// 0 and the children of a cell that is not FLUID are not FLUID.)
+template <typename T>
+__device__ void cz_prolong0(const Geo& g, const Geo& gc, const LevelPtrs<T>& P, const T* zc0,
+                            const T* zc1) {
+  const int n = g.size;
+  for (int b = 0; b < n; b += CZ_SLOTS * COARSE_THREADS) {
+    T zv[CZ_SLOTS], pv[CZ_SLOTS];
+    bool in[CZ_SLOTS];
+#pragma unroll
+    for (int s = 0; s < CZ_SLOTS; ++s) {
+      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+      in[s] = false;
+      if (t >= n) continue;
+      const Slot sl = decode(g, t, 0);
+      if (!sl.interior) continue;
+      in[s] = true;
+      const int pi = sl.i >> 1, pj = sl.j >> 1, pk = sl.k >> 1;
+      const int ps = slot_of(gc, pi, pj, pk);
+      pv[s] = colour_of(pi, pj, pk) ? zc1[ps] : zc0[ps];
+      zv[s] = P.z[0][t];
+    }
+#pragma unroll
+    for (int s = 0; s < CZ_SLOTS; ++s) {
+      const int t = b + s * COARSE_THREADS + (int)threadIdx.x;
+      if (in[s]) P.z[0][t] = zv[s] + pv[s];
+    }
+  }
+  __syncthreads();
+}
+
 template <typename T, int DIM>
 __device__ void cz_post(const Geo& g, const Geo& gc, const LevelPtrs<T>& P, T* const* zc,
                         int nu) {
@@ -1848,7 +1780,8 @@ __device__ void cz_post(const Geo& g, const Geo& gc, const LevelPtrs<T>& P, T* c
     return;
   }
   const Coupling<T> k = coupling(P);
-  cz_prolong_smooth1<T, DIM>(g, gc, P, k, zc[0], zc[1]);
+  cz_prolong0<T>(g, gc, P, zc[0], zc[1]);
+  cz_half<T, DIM>(g, P, k, 1);
   cz_half<T, DIM>(g, P, k, 0);
   for (int i = 1; i < nu; ++i) {
     cz_half<T, DIM>(g, P, k, 1);
