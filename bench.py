@@ -452,6 +452,8 @@ def profile_summary(db_path, bench_path, command):
     total = sum(v[1] for v in agg.values())
     out = []
     w = out.append
+    w(f"# rocprofv3 kernel trace of the served workload: {line['config']['workload']}")
+    w("")
     w(f"command (from the repository root, on the GPU box, after "
       f"`cd /tmp && export TMPDIR=/tmp` in the shell):")
     w("")
@@ -499,7 +501,7 @@ def profile_summary(db_path, bench_path, command):
           f"{100 * t_co / total:.1f}% of kernel time")
     w("")
     w(f"kernel time inside the headline's timed window ({(t1 - t0) / 1e6:.3f} ms of wall "
-      f"time, {total / 1e3:.3f} ms of kernel time over both engine streams):")
+      f"time, {total / 1e3:.3f} ms of kernel time on all streams):")
     w("")
     w(f"{'kernel':<96} {'calls':>6} {'total_us':>10} {'avg_us':>9} {'pct':>6}")
     for k, (n_, t_) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:24]:
@@ -533,8 +535,12 @@ def traffic_summary(fetch_db, write_db, fetch_log, write_log, command):
     w("two counter runs of the same bench command (one counter each), from the repository")
     w("root on the GPU box after `cd /tmp && export TMPDIR=/tmp` in the shell:")
     w("")
+    wl = command.split("--workload", 1)[1].split()[0] if "--workload" in command else "c2"
+    out.insert(0, f"# HBM traffic of the level-0 half-sweep, {wl} workload "
+                  f"({WORKLOADS[wl]['n']}^3 {WORKLOADS[wl]['what']})")
+    out.insert(1, "")
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-        w(f"    timeout -s KILL 240 rocprofv3 --pmc {ctr} -d gpurun_out/pmc_{ctr} -o c2 -- "
+        w(f"    timeout -s KILL 240 rocprofv3 --pmc {ctr} -d gpurun_out/pmc_{wl}_{ctr} -o {wl} -- "
           f"python3 bench.py {command}")
     w("")
     w("--probe-warmup --probe-every 1 make the bench probe every dispatch of the level-0")
