@@ -1094,7 +1094,10 @@ __global__ void __launch_bounds__(BLOCK) k_stage(const LevelDesc<T>* __restrict_
       continue;
     }
     int lb = it.vb - ic.block0;
-    if constexpr (OP == ST_PROLONG_SMOOTH1) {  // before any per-lane exit: the scan reads lanes
+    // level 0's blocks without a FLUID cell keep z = 0 from their packing: no
+    // kernel writes them (a FLUID cell's neighbours there have zero couplings)
+    if constexpr (OP == ST_PROLONG_SMOOTH1 || (OP == ST_PROLONGATE0 && L0 > 0)) {
+      // before any per-lane exit: the scan reads lanes
       const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
       if (e >= 0) continue;
       lb = e & ORD_BLOCK;
@@ -2208,11 +2211,7 @@ __global__ void __launch_bounds__(BLOCK, full_occupancy<T>()) k_cg_xr(const SysD
     run_mark(rs, D.partials, it.vb - ic.block0);
     const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
     const int t = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
-    if (e >= 0) {
-      // x and r change on FLUID only; the first half-sweep's z0 is 0 here
-      if (t < D.g.size && !refine) stgi(D.P.z[0], t, (T)0);
-      continue;
-    }
+    if (e >= 0) continue;  // x and r change on FLUID only; z0 stays 0 here
     DD& acc = rs.acc;
     if (t < D.g.size) {
       // both colours' operands are loaded before either colour is stored
@@ -2299,11 +2298,7 @@ __global__ void __launch_bounds__(BLOCK, full_occupancy<T>()) k_true_residual(co
     run_mark(rs, D.partials, it.vb - ic.block0);
     const int32_t e = ord_here(ws, iter, it.vb, it.step, it.end, ic.block0, ic.end, D.P, D.g, 1);
     const int t = (e & ORD_BLOCK) * BLOCK + threadIdx.x;
-    if (e >= 0) {
-      // r changes on FLUID only; the first half-sweep's z0 is 0 here
-      if (t < D.g.size) stgi(D.P.z[0], t, (T)0);
-      continue;
-    }
+    if (e >= 0) continue;  // r changes on FLUID only; z0 stays 0 here
     DD& acc = rs.acc;
     if (t < D.g.size) {
 #pragma unroll
