@@ -107,9 +107,10 @@ def parse():
     ap.add_argument("--autotune", action="store_true",
                     help="first tune the schedule knobs on this workload (autotune.py) and "
                          "measure with the tuned settings")
-    ap.add_argument("--copy-calibration-mib", type=int, default=0,
-                    help="first run a plain copy kernel over this many MiB (3 launches), the "
-                         "known-size reference for HBM byte counters")
+    ap.add_argument("--copy-calibration-mib", type=int, default=256,
+                    help="first run a plain copy kernel over this many MiB (3 launches of each "
+                         "access width), the known-size reference for HBM byte counters; 0: "
+                         "none")
     a = ap.parse_args()
     w = WORKLOADS[a.workload]
     for k in ("n", "levels", "seeds", "concurrency"):
@@ -536,17 +537,21 @@ def traffic_summary(fetch_db, write_db, fetch_log, write_log, command):
     w("root on the GPU box after `cd /tmp && export TMPDIR=/tmp` in the shell:")
     w("")
     wl = command.split("--workload", 1)[1].split()[0] if "--workload" in command else "c2"
-    out.insert(0, f"# HBM traffic of the level-0 half-sweep, {wl} workload "
-                  f"({WORKLOADS[wl]['n']}^3 {WORKLOADS[wl]['what']})")
+    tag = wl if "--workload" in command else "head"  # the driver's headline command
+    where = (f"in the headline command ({wl} workload, " if tag == "head"
+             else f", {wl} workload (")
+    out.insert(0, f"# HBM traffic of the level-0 half-sweep {where}"
+                  f"{WORKLOADS[wl]['n']}^3 {WORKLOADS[wl]['what']})".replace(" , ", ", "))
     out.insert(1, "")
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-        w(f"    timeout -s KILL 240 rocprofv3 --pmc {ctr} -d gpurun_out/pmc_{wl}_{ctr} -o {wl} -- "
+        w(f"    timeout -s KILL 900 rocprofv3 --pmc {ctr} -d gpurun_out/pmc_{tag}_{ctr} -o {wl} -- "
           f"python3 bench.py {command}")
     w("")
-    w("--probe-warmup --probe-every 1 make the bench probe every dispatch of the level-0")
-    w("half-sweep smooth(0, 0) in the process (instantiation mg::k_stage<float, 3, 0, 2>)")
-    w("and count, on the device, the level-0 cells and FLUID cells those dispatches swept,")
-    w("so the counter totals and the cell totals cover the same dispatches.")
+    w("The bench probes one dispatch in --probe-every of the level-0 half-sweep smooth(0, 0)")
+    w("(instantiation mg::k_stage<float, 3, 0, 2>) inside the headline's timed region (with")
+    w("--probe-warmup from the warm-up on) and counts, on the device, the level-0 cells and")
+    w("FLUID cells those dispatches swept. They are the first `launches` dispatches of that")
+    w("instantiation in the process, so the counters are summed over exactly those.")
     w("")
     res = {}
     for ctr, db, log in (("FETCH_SIZE", fetch_db, fetch_log), ("WRITE_SIZE", write_db, write_log)):
@@ -555,11 +560,15 @@ def traffic_summary(fetch_db, write_db, fetch_log, write_log, command):
         rf = line["roofline"]
         cal_v = [v for n_, v in rows if "MulFunctor" in n_][:3]
         cal_s = [v for n_, v in rows if "CUDAFunctorOnSelf_add" in n_][:3]
-        mib = line["config"].get("copy_calibration_mib") or 1024
+        mib = line["config"]["copy_calibration_mib"]
         kib = mib * 1024.0
         sv = kib / (sum(cal_v) / len(cal_v))
         ss = kib / (sum(cal_s) / len(cal_s))
-        k = [v for n_, v in rows if "k_stage<float, 3, 0, 2>" in n_]
+        # the bench's cell counts cover its first `launches` probed dispatches
+        # (the headline's timed region runs first; with --probe-warmup its
+        # warm-up too)
+        k = [v for n_, v in rows if "k_stage<float, 3, 0, 2>" in n_][:rf["launches"]]
+        assert len(k) == rf["launches"], (len(k), rf["launches"])
         res[ctr] = {"scale_vec": sv, "scale_scalar": ss, "dispatches": len(k),
                     "launches": rf["launches"], "cells": rf["cells_total"],
                     "fluid": rf["fluid_total"], "kib": sum(k)}
@@ -673,8 +682,7 @@ def main():
                    "admit_per_step": args.admit_per_step, "server_depth": args.server_depth,
                    "server_streams": args.server_streams,
                    "assembly": "canonical" if args.canonical_assembly else "packed",
-                   **({"copy_calibration_mib": args.copy_calibration_mib}
-                      if args.copy_calibration_mib else {})},
+                   "copy_calibration_mib": args.copy_calibration_mib},
         **({"autotune": {"knobs": tuned["knobs"], "solves_per_s": tuned["solves_per_s"],
                          "start": tuned["start"], "trials": len(tuned["trials"])}}
            if tuned else {}),
