@@ -72,7 +72,8 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
                                         solved by one workgroup per system (default 4096);
                                         applies to handles created afterwards */
 #define MGPCG_OPT_MAX_GRID 4         /* workgroups of a persistent stage launch (default 2048) */
-#define MGPCG_OPT_FUSE_FIRST 5    /* 1: fill(0)+smooth(0)+smooth(1) in one kernel (default 0) */
+#define MGPCG_OPT_FUSE_FIRST 5    /* fill(0)+smooth(0)+smooth(1) in one kernel: 1 on every
+                                     level, 2 on the levels below level 0 (default 0) */
 #define MGPCG_OPT_FUSE_PROLONG 6  /* 1: prolongate inside the next smooth(1) (default 1) */
 #define MGPCG_OPT_FUSE_P 7        /* 1: update_p inside compute_Ap (default 0) */
 #define MGPCG_OPT_COMPACT 8       /* 1: levels whose nonzero off-diagonals are all one value
@@ -96,6 +97,9 @@ int mgpcg_set_stream(mgpcg_solver_t* solver, void* stream);
                                      replays it until the set or an option changes (default
                                      0); the launch probe and mgpcg_solve_hostsync launch
                                      kernel by kernel */
+#define MGPCG_OPT_RESTRICT_COARSE 15 /* restriction kernels on the levels below level 0, the
+                                        modes of MGPCG_OPT_RESTRICT_ROWS; -1 (default): the
+                                        MGPCG_OPT_RESTRICT_ROWS mode */
 int mgpcg_set_option(mgpcg_solver_t* solver, int option, int value);
 /* The current value of a knob (no reference counterpart; `solver` is unused). */
 int mgpcg_get_option(const mgpcg_solver_t* solver, int option, int* value);

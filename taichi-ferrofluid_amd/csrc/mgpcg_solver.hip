@@ -58,7 +58,7 @@ struct LevelBuf {
 
 int g_coarse_max_slots = 4096;  // levels at most this large go to k_coarse
 int g_max_grid = 2048;          // workgroups of a persistent stage launch
-int g_fuse_first = 0;    // first pre-smoothing pair in one kernel (else two)
+int g_fuse_first = 0;    // first pre-smoothing pair in one kernel: 1 every level, 2 levels >= 1
 int g_fuse_prolong = 1;  // prolongation inside the first post-smoothing half-sweep
 int g_fuse_p = 0;        // update_p inside compute_Ap
 int g_compact = 1;       // flag-word stencils on levels with uniform couplings
@@ -66,6 +66,7 @@ int g_coarse_lds = 1;    // coarse kernel works on an LDS image of its levels wh
 // restriction kernels: 0 one lane per child, 1 row-tiled with LDS, 2 residual
 // pass + child gather (both coalesced)
 int g_restrict_mode = 2;
+int g_restrict_coarse = -1;  // restriction kernels on levels >= 1 (-1: g_restrict_mode)
 int g_probe_every = 4;    // the launch probe times one eligible launch in this many
 int g_server_depth = 2;   // server iterations queued on the device ahead of the host
 int g_server_streams = 2; // engines (streams) a server spreads its members over
@@ -217,7 +218,8 @@ struct Engine {
   StageRef st_init, st_cg, fin_init, fin_cg, fin_all, st_dot_fused, st_dot_plain, st_coarse,
       st_gather;
   bool coarse_lds = false;  // every coarse item's levels fit the coarse kernel's LDS image
-  int restrict_mode = 0;  // restriction stages are laid out for this g_restrict_mode
+  int restrict_mode = 0;    // restriction stages of level 0 are laid out for this mode
+  int restrict_mode_c = 0;  // ... and those of the levels below it for this one
   std::vector<StageRef> st_slot, st_restrict, st_bottom;  // per level
   // the half-sweep stages: virtual blocks of BLOCK * SLOT_LANES slots
   std::vector<StageRef> st_sweep, st_bottom_sweep;
@@ -499,10 +501,17 @@ int engine_add(Engine* e, mgpcg_solver* s, int64_t tag, const SolveParams& prm) 
 
 // virtual blocks of one system's restriction into the coarse level gc, for the
 // kernel the engine's current layout selects
-inline int restrict_blocks(const Engine* e, const Geo& gc) {
-  if (e->restrict_mode == 1) return gc.n0 * gc.n1;
-  if (e->restrict_mode == 2) return nblk(gc.size);
+inline int restrict_blocks(int mode, const Geo& gc) {
+  if (mode == 1) return gc.n0 * gc.n1;
+  if (mode == 2) return nblk(gc.size);
   return rblk(gc.size);
+}
+inline int restrict_blocks(const Engine* e, const Geo& gc) {
+  return restrict_blocks(e->restrict_mode, gc);
+}
+// the restriction mode of level l (the fine level) in the engine's layout
+inline int restrict_mode_of(const Engine* e, int l) {
+  return l == 0 ? e->restrict_mode : e->restrict_mode_c;
 }
 
 // Rebuild the item lists of every stage from the member set.
@@ -527,9 +536,13 @@ int engine_build_items(Engine* e) {
   const int L = e->L;
   // row-tiled restriction only when every member's fine rows fit its LDS tile
   e->restrict_mode = g_restrict_mode;
+  e->restrict_mode_c = g_restrict_coarse >= 0 ? g_restrict_coarse : g_restrict_mode;
   if (e->restrict_mode == 1)
     for (const Member& M : e->mem)
       if (M.active && M.s->geo[0].n2 > RESTRICT_ROW_MAX) e->restrict_mode = 0;
+  if (e->restrict_mode_c == 1)
+    for (const Member& M : e->mem)
+      if (M.active && M.s->L > 1 && M.s->geo[1].n2 > RESTRICT_ROW_MAX) e->restrict_mode_c = 0;
   auto sysidx = [](int m) { return m; };
   auto lvl0 = [](const Member& M) { return nblk(M.s->geo[0].size); };
   auto one = [](const Member&) { return 1; };
@@ -550,7 +563,7 @@ int engine_build_items(Engine* e) {
               [l, L](const Member& M) { return l < M.s->coarse_from && l + 1 < L; },
               [l, L, e](const Member& M) {
                 if (l + 1 >= L) return 0;
-                return restrict_blocks(e, M.s->geo[l + 1]);
+                return restrict_blocks(restrict_mode_of(e, l), M.s->geo[l + 1]);
               },
               [l, L](int m) { return m * L + l; });
   }
@@ -673,10 +686,10 @@ struct Sched {
   // fine: the level's slot stage (the residual pass), r: the restriction
   // stage; sweep (not NULL): the level's half-sweep stage, the last
   // smooth(l, 1) having written the colour-1 residual (ST_SMOOTH1_RES)
-  static void restrict_(Engine* e, const Engine::StageRef& fine, const Engine::StageRef& r,
-                        const Engine::StageRef* sweep = nullptr) {
+  static void restrict_(Engine* e, int mode, const Engine::StageRef& fine,
+                        const Engine::StageRef& r, const Engine::StageRef* sweep = nullptr) {
     if (r.nblocks == 0) return;
-    if (e->restrict_mode == 2) {
+    if (mode == 2) {
       if (sweep)
         hipLaunchKernelGGL((k_residual<T, DIM, true>), dim3(grid_of(*sweep)), dim3(BLOCK), 0,
                            e->stream, levels(e), stage_of(e, *sweep));
@@ -685,7 +698,7 @@ struct Sched {
                            e->stream, levels(e), stage_of(e, fine));
       hipLaunchKernelGGL((k_restrict_gather<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0,
                          e->stream, levels(e), stage_of(e, r));
-    } else if (e->restrict_mode == 1) {
+    } else if (mode == 1) {
       hipLaunchKernelGGL((k_restrict_rows<T, DIM>), dim3(grid_of(r)), dim3(BLOCK), 0, e->stream,
                          levels(e), stage_of(e, r));
     } else {
@@ -710,9 +723,11 @@ struct Sched {
       const Engine::StageRef& W = e->st_sweep[l];
       // the last pre-smoothing smooth(l, 1) also forms the colour-1 residual
       // when the restriction is the two-pass one and that half-sweep exists
-      const bool res1 = e->restrict_mode == 2 && nu > 0 && !(g_fuse_first && nu == 1);
+      const int rm = restrict_mode_of(e, l);
+      const bool ff = g_fuse_first == 1 || (g_fuse_first == 2 && l > 0);
+      const bool res1 = rm == 2 && nu > 0 && !(ff && nu == 1);
       if (nu > 0) {
-        if (g_fuse_first) {
+        if (ff) {
           stage<ST_PAIR_FROM_ZERO>(e, S, l == 0);
         } else {
           if (!(first0 && l == 0)) stage<ST_FIRST>(e, S, l == 0);
@@ -727,7 +742,7 @@ struct Sched {
       } else {
         stage<ST_ZERO>(e, S, l == 0);
       }
-      restrict_(e, e->st_slot[l], e->st_restrict[l], res1 ? &W : nullptr);
+      restrict_(e, rm, e->st_slot[l], e->st_restrict[l], res1 ? &W : nullptr);
     }
     // bottom level on the batched kernels (only when it is too big for k_coarse)
     const Engine::StageRef& B = e->st_bottom[0];
@@ -1319,7 +1334,8 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
       g_max_grid = value;
       return 0;
     case MGPCG_OPT_FUSE_FIRST:
-      g_fuse_first = value != 0;
+      MG_REQUIRE(value >= 0 && value <= 2, "first-pair fusion must be 0, 1 or 2");
+      g_fuse_first = value;
       return 0;
     case MGPCG_OPT_FUSE_PROLONG:
       g_fuse_prolong = value != 0;
@@ -1336,6 +1352,10 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
     case MGPCG_OPT_RESTRICT_ROWS:
       MG_REQUIRE(value >= 0 && value <= 2, "restriction mode must be 0, 1 or 2");
       g_restrict_mode = value;
+      return 0;
+    case MGPCG_OPT_RESTRICT_COARSE:
+      MG_REQUIRE(value >= -1 && value <= 2, "coarse restriction mode must be -1, 0, 1 or 2");
+      g_restrict_coarse = value;
       return 0;
     case MGPCG_OPT_PROBE_EVERY:
       MG_REQUIRE(value >= 1, "probe period must be >= 1");
@@ -1372,6 +1392,7 @@ extern "C" int mgpcg_get_option(const mgpcg_solver_t* s, int option, int* value)
     case MGPCG_OPT_COMPACT: *value = g_compact; return 0;
     case MGPCG_OPT_COARSE_LDS: *value = g_coarse_lds; return 0;
     case MGPCG_OPT_RESTRICT_ROWS: *value = g_restrict_mode; return 0;
+    case MGPCG_OPT_RESTRICT_COARSE: *value = g_restrict_coarse; return 0;
     case MGPCG_OPT_PROBE_EVERY: *value = g_probe_every; return 0;
     case MGPCG_OPT_SERVER_DEPTH: *value = g_server_depth; return 0;
     case MGPCG_OPT_SERVER_STREAMS: *value = g_server_streams; return 0;
@@ -1675,7 +1696,7 @@ struct RefPath {
           Sd::template stage<ST_SMOOTH1>(e, sweep(l));
         }
         fill(l + 1, 3);  // r[l+1].fill(0); z[l+1].fill(0)
-        Sd::restrict_(e, lvl(l, false), lvl(l, true));
+        Sd::restrict_(e, e->restrict_mode, lvl(l, false), lvl(l, true));
       }
       const int half = s->bottom / 2;
       for (int i = 0; i < half; ++i) {
@@ -1886,7 +1907,7 @@ struct RunOp {
       case MGPCG_OP_RESTRICT: {
         Engine::StageRef fine = r;
         fine.nblocks = r.fine_nblocks;
-        Sd::restrict_(e, fine, r);
+        Sd::restrict_(e, e->restrict_mode, fine, r);
         return 0;
       }
       case MGPCG_OP_PROLONGATE:

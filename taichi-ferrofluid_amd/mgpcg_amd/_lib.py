@@ -19,7 +19,8 @@ FIELD_R, FIELD_Z, FIELD_P, FIELD_AP, FIELD_X, FIELD_P_NEXT = 0, 1, 2, 3, 4, 5
  OP_PROLONG_SMOOTH1) = range(7)
 (OPT_COARSE_MAX_SLOTS, OPT_MAX_GRID, OPT_FUSE_FIRST, OPT_FUSE_PROLONG, OPT_FUSE_P,
  OPT_COMPACT, OPT_COARSE_LDS, OPT_RESTRICT_ROWS, OPT_PROBE_EVERY,
- OPT_SERVER_DEPTH, OPT_SERVER_STREAMS, OPT_GRAPH) = 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14
+ OPT_SERVER_DEPTH, OPT_SERVER_STREAMS, OPT_GRAPH,
+ OPT_RESTRICT_COARSE) = 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int

@@ -9,7 +9,9 @@ out on the device.
                   one-workgroup coarse kernel (read when a handle is created)
   restrict_rows   restriction: 2 residual pass + child gather, 1 row-tiled
                   with the residuals in LDS, 0 one lane per child
-  fuse_first      fill(0) + smooth(0) + smooth(1) in one launch
+  restrict_coarse the same on the levels below level 0 (-1: as restrict_rows)
+  fuse_first      fill(0) + smooth(0) + smooth(1) in one launch (1 every
+                  level, 2 the levels below level 0)
   fuse_prolong    prolongation inside the next smooth(1)
   fuse_p          update_p inside compute_Ap
   server_streams  engine lanes of a server (read when a server is created)
@@ -34,6 +36,7 @@ OPTIONS = {
     "max_grid": _lib.OPT_MAX_GRID,
     "coarse_max": _lib.OPT_COARSE_MAX_SLOTS,
     "restrict_rows": _lib.OPT_RESTRICT_ROWS,
+    "restrict_coarse": _lib.OPT_RESTRICT_COARSE,
     "fuse_first": _lib.OPT_FUSE_FIRST,
     "fuse_prolong": _lib.OPT_FUSE_PROLONG,
     "fuse_p": _lib.OPT_FUSE_P,
@@ -46,7 +49,8 @@ SPACE = {
     "max_grid": [512, 1024, 2048, 4096],
     "coarse_max": [1024, 4096, 16384],
     "restrict_rows": [2, 1, 0],
-    "fuse_first": [0, 1],
+    "restrict_coarse": [-1, 0, 1, 2],
+    "fuse_first": [0, 1, 2],
     "fuse_prolong": [1, 0],
     "fuse_p": [0, 1],
     "server_streams": [1, 2, 3],

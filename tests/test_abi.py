@@ -83,7 +83,8 @@ def test_constants_match_header():
              "OPT_RESTRICT_ROWS": "MGPCG_OPT_RESTRICT_ROWS",
              "OPT_PROBE_EVERY": "MGPCG_OPT_PROBE_EVERY",
              "OPT_SERVER_DEPTH": "MGPCG_OPT_SERVER_DEPTH",
-             "OPT_SERVER_STREAMS": "MGPCG_OPT_SERVER_STREAMS", "OPT_GRAPH": "MGPCG_OPT_GRAPH"}
+             "OPT_SERVER_STREAMS": "MGPCG_OPT_SERVER_STREAMS", "OPT_GRAPH": "MGPCG_OPT_GRAPH",
+             "OPT_RESTRICT_COARSE": "MGPCG_OPT_RESTRICT_COARSE"}
     for py, c in pairs.items():
         assert getattr(_lib, py) == int(defs[c]), (py, c)
     # every #define of the header is mirrored

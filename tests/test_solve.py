@@ -101,7 +101,8 @@ def test_reference_structured_equals_device_loop(dev):
 
 
 KNOB_DEFAULTS = {"coarse_max": 4096, "max_grid": 2048, "fuse_first": 0, "fuse_prolong": 1,
-                 "fuse_p": 0, "compact": 1, "coarse_lds": 1, "restrict_rows": 2}
+                 "fuse_p": 0, "compact": 1, "coarse_lds": 1, "restrict_rows": 2,
+                 "restrict_coarse": -1}
 
 
 def set_knobs(**kw):
@@ -112,6 +113,7 @@ def set_knobs(**kw):
            "fuse_first": _lib.OPT_FUSE_FIRST, "fuse_prolong": _lib.OPT_FUSE_PROLONG,
            "fuse_p": _lib.OPT_FUSE_P, "compact": _lib.OPT_COMPACT,
            "coarse_lds": _lib.OPT_COARSE_LDS, "restrict_rows": _lib.OPT_RESTRICT_ROWS,
+           "restrict_coarse": _lib.OPT_RESTRICT_COARSE,
            "probe_every": _lib.OPT_PROBE_EVERY, "server_depth": _lib.OPT_SERVER_DEPTH,
            "server_streams": _lib.OPT_SERVER_STREAMS}
     for k, v in kw.items():
@@ -141,7 +143,8 @@ def test_smoothing_counts_match_oracle(dev, nu, fuse_first):
 # single-knob settings that a combined setting below also exercises (the
 # emulator runs only the combined one; the GPU runs both)
 EMU_COVERED = [{"max_grid": 64}, {"fuse_first": 1}, {"fuse_p": 1}, {"coarse_max": 1024},
-               {"compact": 0}, {"coarse_lds": 0}, {"restrict_rows": 1}, {"restrict_rows": 0}]
+               {"compact": 0}, {"coarse_lds": 0}, {"restrict_rows": 1}, {"restrict_rows": 0},
+               {"fuse_first": 2}, {"restrict_coarse": 0}]
 
 
 @pytest.mark.parametrize("knobs", [
@@ -152,6 +155,8 @@ EMU_COVERED = [{"max_grid": 64}, {"fuse_first": 1}, {"fuse_p": 1}, {"coarse_max"
     {"coarse_lds": 0}, {"coarse_lds": 0, "compact": 0, "fuse_prolong": 1},
     {"restrict_rows": 1}, {"restrict_rows": 1, "compact": 0, "max_grid": 8},
     {"restrict_rows": 0}, {"restrict_rows": 0, "compact": 0, "fuse_prolong": 0},
+    {"fuse_first": 2}, {"restrict_coarse": 0}, {"restrict_coarse": 1},
+    {"fuse_first": 2, "restrict_coarse": 0, "compact": 0},
 ], ids=lambda k: "-".join(f"{a}{b}" for a, b in k.items()))
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
 def test_schedule_knobs_agree(dev, knobs, precision):
