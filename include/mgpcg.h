@@ -207,12 +207,16 @@ int mgpcg_batch_solve(mgpcg_batch_t* batch, mgpcg_solver_t* const* systems, int 
 
 /* ---------------------------------------------------------------------------
  * Server: continuous batching.  mgpcg_server_submit queues an assembled
- * system; each mgpcg_server_step admits the queued systems at the iteration
- * boundary, enqueues one iteration for every member and retires the members
- * whose state (read one iteration behind) says they are done; their results
- * are returned by mgpcg_server_poll with the submit tag.  A system must not
- * be modified between submit and its result.  Admission policy (HBM budget)
- * is the caller's: it decides when to create and submit systems.
+ * system: its assembly is the work enqueued on the system's stream
+ * (mgpcg_set_stream) before the call, and systems may be assembled on
+ * different streams.  Each mgpcg_server_step admits queued systems at the
+ * iteration boundary -- while others are running, those whose assembly has
+ * finished on the device; when none is running, all of them -- enqueues one
+ * iteration for every member and retires the members whose state (read one
+ * iteration behind) says they are done; their results are returned by
+ * mgpcg_server_poll with the submit tag.  A system must not be modified
+ * between submit and its result.  Admission policy (HBM budget) is the
+ * caller's: it decides when to create and submit systems.
  * ------------------------------------------------------------------------- */
 typedef struct mgpcg_server mgpcg_server_t;
 int mgpcg_server_create(int max_iters, double rel_tol, double abs_tol, double eps,

@@ -103,6 +103,9 @@ struct mgpcg_solver {
   // engine, waits for this event before it resets the state.
   hipEvent_t ev_retired = nullptr;
   bool retired_pending = false;
+  // Recorded on the system's stream when it is submitted to a server: its
+  // assembly is complete when this event is.
+  hipEvent_t ev_submitted = nullptr;
 };
 
 namespace {
@@ -173,6 +176,25 @@ SysDesc<T, TX> sys_desc(const mgpcg_solver* s, int desc0) {
   return d;
 }
 
+// Host-to-device copies of the engine's tables (descriptors, initial states,
+// work items), read by one kernel from their mapped pinned staging: each
+// copied as a DMA of its own cost a dispatch with tens of microseconds of
+// idle device before it.  One workgroup per copy; sizes in 4-byte words.
+struct PullList {
+  static const int N = 32;
+  const uint32_t* src[N];
+  uint32_t* dst[N];
+  int words[N];
+  int n;
+};
+__global__ void __launch_bounds__(256) k_pull(PullList pl) {
+  const int i = blockIdx.x;
+  if (i >= pl.n) return;
+  const uint32_t* src = pl.src[i];
+  uint32_t* dst = pl.dst[i];
+  for (int w = threadIdx.x; w < pl.words[i]; w += 256) dst[w] = src[w];
+}
+
 // gather the CG states of the listed systems into one contiguous array
 template <typename T, typename TX>
 __global__ void k_gather_states(const SysDesc<T, TX>* __restrict__ sys,
@@ -180,6 +202,7 @@ __global__ void k_gather_states(const SysDesc<T, TX>* __restrict__ sys,
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n) return;
   out[i] = *sys[items[i].desc].st;
+  __threadfence_system();  // `out` is the host's mapped readback slot
 }
 
 }  // namespace
@@ -227,6 +250,7 @@ struct Engine {
   Item* d_items = nullptr;
   static const int RING = 4;
   Item* h_items[RING] = {nullptr, nullptr, nullptr, nullptr};
+  Item* hd_items[RING] = {nullptr, nullptr, nullptr, nullptr};  // their device addresses
   hipEvent_t ev_items[RING] = {nullptr, nullptr, nullptr, nullptr};
   int items_slot = 0;
   bool dirty = true;
@@ -236,13 +260,21 @@ struct Engine {
   // state readbacks in flight (a ring: the server keeps several iterations queued)
   static const int NSLOT = 4;
   CGState* h_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
+  // the same pinned slots as the device addresses them: the gather kernel
+  // writes the states there itself (no copy dispatched after it)
+  CGState* hd_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t ev_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<std::pair<int, uint32_t>> gathered[NSLOT];  // (member slot, generation)
   long iteration = 0;
 
-  // pinned staging of admissions (per member slot): descriptors, initial state
+  // pinned staging of admissions (per member slot): descriptors, initial state,
+  // and the device addresses of the same (mapped) memory
   char* h_desc = nullptr;
   CGState* h_stinit = nullptr;
+  char* hd_desc = nullptr;
+  CGState* hd_stinit = nullptr;
+  // table copies queued for the next k_pull launch
+  PullList pulls{};
 
   // launch-duration probe of the level-0 half-sweeps, timed on the device
   bool probe_on = false;
@@ -327,6 +359,25 @@ int engine_init(Engine* e) {
 }
 
 // ordered after the caller's stream, and the caller's stream after us
+// Queue a copy of `bytes` (a multiple of 4) from mapped pinned staging (its
+// device address) into a device table; engine_flush_pulls launches the queue.
+int engine_flush_pulls(Engine* e) {
+  if (e->pulls.n == 0) return 0;
+  hipLaunchKernelGGL(k_pull, dim3(e->pulls.n), dim3(256), 0, e->stream, e->pulls);
+  e->pulls.n = 0;
+  MG_HIP(hipGetLastError());
+  return 0;
+}
+int engine_pull(Engine* e, void* dst, const void* src_dev, size_t bytes) {
+  MG_REQUIRE(bytes % 4 == 0, "table copy of %zu bytes is not a whole number of words", bytes);
+  if (e->pulls.n == PullList::N && engine_flush_pulls(e)) return -2;
+  const int i = e->pulls.n++;
+  e->pulls.src[i] = (const uint32_t*)src_dev;
+  e->pulls.dst[i] = (uint32_t*)dst;
+  e->pulls.words[i] = (int)(bytes / 4);
+  return 0;
+}
+
 int engine_begin(Engine* e) {
   MG_HIP(hipSetDevice(e->device));
   MG_HIP(hipEventRecord(e->ev_user, e->user_stream));
@@ -351,6 +402,8 @@ size_t sys_desc_size(int precision) {
 int engine_reserve(Engine* e, int cap) {
   if (cap <= e->cap) return 0;
   int ncap = std::max(cap, std::max(8, 2 * e->cap));
+  // queued copies name the old tables and staging: they run first
+  if (engine_flush_pulls(e)) return -2;
   MG_HIP(hipStreamSynchronize(e->stream));
   char *nl = nullptr, *ns = nullptr;
   MG_HIP(hipMalloc(&nl, e->level_sz * (size_t)ncap * e->L));
@@ -368,17 +421,24 @@ int engine_reserve(Engine* e, int cap) {
   MG_HIP(hipMalloc(&e->d_states, sizeof(CGState) * (size_t)ncap));
   for (int i = 0; i < Engine::NSLOT; ++i) {
     if (e->h_states[i]) MG_HIP(hipHostFree(e->h_states[i]));
-    MG_HIP(hipHostMalloc((void**)&e->h_states[i], sizeof(CGState) * (size_t)ncap, 0));
+    MG_HIP(hipHostMalloc((void**)&e->h_states[i], sizeof(CGState) * (size_t)ncap,
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    MG_HIP(hipHostGetDevicePointer((void**)&e->hd_states[i], e->h_states[i], 0));
   }
   const size_t dbytes = e->level_sz * e->L + e->sys_sz;
   char* nd = nullptr;
   CGState* nst = nullptr;
-  MG_HIP(hipHostMalloc((void**)&nd, dbytes * (size_t)ncap, 0));
-  MG_HIP(hipHostMalloc((void**)&nst, sizeof(CGState) * (size_t)ncap, 0));
+  // coherent: the device reads them where they lie, never from a cached copy
+  MG_HIP(hipHostMalloc((void**)&nd, dbytes * (size_t)ncap,
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  MG_HIP(hipHostMalloc((void**)&nst, sizeof(CGState) * (size_t)ncap,
+                       hipHostMallocMapped | hipHostMallocCoherent));
   if (e->h_desc) MG_HIP(hipHostFree(e->h_desc));
   if (e->h_stinit) MG_HIP(hipHostFree(e->h_stinit));
   e->h_desc = nd;
   e->h_stinit = nst;
+  MG_HIP(hipHostGetDevicePointer((void**)&e->hd_desc, e->h_desc, 0));
+  MG_HIP(hipHostGetDevicePointer((void**)&e->hd_stinit, e->h_stinit, 0));
   e->cap = ncap;
   return 0;
 }
@@ -413,10 +473,11 @@ int engine_write_desc(Engine* e, int m) {
     SysDesc<float, float> d = sys_desc<float, float>(s, desc0);
     memcpy(sd, &d, sizeof(d));
   }
-  MG_HIP(hipMemcpyAsync(e->d_levels + e->level_sz * (size_t)m * e->L, buf, e->level_sz * e->L,
-                        hipMemcpyHostToDevice, e->stream));
-  MG_HIP(hipMemcpyAsync(e->d_sys + e->sys_sz * (size_t)m, sd, e->sys_sz, hipMemcpyHostToDevice,
-                        e->stream));
+  const char* dbuf = e->hd_desc + (buf - e->h_desc);
+  if (engine_pull(e, e->d_levels + e->level_sz * (size_t)m * e->L, dbuf, e->level_sz * e->L))
+    return -2;
+  if (engine_pull(e, e->d_sys + e->sys_sz * (size_t)m, dbuf + e->level_sz * e->L, e->sys_sz))
+    return -2;
   return 0;
 }
 
@@ -463,8 +524,7 @@ int engine_reset_state(Engine* e, int m, const SolveParams& prm) {
   init.precision = s->precision;
   init.phase = 0;
   init.true_rr = -1.0;
-  MG_HIP(hipMemcpyAsync(s->st, &init, sizeof(CGState), hipMemcpyHostToDevice, e->stream));
-  return 0;
+  return engine_pull(e, s->st, e->hd_stinit + m, sizeof(CGState));
 }
 
 int engine_add(Engine* e, mgpcg_solver* s, int64_t tag, const SolveParams& prm) {
@@ -610,7 +670,9 @@ int engine_build_items(Engine* e) {
     MG_HIP(hipMalloc(&e->d_items, sizeof(Item) * ncap));
     for (int i = 0; i < Engine::RING; ++i) {
       if (e->h_items[i]) MG_HIP(hipHostFree(e->h_items[i]));
-      MG_HIP(hipHostMalloc((void**)&e->h_items[i], sizeof(Item) * ncap, 0));
+      MG_HIP(hipHostMalloc((void**)&e->h_items[i], sizeof(Item) * ncap,
+                           hipHostMallocMapped | hipHostMallocCoherent));
+      MG_HIP(hipHostGetDevicePointer((void**)&e->hd_items[i], e->h_items[i], 0));
     }
     e->n_items_cap = ncap;
   }
@@ -619,9 +681,11 @@ int engine_build_items(Engine* e) {
   MG_HIP(hipEventSynchronize(e->ev_items[sl]));
   if (n > 0) {
     memcpy(e->h_items[sl], it.data(), sizeof(Item) * n);
-    MG_HIP(hipMemcpyAsync(e->d_items, e->h_items[sl], sizeof(Item) * n, hipMemcpyHostToDevice,
-                          e->stream));
+    if (engine_pull(e, e->d_items, e->hd_items[sl], sizeof(Item) * n)) return -2;
   }
+  // every queued table copy (this layout's items, the admissions' descriptors
+  // and states) lands before the kernels that read them
+  if (engine_flush_pulls(e)) return -2;
   MG_HIP(hipEventRecord(e->ev_items[sl], e->stream));
   e->dirty = false;
   e->epoch++;
@@ -849,9 +913,7 @@ struct Sched {
     const Engine::StageRef& g = e->st_gather;
     if (g.n == 0) return;
     hipLaunchKernelGGL((k_gather_states<T, TX>), dim3((g.n + 63) / 64), dim3(64), 0, e->stream,
-                       sys(e), (const Item*)(e->d_items + g.off), g.n, e->d_states);
-    (void)hipMemcpyAsync(e->h_states[slot], e->d_states, sizeof(CGState) * g.n,
-                         hipMemcpyDeviceToHost, e->stream);
+                       sys(e), (const Item*)(e->d_items + g.off), g.n, e->hd_states[slot]);
   }
 };
 
@@ -1311,6 +1373,7 @@ extern "C" int mgpcg_destroy(mgpcg_solver_t* s) {
     (void)hipFree(s->pool);
   }
   if (s->ev_retired) (void)hipEventDestroy(s->ev_retired);
+  if (s->ev_submitted) (void)hipEventDestroy(s->ev_submitted);
   delete s;
   return 0;
 }
@@ -2136,6 +2199,9 @@ extern "C" int mgpcg_server_submit(mgpcg_server_t* sv, mgpcg_solver_t* s, int64_
       MG_REQUIRE(!(M.active && M.s == s), "system is already being solved");
   for (const auto& pr : sv->pending)
     MG_REQUIRE(pr.first != s, "system is already submitted");
+  MG_HIP(hipSetDevice(s->device));
+  if (!s->ev_submitted) MG_HIP(hipEventCreateWithFlags(&s->ev_submitted, hipEventDisableTiming));
+  MG_HIP(hipEventRecord(s->ev_submitted, s->user_stream));
   sv->pending.push_back({s, tag});
   return 0;
 }
@@ -2146,21 +2212,38 @@ extern "C" int mgpcg_server_step(mgpcg_server_t* sv, int* n_active) {
   // admit submitted systems at this iteration boundary, each into the lane
   // with the fewest members
   std::vector<int> count(nl);
+  int running = 0;
+  for (int i = 0; i < nl; ++i) running += (count[i] = engine_active(sv->lanes[i].e));
+  // While systems are running, a submitted system joins once its assembly has
+  // finished on the device: a lane made to wait for a running assembly would
+  // hold up every member meanwhile.  With none running, all join at once (the
+  // lane waits for their assemblies, which may run side by side on their own
+  // streams).
+  const bool wait_all = running == 0;
   std::vector<char> begun(nl, 0);
-  for (int i = 0; i < nl; ++i) count[i] = engine_active(sv->lanes[i].e);
+  decltype(sv->pending) later;
   while (!sv->pending.empty()) {
     auto pr = sv->pending.front();
     sv->pending.pop_front();
+    if (!wait_all) {
+      const hipError_t q = hipEventQuery(pr.first->ev_submitted);
+      if (q == hipErrorNotReady) {
+        later.push_back(pr);
+        continue;
+      }
+      MG_HIP(q);
+    }
     int li = 0;
     for (int i = 1; i < nl; ++i)
       if (count[i] < count[li]) li = i;
     Engine* e = sv->lanes[li].e;
-    // newly submitted systems were assembled on the caller's stream: order
-    // the lane after it (only then, so assembly and iterations overlap otherwise)
+    // the lane also follows the server's stream (the caller's readbacks of the
+    // systems' previous results come before their new iterations)
     if (!begun[li]) {
       if (engine_begin(e)) return -2;
       begun[li] = 1;
     }
+    MG_HIP(hipStreamWaitEvent(e->stream, pr.first->ev_submitted, 0));
     // free member slots of retired systems are reused
     for (Member& M : e->mem)
       if (!M.active) M.s = nullptr;
@@ -2169,6 +2252,7 @@ extern "C" int mgpcg_server_step(mgpcg_server_t* sv, int* n_active) {
     if (engine_add(e, pr.first, pr.second, prm) < 0) return -2;
     ++count[li];
   }
+  sv->pending = std::move(later);
   // every lane's iteration is enqueued before the host waits for any readback
   for (mgpcg_server::Lane& ln : sv->lanes) {
     if (engine_active(ln.e) == 0) continue;

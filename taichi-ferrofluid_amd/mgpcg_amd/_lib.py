@@ -169,12 +169,26 @@ def int3(vals):
     return (ctypes.c_int32 * 3)(*v)
 
 
+_dev_index = {}
+
+
 def stream_handle(device=None):
     """torch's current HIP stream of `device` (NULL for CPU tensors, which only
     the CPU emulation build accepts)."""
-    if device is not None and torch.device(device).type == "cpu":
+    idx = _dev_index.get(device)
+    if idx is None:
+        d = torch.device(device) if device is not None else None
+        if d is not None and d.type == "cpu":
+            idx = -1
+        else:
+            idx = d.index if d is not None and d.index is not None else torch.cuda.current_device()
+        if device is not None:  # the current device may change between calls
+            _dev_index[device] = idx
+    if idx < 0:
         return ctypes.c_void_p(None)
-    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    # the raw stream of torch's current stream on that device (torch.cuda.current_stream
+    # without building a Stream object)
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(idx))
 
 
 def synchronize(device):

@@ -4,10 +4,12 @@ A request is a scene (scenes.py) to assemble and solve.  The loop:
   * admits queued requests while the HBM budget allows, onto a resident
     hierarchy of the request's shape (hierarchies are kept on the device and
     reused; a shape that does not fit evicts idle hierarchies of other shapes);
-  * assembles the admitted system on the caller's stream (strategy kernels;
-    pressure systems straight into the packed hierarchy unless
-    ``packed=False``) while the engine's stream iterates the systems already
-    running, so the two overlap on the device;
+  * assembles the admitted system (strategy kernels; pressure systems
+    straight into the packed hierarchy unless ``packed=False``) on one of
+    ``assembly_streams`` side streams in turn, while the engine's stream
+    iterates the systems already running: assemblies overlap each other and
+    the iterations, and a system joins the running ones at the first
+    iteration boundary after its assembly finished;
   * calls mgpcg_server_step: the new systems join at the next iteration
     boundary, one iteration of every member is enqueued, and members the
     device reports done (one iteration behind) retire;
@@ -31,7 +33,8 @@ from . import systems
 class Server:
     def __init__(self, hbm_budget_bytes, precision="mixed", n_levels=5, nu=2, bottom=10,
                  max_iters=500, rel_tol=1e-12, abs_tol=1e-14, eps=1e-12, refine_every=10,
-                 max_active=None, admit_per_step=None, packed=True, device="cuda:0"):
+                 max_active=None, admit_per_step=None, packed=True, assembly_streams=8,
+                 device="cuda:0"):
         self.device = torch.device(device)
         self.budget = int(hbm_budget_bytes)
         self.precision = precision
@@ -43,6 +46,10 @@ class Server:
         # finished together re-enters staggered
         self.admit_per_step = admit_per_step
         self.packed = packed
+        # default-priority side streams for assemblies (none for the CPU emulation)
+        self._asm = ([torch.cuda.Stream(device=self.device) for _ in range(assembly_streams)]
+                     if self.device.type == "cuda" and assembly_streams > 0 else [])
+        self._asm_next = 0
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         _lib.check(self._lib.mgpcg_server_create(int(max_iters), float(rel_tol), float(abs_tol),
@@ -109,6 +116,7 @@ class Server:
 
     def _admit(self):
         admitted = 0
+        ready = None  # the server's stream so far: scene uploads, the last readbacks
         while self.queue:
             if self.max_active is not None and len(self.running) >= self.max_active:
                 break
@@ -122,14 +130,30 @@ class Server:
                     raise RuntimeError("request does not fit the HBM budget even alone")
                 break
             self.queue.popleft()
-            systems.assemble(s, req["scene"], packed=self.packed)
-            s.load()
-            _lib.check(self._lib.mgpcg_set_stream(s._h, _lib.stream_handle(self.device)))
-            _lib.check(self._lib.mgpcg_server_submit(self._h, s._h, req["tag"]))
+            if self._asm:
+                # the assembly reads the scene, which was uploaded on the
+                # server's stream (one event per admission round)
+                if ready is None:
+                    ready = torch.cuda.Event()
+                    ready.record(torch.cuda.current_stream(self.device))
+                st = self._asm[self._asm_next]
+                self._asm_next = (self._asm_next + 1) % len(self._asm)
+                st.wait_event(ready)
+                with torch.cuda.stream(st):
+                    self._assemble(s, req)
+            else:
+                self._assemble(s, req)
             req["admitted"] = time.perf_counter()
             self.running[req["tag"]] = (s, req)
             admitted += 1
         return admitted
+
+    def _assemble(self, s, req):
+        """Assemble and submit on torch's current stream."""
+        systems.assemble(s, req["scene"], packed=self.packed)
+        s.load()
+        _lib.check(self._lib.mgpcg_set_stream(s._h, _lib.stream_handle(self.device)))
+        _lib.check(self._lib.mgpcg_server_submit(self._h, s._h, req["tag"]))
 
     def step(self):
         """Admit, advance every member one iteration, collect finished results."""

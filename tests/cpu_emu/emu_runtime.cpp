@@ -426,6 +426,10 @@ hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
   *p = hipemu::guarded_alloc(bytes);
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
+hipError_t hipHostGetDevicePointer(void** devPtr, void* hostPtr, unsigned) {
+  *devPtr = hostPtr;  // one address space
+  return hipSuccess;
+}
 hipError_t hipHostFree(void* p) {
   hipemu::drain_all();
   hipemu::guarded_free(p);

@@ -120,6 +120,7 @@ inline T __shfl(T v, int src, int width = 64) {
   return hipemu::shfl_idx(v, src, width);
 }
 inline void __threadfence() {}
+inline void __threadfence_system() {}
 // the emulator runs every lane on one host thread, so a plain update is atomic
 inline unsigned atomicAdd(unsigned* p, unsigned v) {
   unsigned o = *p;
@@ -191,6 +192,9 @@ hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKin
                           hipStream_t s);
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned flags);
 hipError_t hipHostFree(void* p);
+hipError_t hipHostGetDevicePointer(void** devPtr, void* hostPtr, unsigned flags);
+#define hipHostMallocMapped 0x2
+#define hipHostMallocCoherent 0x40000000
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned flags);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);
