@@ -2755,7 +2755,7 @@ __device__ __forceinline__ void pack_slot(const SlotCoefs<DIM>& v, int c, int t,
 
 // End of a packing workgroup: its block's FLUID flag and natural-order work
 // entry, and its share of the level's summary in `part`, a structure of
-// arrays over the nb = gridDim.x workgroups: [0, nb) the smallest and
+// arrays over the level's nb workgroups (b: this one): [0, nb) the smallest and
 // [nb, 2 nb) the largest nonzero upper-coupling bit pattern, [2 nb, 3 nb) the
 // FLUID cells.  k_meta_reduce folds them into the level's LevelMeta.  (One
 // device-scope atomic per workgroup on the level's counters instead costs a
@@ -2763,7 +2763,7 @@ __device__ __forceinline__ void pack_slot(const SlotCoefs<DIM>& v, int c, int t,
 // workgroups serialise.)  Called by the whole workgroup.
 __device__ __forceinline__ void pack_block_tail(uint32_t mn, uint32_t mx, uint32_t wet,
                                                 uint32_t nfl, uint8_t* wet_out, int32_t* ord0,
-                                                uint32_t* part) {
+                                                uint32_t* part, unsigned b, unsigned nb) {
   __shared__ uint32_t smn[BLOCK / 64], smx[BLOCK / 64], swet[BLOCK / 64], snfl[BLOCK / 64];
   mn = wave_min_u32(mn);
   mx = wave_max_u32(mx);
@@ -2784,7 +2784,6 @@ __device__ __forceinline__ void pack_block_tail(uint32_t mn, uint32_t mx, uint32
       wet = swet[w] > wet ? swet[w] : wet;
       nfl += snfl[w];
     }
-    const unsigned nb = gridDim.x, b = blockIdx.x;
     part[b] = mn;
     part[nb + b] = mx;
     part[2 * nb + b] = nfl;
@@ -2802,9 +2801,8 @@ struct MetaReduceList {
   int nblocks[16];
 };
 constexpr int META_THREADS = 1024;
-__global__ void __launch_bounds__(META_THREADS) k_meta_reduce(MetaReduceList ml) {
+__device__ __forceinline__ void meta_reduce_level(const MetaReduceList& ml, int l) {
   __shared__ uint32_t smn[META_THREADS / 64], smx[META_THREADS / 64], snf[META_THREADS / 64];
-  const int l = blockIdx.x;
   const int nb = ml.nblocks[l];
   const uint32_t* p = ml.part[l];
   uint32_t mn = 0xffffffffu, mx = 0u, nf = 0u;
@@ -2834,6 +2832,9 @@ __global__ void __launch_bounds__(META_THREADS) k_meta_reduce(MetaReduceList ml)
     ml.meta[l]->maxbits = mx;
     ml.meta[l]->nfluid = nf;
   }
+}
+__global__ void __launch_bounds__(META_THREADS) k_meta_reduce(MetaReduceList ml) {
+  meta_reduce_level(ml, blockIdx.x);
 }
 
 // Packs one level from the canonical arrays (the strategies' output).
@@ -2875,7 +2876,8 @@ __global__ void __launch_bounds__(BLOCK) k_pack_level(Geo g, const int8_t* __res
 #pragma unroll
     for (int c = 0; c < 2; ++c) pack_slot<T, DIM>(v[c], c, t, P, mn, mx, wet, nfl);
   }
-  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], part);
+  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], part, blockIdx.x,
+                  gridDim.x);
 }
 
 // The pressure operator of one level assembled straight into the packed
@@ -2893,6 +2895,80 @@ __device__ __forceinline__ float gfm_weight(float phi_i, float phi_j, float c_ma
   return phi_i == 0.0f ? c_max : fminf(c, c_max);
 }
 
+// One slot (both colours) of the pressure operator packed straight into the
+// level's arrays (see k_pack_pressure); folds its coupling extremes and FLUID
+// count into mn / mx / wet / nfl.
+template <typename T, int DIM>
+__device__ __forceinline__ void pack_pressure_slot(const Geo& g, const int8_t* __restrict__ gt,
+                                                   const float* __restrict__ phi, float sA,
+                                                   float c_max, const LevelPtrs<T>& P, int t,
+                                                   uint32_t& mn, uint32_t& mx, uint32_t& wet,
+                                                   uint32_t& nfl) {
+  SlotCoefs<DIM> v[2];
+  int8_t nbt[2][DIM][2];
+  float nbp[2][DIM][2], ph[2];
+  // every load of both colours first (out-of-range neighbours read the
+  // cell itself and count as SOLID)
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const Slot sl = decode(g, t, c);
+    v[c].inn = sl.interior;
+    const long ci = sl.interior ? canon_index(g, sl.i, sl.j, sl.k) : 0;
+    const int x[3] = {sl.i, sl.j, sl.k};
+    const int nn[3] = {g.n0, g.n1, g.n2};
+    v[c].gv = gt[ci];
+    ph[c] = phi ? phi[ci] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const int va = k + 3 - DIM;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        int w[3] = {sl.i, sl.j, sl.k};
+        w[va] += h ? 1 : -1;
+        const bool in = sl.interior && w[va] >= 0 && w[va] < nn[va];
+        const long wi = in ? canon_index(g, w[0], w[1], w[2]) : ci;
+        const int8_t tv = gt[wi];
+        nbt[c][k][h] = in ? tv : (int8_t)MG_SOLID;
+        nbp[c][k][h] = phi ? phi[wi] : 0.0f;
+      }
+      v[c].hdn[k] = sl.interior && x[va] > 0;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    float ad = 0.0f, up[3] = {0.0f, 0.0f, 0.0f};
+    const bool fl = v[c].inn && v[c].gv == MG_FLUID;
+    if (fl) {
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {  // sgn = -1, then +1
+          const int nb = nbt[c][k][h];
+          if (nb == MG_FLUID) {
+            ad = ad + sA;
+            if (h) up[k] = -sA;
+          } else if (nb == MG_AIR) {
+            if (phi != nullptr) {
+              const float cw = gfm_weight(ph[c], nbp[c][k][h], c_max);
+              ad = ad + sA * cw;
+            } else {
+              ad = ad + sA;
+            }
+          }
+        }
+      }
+    }
+    v[c].ad = ad;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      v[c].up[k] = up[k];
+      v[c].dn[k] = (fl && nbt[c][k][0] == MG_FLUID) ? -sA : 0.0f;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) pack_slot<T, DIM>(v[c], c, t, P, mn, mx, wet, nfl);
+}
+
 template <typename T, int DIM>
 __global__ void __launch_bounds__(BLOCK) k_pack_pressure(Geo g, const int8_t* __restrict__ gt,
                                                          const float* __restrict__ phi,
@@ -2900,72 +2976,9 @@ __global__ void __launch_bounds__(BLOCK) k_pack_pressure(Geo g, const int8_t* __
                                                          uint32_t* part) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
-  if (t < g.size) {
-    SlotCoefs<DIM> v[2];
-    int8_t nbt[2][DIM][2];
-    float nbp[2][DIM][2], ph[2];
-    // every load of both colours first (out-of-range neighbours read the
-    // cell itself and count as SOLID)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const Slot sl = decode(g, t, c);
-      v[c].inn = sl.interior;
-      const long ci = sl.interior ? canon_index(g, sl.i, sl.j, sl.k) : 0;
-      const int x[3] = {sl.i, sl.j, sl.k};
-      const int nn[3] = {g.n0, g.n1, g.n2};
-      v[c].gv = gt[ci];
-      ph[c] = phi ? phi[ci] : 0.0f;
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        const int va = k + 3 - DIM;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          int w[3] = {sl.i, sl.j, sl.k};
-          w[va] += h ? 1 : -1;
-          const bool in = sl.interior && w[va] >= 0 && w[va] < nn[va];
-          const long wi = in ? canon_index(g, w[0], w[1], w[2]) : ci;
-          const int8_t tv = gt[wi];
-          nbt[c][k][h] = in ? tv : (int8_t)MG_SOLID;
-          nbp[c][k][h] = phi ? phi[wi] : 0.0f;
-        }
-        v[c].hdn[k] = sl.interior && x[va] > 0;
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      float ad = 0.0f, up[3] = {0.0f, 0.0f, 0.0f};
-      const bool fl = v[c].inn && v[c].gv == MG_FLUID;
-      if (fl) {
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {  // sgn = -1, then +1
-            const int nb = nbt[c][k][h];
-            if (nb == MG_FLUID) {
-              ad = ad + sA;
-              if (h) up[k] = -sA;
-            } else if (nb == MG_AIR) {
-              if (phi != nullptr) {
-                const float cw = gfm_weight(ph[c], nbp[c][k][h], c_max);
-                ad = ad + sA * cw;
-              } else {
-                ad = ad + sA;
-              }
-            }
-          }
-        }
-      }
-      v[c].ad = ad;
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        v[c].up[k] = up[k];
-        v[c].dn[k] = (fl && nbt[c][k][0] == MG_FLUID) ? -sA : 0.0f;
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) pack_slot<T, DIM>(v[c], c, t, P, mn, mx, wet, nfl);
-  }
-  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], part);
+  if (t < g.size) pack_pressure_slot<T, DIM>(g, gt, phi, sA, c_max, P, t, mn, mx, wet, nfl);
+  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], part, blockIdx.x,
+                  gridDim.x);
 }
 
 // The pressure right-hand side straight into the packed b (both colours of a
@@ -2976,15 +2989,13 @@ __global__ void __launch_bounds__(BLOCK) k_pack_pressure(Geo g, const int8_t* __
 // phi is given) -- then, when add_c, the volume-control constant c added on
 // FLUID cells (volume_control.py:33-37).  u[k]: MAC component k, shape
 // res + e_k (canonical).
+// b of one slot (both colours) packed straight into level 0 (see k_pack_pressure_b)
 template <typename T, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_pack_pressure_b(Geo g, const int8_t* __restrict__ gt,
-                                                           MacVel V,
-                                                           const float* __restrict__ phi,
-                                                           float sA, float sb, float p0,
-                                                           float c_max, int add_c, float cadd,
-                                                           T* b0, T* b1) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= g.size) return;
+__device__ __forceinline__ void pack_pressure_b_slot(const Geo& g, const int8_t* __restrict__ gt,
+                                                     const MacVel& V,
+                                                     const float* __restrict__ phi, float sA,
+                                                     float sb, float p0, float c_max, int add_c,
+                                                     float cadd, T* b0, T* b1, int t) {
   float out[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
@@ -3059,6 +3070,58 @@ __global__ void __launch_bounds__(BLOCK) k_pack_pressure_b(Geo g, const int8_t* 
   b1[t] = (T)out[1];
 }
 
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_pack_pressure_b(Geo g, const int8_t* __restrict__ gt,
+                                                           MacVel V,
+                                                           const float* __restrict__ phi,
+                                                           float sA, float sb, float p0,
+                                                           float c_max, int add_c, float cadd,
+                                                           T* b0, T* b1) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t < g.size)
+    pack_pressure_b_slot<T, DIM>(g, gt, V, phi, sA, sb, p0, c_max, add_c, cadd, b0, b1, t);
+}
+
+// A whole pressure hierarchy packed in one launch: the workgroups of level l
+// are [block0[l], block0[l + 1]); level 0's also write b.  The per-level
+// geometry and packed arrays come from the handle's table `tab`, the rest
+// (cell types, the level set and the velocity) from the call.
+template <typename T>
+struct PackLevel {
+  Geo g;
+  LevelPtrs<T> P;
+  uint32_t* part;
+};
+struct PackCall {
+  const int8_t* gt[16];  // canonical cell types of each level
+  int block0[17];
+  int nl;
+  MacVel V;
+  const float* phi;  // ghost-fluid weights at level 0 (NULL: none)
+  void* b[2];        // level 0's packed b, both colours
+  float sA, sb, p0, c_max, cadd;
+  int add_c;
+};
+template <typename T, int DIM>
+__global__ void __launch_bounds__(BLOCK) k_pack_pressure_all(const PackLevel<T>* __restrict__ tab,
+                                                             PackCall c) {
+  int l = 0;
+  while ((int)blockIdx.x >= c.block0[l + 1]) ++l;
+  const PackLevel<T>& L = tab[l];
+  const int b = (int)blockIdx.x - c.block0[l];
+  const int t = b * BLOCK + threadIdx.x;
+  uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
+  if (t < L.g.size) {
+    const float* ph = l == 0 ? c.phi : nullptr;  // ghost-fluid weights at level 0 only
+    if (l == 0)
+      pack_pressure_b_slot<T, DIM>(L.g, c.gt[0], c.V, c.phi, c.sA, c.sb, c.p0, c.c_max, c.add_c,
+                                   c.cadd, (T*)c.b[0], (T*)c.b[1], t);
+    pack_pressure_slot<T, DIM>(L.g, c.gt[l], ph, c.sA, c.c_max, L.P, t, mn, mx, wet, nfl);
+  }
+  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)L.P.wet, (int32_t*)L.P.ord[0], L.part, b,
+                  c.block0[l + 1] - c.block0[l]);
+}
+
 template <typename S, typename D>
 __global__ void __launch_bounds__(BLOCK) k_pack_vec(Geo g, const S* __restrict__ src, D* d0,
                                                     D* d1) {
@@ -3097,9 +3160,8 @@ __global__ void __launch_bounds__(BLOCK) k_unpack_vec(Geo g, const S* __restrict
 // a multiple of a plane's length gives some workgroups only wet blocks and
 // others none.)  One workgroup; `tmp` holds n ints.
 constexpr int ORDER_THREADS = 1024;
-__global__ void __launch_bounds__(ORDER_THREADS) k_level_order(const uint8_t* __restrict__ wet,
-                                                               int n1, int32_t* ord,
-                                                               int32_t* tmp) {
+__device__ __forceinline__ void level_order_body(const uint8_t* __restrict__ wet, int n1,
+                                                 int32_t* ord, int32_t* tmp) {
   __shared__ int sc[ORDER_THREADS];
   const int tid = threadIdx.x;
   constexpr int span = SLOT_LANES;
@@ -3139,6 +3201,12 @@ __global__ void __launch_bounds__(ORDER_THREADS) k_level_order(const uint8_t* __
     ord[k] = m1 > m ? (tmp[m] | ORD_WET) : tmp[W + (k - m)];
   }
 }
+__global__ void __launch_bounds__(ORDER_THREADS) k_level_order(const uint8_t* __restrict__ wet,
+                                                               int n1, int32_t* ord,
+                                                               int32_t* tmp) {
+  level_order_body(wet, n1, ord, tmp);
+}
+static_assert(ORDER_THREADS == META_THREADS, "k_finish_levels runs both bodies");
 
 // init_gridtype (mgpcg.py:46-60) on canonical int8 arrays; bit-exact.
 __global__ void __launch_bounds__(BLOCK) k_coarsen(int dim, int f0, int f1, int f2,
@@ -3163,6 +3231,17 @@ __global__ void __launch_bounds__(BLOCK) k_coarsen(int dim, int f0, int f1, int 
         else if (attr == MG_FLUID) tot_fluid++;
       }
   coarse[t] = tot_air > 0 ? MG_AIR : (tot_fluid > 0 ? MG_FLUID : MG_SOLID);
+}
+
+// The end of a packing: workgroup l < ml_levels folds level l's packing
+// partials into its LevelMeta (meta_reduce_level); one more workgroup, when
+// `ord` is given, writes level 0's half-sweep work order (level_order_body).
+__global__ void __launch_bounds__(META_THREADS) k_finish_levels(MetaReduceList ml, int ml_levels,
+                                                                const uint8_t* __restrict__ wet,
+                                                                int n1, int32_t* ord,
+                                                                int32_t* tmp) {
+  if ((int)blockIdx.x < ml_levels) meta_reduce_level(ml, blockIdx.x);
+  else level_order_body(wet, n1, ord, tmp);
 }
 
 }  // namespace mg

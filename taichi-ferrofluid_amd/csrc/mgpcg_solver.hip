@@ -93,6 +93,7 @@ struct mgpcg_solver {
   int coarse_from = 0;
   char* pool = nullptr;
   size_t pool_bytes = 0;
+  void* packtab = nullptr;  // PackLevel<T>[L] of k_pack_pressure_all (in the pool)
   hipStream_t user_stream = nullptr;
   Engine* engine = nullptr;  // private engine of the single-system API
   int device = 0;
@@ -1184,14 +1185,14 @@ int solve_single(mgpcg_solver* s, int max_iters, double rel_tol, double abs_tol,
 // order per level (both colours of gt, Adiag, Ax per axis, r, z with its zero
 // pads, flag words, residual scratch), then the level's metadata, block flags,
 // work orders and packing partials; after the levels, level 0's b, x, both p, Ap, the CG state
-// and the reduction partials.
+// the reduction partials and the packing table.
 struct PoolPlan {
   struct Offs {
     size_t gt[2], Ad[2], Ax[3][2], r[2], z[2], nb[2], res[2], meta, wet, ord[2], ordtmp, part;
   };
   std::vector<Geo> geo;
   std::vector<Offs> lo;
-  size_t ob[2], ox[2], op[2][2], oap[2], ost, opart;
+  size_t ob[2], ox[2], op[2][2], oap[2], ost, opart, opack;
   int nparts;
   size_t bytes;
 };
@@ -1239,6 +1240,9 @@ void plan_pool(int dim, const int32_t* res, int n_levels, size_t esz, size_t xsz
   P->ost = take(sizeof(CGState));
   P->nparts = nblk(g0.size);
   P->opart = take(sizeof(DD) * P->nparts);
+  // the fused packing kernel's per-level table (same size for either precision)
+  static_assert(sizeof(PackLevel<float>) == sizeof(PackLevel<double>), "");
+  P->opack = take(sizeof(PackLevel<double>) * (size_t)n_levels);
   P->bytes = off;
 }
 
@@ -1347,6 +1351,21 @@ extern "C" int mgpcg_create(int dim, const int32_t* res, int n_levels, int nu, i
   }
   s->st = (CGState*)(s->pool + ost);
   s->partials = (DD*)(s->pool + opart);
+  s->packtab = s->pool + P.opack;
+  {
+    std::vector<char> tab(sizeof(PackLevel<double>) * (size_t)n_levels);
+    for (int l = 0; l < n_levels; ++l) {
+      if (precision == F64) {
+        const PackLevel<double> pl{s->geo[l], level_ptrs<double>(s, l), s->lv[l].part};
+        memcpy(tab.data() + sizeof(pl) * l, &pl, sizeof(pl));
+      } else {
+        const PackLevel<float> pl{s->geo[l], level_ptrs<float>(s, l), s->lv[l].part};
+        memcpy(tab.data() + sizeof(pl) * l, &pl, sizeof(pl));
+      }
+    }
+    if ((e = hipMemcpy(s->packtab, tab.data(), tab.size(), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(e, "hipMemcpy(packing table)");
+  }
   if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "hipDeviceSynchronize");
   *out = s;
   return 0;
@@ -1598,44 +1617,52 @@ int assemble_pressure_t(mgpcg_solver* s, int8_t* const* gt, const float* const* 
                         const float* phi, float sA, float sb, float p0, int gfm, float c_max,
                         int add_c, float cadd) {
   hipStream_t st = s->user_stream;
-  MacVel V;
-  for (int k = 0; k < 3; ++k) V.u[k] = k < s->dim ? velocity[k] : nullptr;
   const Geo& g0 = s->geo[0];
-  const float* ph = gfm ? phi : nullptr;
-  if (s->dim == 3)
-    hipLaunchKernelGGL((k_pack_pressure_b<T, 3>), dim3(nblk(g0.size)), dim3(BLOCK), 0, st, g0,
-                       gt[0], V, ph, sA, sb, p0, c_max, add_c, cadd, (T*)s->b[0], (T*)s->b[1]);
-  else
-    hipLaunchKernelGGL((k_pack_pressure_b<T, 2>), dim3(nblk(g0.size)), dim3(BLOCK), 0, st, g0,
-                       gt[0], V, ph, sA, sb, p0, c_max, add_c, cadd, (T*)s->b[0], (T*)s->b[1]);
-  for (int l = 0; l < s->L; ++l) {
+  const int L = s->L;
+  // the coarse levels' cell types, level by level (init_gridtype, mgpcg.py:46-60)
+  for (int l = 1; l < L; ++l) {
     const Geo& g = s->geo[l];
-    if (l > 0) {
-      const Geo& gf = s->geo[l - 1];
-      const int nc = (s->dim == 3 ? g.n0 : 1) * g.n1 * g.n2;
-      hipLaunchKernelGGL(k_coarsen, dim3(nblk(nc)), dim3(BLOCK), 0, st, s->dim,
-                         s->dim == 3 ? gf.n0 : 1, gf.n1, gf.n2, gt[l - 1], gt[l], nc);
-    }
-    const float* phl = l == 0 ? ph : nullptr;  // ghost-fluid weights at level 0 only
-    if (s->dim == 3)
-      hipLaunchKernelGGL((k_pack_pressure<T, 3>), dim3(nblk(g.size)), dim3(BLOCK), 0, st, g,
-                         gt[l], phl, sA, c_max, level_ptrs<T>(s, l), s->lv[l].part);
-    else
-      hipLaunchKernelGGL((k_pack_pressure<T, 2>), dim3(nblk(g.size)), dim3(BLOCK), 0, st, g,
-                         gt[l], phl, sA, c_max, level_ptrs<T>(s, l), s->lv[l].part);
+    const Geo& gf = s->geo[l - 1];
+    const int nc = (s->dim == 3 ? g.n0 : 1) * g.n1 * g.n2;
+    hipLaunchKernelGGL(k_coarsen, dim3(nblk(nc)), dim3(BLOCK), 0, st, s->dim,
+                       s->dim == 3 ? gf.n0 : 1, gf.n1, gf.n2, gt[l - 1], gt[l], nc);
   }
+  // b and every level's operator, flags and block records: one launch
+  PackCall c;
+  c.nl = L;
+  c.block0[0] = 0;
+  for (int l = 0; l < L; ++l) {
+    c.gt[l] = gt[l];
+    c.block0[l + 1] = c.block0[l] + nblk(s->geo[l].size);
+  }
+  for (int k = 0; k < 3; ++k) c.V.u[k] = k < s->dim ? velocity[k] : nullptr;
+  c.phi = gfm ? phi : nullptr;
+  c.b[0] = s->b[0];
+  c.b[1] = s->b[1];
+  c.sA = sA;
+  c.sb = sb;
+  c.p0 = p0;
+  c.c_max = c_max;
+  c.cadd = cadd;
+  c.add_c = add_c;
+  const PackLevel<T>* tab = (const PackLevel<T>*)s->packtab;
+  if (s->dim == 3)
+    hipLaunchKernelGGL((k_pack_pressure_all<T, 3>), dim3(c.block0[L]), dim3(BLOCK), 0, st, tab,
+                       c);
+  else
+    hipLaunchKernelGGL((k_pack_pressure_all<T, 2>), dim3(c.block0[L]), dim3(BLOCK), 0, st, tab,
+                       c);
+  // every level's record, and level 0's half-sweep order: one launch
   MetaReduceList ml;
-  for (int l = 0; l < s->L; ++l) {
+  for (int l = 0; l < L; ++l) {
     ml.meta[l] = s->lv[l].meta;
     ml.part[l] = s->lv[l].part;
     ml.nblocks[l] = nblk(s->geo[l].size);
   }
-  hipLaunchKernelGGL(k_meta_reduce, dim3(s->L), dim3(META_THREADS), 0, st, ml);
-  if (s->coarse_from > 0) {
-    const LevelBuf& B = s->lv[0];
-    hipLaunchKernelGGL(k_level_order, dim3(1), dim3(ORDER_THREADS), 0, st, B.wet,
-                       nblk(g0.size), B.ord[1], B.ordtmp);
-  }
+  const LevelBuf& B = s->lv[0];
+  const bool order = s->coarse_from > 0;
+  hipLaunchKernelGGL(k_finish_levels, dim3(L + (order ? 1 : 0)), dim3(META_THREADS), 0, st, ml,
+                     L, B.wet, nblk(g0.size), order ? B.ord[1] : nullptr, B.ordtmp);
   MG_HIP(hipGetLastError());
   return 0;
 }

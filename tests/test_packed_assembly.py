@@ -93,6 +93,18 @@ def test_ragged_shapes(dev, res):
     _same_solve(a, b)
 
 
+@pytest.mark.parametrize("res,L", [([32, 32, 32], 6), ([64, 96], 6), ([128, 64], 7)])
+def test_deep_hierarchies(dev, res, L):
+    """Hierarchies of 6 and 7 levels, 3D and 2D: one packing launch over all
+    of them."""
+    from mgpcg_amd import scenes
+
+    sysd = scenes.pool_sphere_system(res, seed=13, L=L)
+    a, b = _pair(dev, sysd, "mixed")
+    _same_hierarchy(a, b)
+    _same_solve(a, b)
+
+
 def test_seeded_volume_control(dev):
     from mgpcg_amd import scenes
 
