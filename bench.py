@@ -61,13 +61,14 @@ SMOOTH_FLUID_BYTES_PER_CELL_3D = SMOOTH_BYTES_PER_CELL_3D - SMOOTH_TYPE_BYTES_PE
 XR_FLUID_BYTES_PER_CELL = 34.0
 XR_TYPE_BYTES_PER_CELL = 1.0
 # The kernel moves fewer bytes than that model: on a level with uniform
-# couplings it reads a 16-bit flag word (couplings and diagonal) instead of Ax
-# and Adiag (7.5 B per cell: own r, type and flag word 7 / 2, other-colour z
-# 4 / 2, own z written 4 / 2), and it skips blocks of 256 slots without a
-# FLUID cell.  Its physical HBM bytes per cell are measured with rocprofv3
-# counters (the committed traffic summary) and reported beside the model's.
-# On a dense system (every block FLUID) it moves 7.5 B per cell.
-SMOOTH_BYTES_PER_CELL_3D_DENSE = 7.5
+# couplings it reads a 16-bit flag word (couplings, diagonal and the FLUID
+# bit) instead of Ax, Adiag and the cell type (7 B per cell: own r and flag
+# word 6 / 2, other-colour z 4 / 2, own z written 4 / 2), and it skips blocks
+# of 256 slots without a FLUID cell.  Its physical HBM bytes per cell are
+# measured with rocprofv3 counters (the committed traffic summary) and reported
+# beside the model's.  On a dense system (every block FLUID) it moves 7 B per
+# cell.
+SMOOTH_BYTES_PER_CELL_3D_DENSE = 7.0
 
 
 def parse():
@@ -327,7 +328,7 @@ def measure_dense(args, dev, n=256, levels=6, reps=100):
     own = SMOOTH_BYTES_PER_CELL_3D_DENSE * cells / launch_s / 1e9
     del s
     # On this system the kernel reads 16-bit flag words where the reference
-    # model reads FP32 couplings and diagonals: its own 7.5 B per cell are the
+    # model reads FP32 couplings and diagonals: its own 7 B per cell are the
     # bytes it moves, and the frac; the 20.5 B model's figure is beside it.
     return {"system": f"{n}^3, every interior cell FLUID, uniform couplings, plain-sum diagonals",
             "kernel": "mg::k_stage<float, 3, 0, 1>", "launch_us": round(ms * 1e3, 3),

@@ -57,10 +57,13 @@ struct LevelMeta {
 // nonzero (Ax[I][a]), bit NB_DN + a = coupling to the lower one (Ax[I - e_a][a]),
 // bits NB_AD.. = the diagonal's code: n < NB_AD_STORED when Adiag equals n
 // couplings of the slot's magnitude added one at a time from 0 (diag_of_code),
-// NB_AD_STORED when it is read from the Adiag array
+// NB_AD_STORED when it is read from the Adiag array; bit NB_FL = the slot is
+// FLUID (the half-sweeps on uniform-coupling levels read it instead of the
+// cell type)
 constexpr int NB_DN = 3;
 constexpr int NB_AD = 6;
 constexpr uint32_t NB_AD_STORED = 7u;
+constexpr int NB_FL = 9;
 
 template <typename T>
 struct LevelPtrs {
@@ -550,7 +553,7 @@ template <typename T, int DIM, bool COMPACT>
 __device__ __forceinline__ void smooth_load(const Geo& g, const LevelPtrs<T>& P, int c, int t,
                                             SmoothVals<T, DIM, COMPACT>& v) {
   const int q = row_q(g, t, c);
-  v.gt = ldgi(P.gt[c], t);
+  if constexpr (!COMPACT) v.gt = ldgi(P.gt[c], t);
   v.rv = ldgi(P.r[c], t);
   if constexpr (COMPACT) {
     // a z colour array has g.zpad zero slots before and after it, so the
@@ -564,6 +567,7 @@ __device__ __forceinline__ void smooth_load(const Geo& g, const LevelPtrs<T>& P,
       v.zh[a] = ldgi(zb, n.hi[a] + g.zpad);
     }
     v.f = ldgi(P.nb[c], t);
+    v.gt = ((v.f >> NB_FL) & 1u) ? MG_FLUID : MG_SOLID;  // FLUID or not is all that is read
   } else {
     const Nbr<DIM> n = neighbours_clamped<DIM>(g, t, q);
     v.ad = ldgi(P.Adiag[c], t);
@@ -584,10 +588,10 @@ __device__ __forceinline__ void smooth_load(const Geo& g, const LevelPtrs<T>& P,
 // keeps a slot's loaded values live here (see MG_KEEP)
 template <typename T, int DIM, bool COMPACT>
 __device__ __forceinline__ void smooth_keep(const SmoothVals<T, DIM, COMPACT>& v) {
-  MG_KEEP(v.gt);
   if constexpr (COMPACT) {
     MG_KEEP(v.f);
   } else {
+    MG_KEEP(v.gt);
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
       MG_KEEP(v.al[a]);
@@ -2743,6 +2747,7 @@ __device__ __forceinline__ void pack_slot(const SlotCoefs<DIM>& v, int c, int t,
       break;
     }
   }
+  if (gvc == MG_FLUID) f |= 1u << NB_FL;
   ((uint16_t*)P.nb[c])[t] = (uint16_t)(f | (code << NB_AD));
   P.r[c][t] = (T)0;
   P.z[c][t] = (T)0;

@@ -8,6 +8,7 @@ writes are recomputed here from the canonical arrays and compared exactly:
 - bits 6..8: the smallest n < 7 for which -c added n times to 0 in FP32
   reproduces Adiag bit for bit, c being the slot's first nonzero coupling
   (upper then lower, axis by axis; n = 0 needs no coupling), else 7;
+- bit 9: the slot is FLUID;
 - one flag per 256 slots: 1 when a slot of either colour is FLUID;
 - the work orders: per block of 256 slots the natural order with the wet
   bits; per half-sweep virtual block of 512 slots (level 0, on the batched
@@ -76,8 +77,8 @@ def _expected(gt, Ad, Ax):
                     if _bits(_fold(n, cl)) == _bits(Ad[i, j, k]):
                         code = n
                         break
-                flags[c, t] = f | (code << 6)
                 fluid[c, t] = gt[i, j, k] == FLUID
+                flags[c, t] = f | (code << 6) | (int(fluid[c, t]) << 9)
     nb = (size + 255) // 256
     pad = np.zeros((2, nb * 256), bool)
     pad[:, :size] = fluid
