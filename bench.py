@@ -85,7 +85,7 @@ def parse():
     ap.add_argument("--precision", default="mixed", choices=["plain", "mixed", "f64"])
     ap.add_argument("--admit-per-step", type=int, default=0,
                     help="admissions per server step (0: no limit)")
-    ap.add_argument("--server-depth", type=int, default=2,
+    ap.add_argument("--server-depth", type=int, default=1,
                     help="iterations the server keeps queued on the device ahead of the host")
     ap.add_argument("--server-streams", type=int, default=1,
                     help="engine streams the server spreads the requests in flight over")
