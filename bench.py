@@ -698,16 +698,21 @@ def main():
         if args.server_streams > 1:
             # the same kernel with no other lane's kernels beside it on the
             # device: the headline's launches share it with the second lane
+            # warm-up of 2 x concurrency requests: the new server's first
+            # requests create their hierarchies (mgpcg_create synchronizes)
             iso = measure_served(args, wl, n, L, args.seeds, args.concurrency,
-                                 max(16, args.steps // 4), 4, dev, streams=1)
+                                 max(32, args.steps // 2), 2 * args.concurrency, dev,
+                                 streams=1)
             iso.pop("scenes")
             line["roofline_isolated"] = dict(iso["roofline"] or {}, server_streams=1,
                                              solves_per_s=round(iso["value"], 3))
         else:
             # the same requests spread over two engine streams: one lane's
             # latency-bound kernels beside the other's level-0 kernels
+            # (warm-up of 2 x concurrency requests, as above)
             two = measure_served(args, wl, n, L, args.seeds, args.concurrency,
-                                 max(16, args.steps // 2), 4, dev, streams=2)
+                                 max(32, args.steps // 2), 2 * args.concurrency, dev,
+                                 streams=2)
             two.pop("scenes")
             line["two_lanes"] = {"solves_per_s": round(two["value"], 3),
                                  "p50_ms": round(two["lat"]["p50_ms"], 3),
