@@ -2901,8 +2901,8 @@ __device__ __forceinline__ float gfm_weight(float phi_i, float phi_j, float c_ma
 }
 
 // One slot (both colours) of the pressure operator packed straight into the
-// level's arrays (see k_pack_pressure); folds its coupling extremes and FLUID
-// count into mn / mx / wet / nfl.
+// level's arrays (run by k_pack_pressure_all); folds its coupling extremes and
+// FLUID count into mn / mx / wet / nfl.
 template <typename T, int DIM>
 __device__ __forceinline__ void pack_pressure_slot(const Geo& g, const int8_t* __restrict__ gt,
                                                    const float* __restrict__ phi, float sA,
@@ -2974,18 +2974,6 @@ __device__ __forceinline__ void pack_pressure_slot(const Geo& g, const int8_t* _
   for (int c = 0; c < 2; ++c) pack_slot<T, DIM>(v[c], c, t, P, mn, mx, wet, nfl);
 }
 
-template <typename T, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_pack_pressure(Geo g, const int8_t* __restrict__ gt,
-                                                         const float* __restrict__ phi,
-                                                         float sA, float c_max, LevelPtrs<T> P,
-                                                         uint32_t* part) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
-  uint32_t mn = 0xffffffffu, mx = 0u, wet = 0u, nfl = 0u;
-  if (t < g.size) pack_pressure_slot<T, DIM>(g, gt, phi, sA, c_max, P, t, mn, mx, wet, nfl);
-  pack_block_tail(mn, mx, wet, nfl, (uint8_t*)P.wet, (int32_t*)P.ord[0], part, blockIdx.x,
-                  gridDim.x);
-}
-
 // The pressure right-hand side straight into the packed b (both colours of a
 // slot per thread): PressureProjectStrategy.build_b_kernel
 // (pressure_project.py:15-39) in the operation order of the canonical
@@ -2993,8 +2981,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack_pressure(Geo g, const int8_t* __
 // velocity term, AIR faces adding sA * p0 (times the ghost-fluid weight when
 // phi is given) -- then, when add_c, the volume-control constant c added on
 // FLUID cells (volume_control.py:33-37).  u[k]: MAC component k, shape
-// res + e_k (canonical).
-// b of one slot (both colours) packed straight into level 0 (see k_pack_pressure_b)
+// res + e_k (canonical).  Run by k_pack_pressure_all on level 0's workgroups.
 template <typename T, int DIM>
 __device__ __forceinline__ void pack_pressure_b_slot(const Geo& g, const int8_t* __restrict__ gt,
                                                      const MacVel& V,
@@ -3073,18 +3060,6 @@ __device__ __forceinline__ void pack_pressure_b_slot(const Geo& g, const int8_t*
   }
   b0[t] = (T)out[0];
   b1[t] = (T)out[1];
-}
-
-template <typename T, int DIM>
-__global__ void __launch_bounds__(BLOCK) k_pack_pressure_b(Geo g, const int8_t* __restrict__ gt,
-                                                           MacVel V,
-                                                           const float* __restrict__ phi,
-                                                           float sA, float sb, float p0,
-                                                           float c_max, int add_c, float cadd,
-                                                           T* b0, T* b1) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t < g.size)
-    pack_pressure_b_slot<T, DIM>(g, gt, V, phi, sA, sb, p0, c_max, add_c, cadd, b0, b1, t);
 }
 
 // A whole pressure hierarchy packed in one launch: the workgroups of level l

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -70,6 +71,7 @@ int g_restrict_coarse = -1;  // restriction kernels on levels >= 1 (-1: g_restri
 int g_probe_every = 4;    // the launch probe times one eligible launch in this many
 int g_server_depth = 2;   // server iterations queued on the device ahead of the host
 int g_server_streams = 2; // engines (streams) a server spreads its members over
+constexpr int MAX_LANES = 4;  // most engines per server (the lane stream pool's size)
 int g_graph = 0;          // replay iterations of an unchanged member set from a hipGraph
 long g_knob_epoch = 0;    // bumped by every option change (captured graphs go stale)
 
@@ -104,6 +106,7 @@ struct mgpcg_solver {
   // engine, waits for this event before it resets the state.
   hipEvent_t ev_retired = nullptr;
   bool retired_pending = false;
+  const Engine* retired_on = nullptr;  // the engine that recorded ev_retired
   // Recorded on the system's stream when it is submitted to a server: its
   // assembly is complete when this event is.
   hipEvent_t ev_submitted = nullptr;
@@ -223,6 +226,7 @@ struct Member {
 struct Engine {
   int precision = -1, dim = 0, L = 0, nu = 0, bottom = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: a server lane's stream from the process-wide pool
   hipStream_t user_stream = nullptr;
   hipEvent_t ev_user = nullptr, ev_done = nullptr;
   int device = 0;
@@ -337,7 +341,7 @@ struct Engine {
     d_swept = nullptr;
     if (ev_user) (void)hipEventDestroy(ev_user);
     if (ev_done) (void)hipEventDestroy(ev_done);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
     d_levels = d_sys = nullptr;
     d_items = nullptr;
     d_states = nullptr;
@@ -347,9 +351,15 @@ struct Engine {
 
 namespace {
 
-int engine_init(Engine* e) {
+// pooled: a stream the engine uses without owning it (NULL: its own stream)
+int engine_init(Engine* e, hipStream_t pooled) {
   MG_HIP(hipGetDevice(&e->device));
-  MG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  if (pooled) {
+    e->stream = pooled;
+    e->own_stream = false;
+  } else {
+    MG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  }
   MG_HIP(hipEventCreateWithFlags(&e->ev_user, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&e->ev_done, hipEventDisableTiming));
   for (int i = 0; i < Engine::RING; ++i)
@@ -549,9 +559,10 @@ int engine_add(Engine* e, mgpcg_solver* s, int64_t tag, const SolveParams& prm) 
   m.started = e->iteration;
   m.gen += 1;
   // iterations still queued where the system last ran read its state; the
-  // reset below must come after them
+  // reset below must come after them (on the same engine the stream's order
+  // already places it there)
   if (s->retired_pending) {
-    MG_HIP(hipStreamWaitEvent(e->stream, s->ev_retired, 0));
+    if (s->retired_on != e) MG_HIP(hipStreamWaitEvent(e->stream, s->ev_retired, 0));
     s->retired_pending = false;
   }
   if (engine_write_desc(e, slot)) return -2;
@@ -1080,6 +1091,7 @@ int engine_collect(Engine* e, int slot, std::vector<Finished>* done) {
       if (!s->ev_retired) MG_HIP(hipEventCreateWithFlags(&s->ev_retired, hipEventDisableTiming));
       MG_HIP(hipEventRecord(s->ev_retired, e->stream));
       s->retired_pending = true;
+      s->retired_on = e;
       if (done) done->push_back({g[i].first, st});
     }
   }
@@ -1123,10 +1135,10 @@ void engine_clear_retired(Engine* e) {
   e->dirty = true;
 }
 
-Engine* new_engine() {
+Engine* new_engine(hipStream_t pooled = nullptr) {
   Engine* e = new (std::nothrow) Engine();
   if (!e) return nullptr;
-  if (engine_init(e)) {
+  if (engine_init(e, pooled)) {
     delete e;
     return nullptr;
   }
@@ -1449,7 +1461,8 @@ extern "C" int mgpcg_set_option(mgpcg_solver_t* s, int option, int value) {
       g_server_depth = value;
       return 0;
     case MGPCG_OPT_SERVER_STREAMS:
-      MG_REQUIRE(value >= 1 && value <= 4, "server streams must be in [1, 4]");
+      MG_REQUIRE(value >= 1 && value <= MAX_LANES, "server streams must be in [1, %d]",
+                 MAX_LANES);
       g_server_streams = value;
       return 0;
     case MGPCG_OPT_GRAPH:
@@ -2160,6 +2173,41 @@ extern "C" int mgpcg_batch_solve(mgpcg_batch_t* b, mgpcg_solver_t* const* system
   return engine_end(e);
 }
 
+// Server lanes run on streams from a per-device pool, created together the
+// first time a server on that device is made and kept until the process ends.
+// The runtime maps each stream onto one of a few hardware queues when the
+// stream is created, and two lanes on one queue run one after the other:
+// lanes created afresh for each server were measured sharing one queue in a
+// process that had made a server before (two lanes then ran at two thirds of
+// one lane's rate), while streams created together went to distinct queues.
+// Lane i of every server uses pool stream i, so servers alive at the same time
+// share lane streams (correct, but their iterations then take turns).
+namespace {
+int lane_stream(int lane, hipStream_t* out) {
+  static std::mutex mu;
+  static std::vector<std::vector<hipStream_t>> pool;  // [device][lane]
+  int dev = 0;
+  MG_HIP(hipGetDevice(&dev));
+  MG_REQUIRE(lane >= 0 && lane < MAX_LANES, "lane %d outside [0, %d)", lane, MAX_LANES);
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)pool.size() <= dev) pool.resize(dev + 1);
+  if (pool[dev].empty()) {
+    std::vector<hipStream_t> v(MAX_LANES, nullptr);
+    for (hipStream_t& st : v) {
+      const hipError_t rc = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+      if (rc != hipSuccess) {
+        for (hipStream_t x : v)
+          if (x) (void)hipStreamDestroy(x);
+        MG_HIP(rc);
+      }
+    }
+    pool[dev] = v;
+  }
+  *out = pool[dev][lane];
+  return 0;
+}
+}  // namespace
+
 // The server: continuous batching.  Systems are submitted at any time; each
 // call of mgpcg_server_step admits the submitted ones (they join at the next
 // iteration boundary), enqueues one iteration for every member and retires
@@ -2192,8 +2240,14 @@ extern "C" int mgpcg_server_create(int max_iters, double rel_tol, double abs_tol
   mgpcg_server* sv = new (std::nothrow) mgpcg_server();
   MG_REQUIRE(sv != nullptr, "out of host memory");
   sv->lanes.resize(g_server_streams);
-  for (mgpcg_server::Lane& ln : sv->lanes) {
-    ln.e = new_engine();
+  for (int i = 0; i < (int)sv->lanes.size(); ++i) {
+    mgpcg_server::Lane& ln = sv->lanes[i];
+    hipStream_t st = nullptr;
+    if (lane_stream(i, &st)) {
+      delete sv;
+      return -2;
+    }
+    ln.e = new_engine(st);
     if (!ln.e) {
       delete sv;
       return -2;
@@ -2260,9 +2314,15 @@ extern "C" int mgpcg_server_step(mgpcg_server_t* sv, int* n_active) {
       }
       MG_HIP(q);
     }
+    // the lane with the fewest members; among those, the one the system last
+    // ran on (its queued iterations there need no wait across lanes)
     int li = 0;
     for (int i = 1; i < nl; ++i)
       if (count[i] < count[li]) li = i;
+    for (int i = 0; i < nl; ++i)
+      if (count[i] == count[li] && pr.first->retired_pending &&
+          pr.first->retired_on == sv->lanes[i].e)
+        li = i;
     Engine* e = sv->lanes[li].e;
     // the lane also follows the server's stream (the caller's readbacks of the
     // systems' previous results come before their new iterations)
