@@ -2423,13 +2423,22 @@ __global__ void __launch_bounds__(BLOCK) k_dot_zr(const SysDesc<T, TX>* __restri
 // ---------------------------------------------------------------------------
 enum FinOp { FIN_INIT = 0, FIN_ALPHA = 1, FIN_RR = 2, FIN_TRUE = 3, FIN_BETA = 4 };
 
+// out (FIN_BETA only, may be NULL): the host's mapped readback slot; item i's
+// state after the step goes to out[i], done or not.
 template <typename T, typename TX, int OP>
 __global__ void __launch_bounds__(BLOCK) k_finalize(const SysDesc<T, TX>* __restrict__ sys,
-                                                    const Item* __restrict__ items, int nitems) {
+                                                    const Item* __restrict__ items, int nitems,
+                                                    CGState* out) {
   if ((int)blockIdx.x >= nitems) return;
   const SysDesc<T, TX>& D = sys[items[blockIdx.x].desc];
   CGState* st = D.st;
-  if (st->done) return;
+  if (st->done) {
+    if (OP == FIN_BETA && out != nullptr && threadIdx.x == 0) {
+      out[blockIdx.x] = *st;
+      __threadfence_system();
+    }
+    return;
+  }
   if (OP == FIN_TRUE && !st->pending_true) return;
   // four independent accumulators: four partials per thread in flight
   DD v0 = dd_zero(), v1 = dd_zero(), v2 = dd_zero(), v3 = dd_zero();
@@ -2520,6 +2529,10 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const SysDesc<T, TX>* __rest
         st->done = 1;
         st->status = MAX_ITERS;
       }
+    }
+    if (out != nullptr) {
+      out[blockIdx.x] = *st;
+      __threadfence_system();
     }
   }
 }

@@ -268,6 +268,10 @@ struct Engine {
   // the same pinned slots as the device addresses them: the gather kernel
   // writes the states there itself (no copy dispatched after it)
   CGState* hd_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
+  // Set while an eager iteration is enqueued: the readback slot FIN_BETA's
+  // finalize writes (its items are the gather's, in the same order).  NULL
+  // during graph capture, whose replays are followed by k_gather_states.
+  CGState* fused_gather = nullptr;
   hipEvent_t ev_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<std::pair<int, uint32_t>> gathered[NSLOT];  // (member slot, generation)
   long iteration = 0;
@@ -782,11 +786,12 @@ struct Sched {
                          levels(e), stage_of(e, r));
     }
   }
+  // out: FIN_BETA's readback slot (see Engine::fused_gather)
   template <int OP>
-  static void fin(Engine* e, const Engine::StageRef& r) {
+  static void fin(Engine* e, const Engine::StageRef& r, CGState* out = nullptr) {
     if (r.n == 0) return;
     hipLaunchKernelGGL((k_finalize<T, TX, OP>), dim3(r.n), dim3(BLOCK), 0, e->stream, sys(e),
-                       (const Item*)(e->d_items + r.off), r.n);
+                       (const Item*)(e->d_items + r.off), r.n, out);
   }
 
   // v_cycle (mgpcg.py:125-148) for every member.  first0: level 0's first
@@ -917,7 +922,7 @@ struct Sched {
       sync();
     }
     vcycle(e, true);
-    fin<FIN_BETA>(e, e->fin_all);
+    fin<FIN_BETA>(e, e->fin_all, e->fused_gather);
     sync();
   }
 
@@ -1048,11 +1053,14 @@ int engine_enqueue(Engine* e, int slot, bool sync_each) {
   if (!e->graph_retired.empty() && engine_graph_reap(e, false)) return -2;
   if (g_graph && !sync_each && !e->probe_on && e->epoch_eager > 0) {
     if (engine_graph_launch(e)) return -2;
+    dispatch<RunGather>(e->precision, e->dim, e, slot);
   } else {
+    // the iteration's last finalize writes the readback slot itself
+    e->fused_gather = e->hd_states[slot];
     dispatch<RunIteration>(e->precision, e->dim, e, sync_each);
+    e->fused_gather = nullptr;
     e->epoch_eager++;
   }
-  dispatch<RunGather>(e->precision, e->dim, e, slot);
   e->gathered[slot].clear();
   for (int m = 0; m < (int)e->mem.size(); ++m)
     if (e->mem[m].active) e->gathered[slot].push_back({m, e->mem[m].gen});
