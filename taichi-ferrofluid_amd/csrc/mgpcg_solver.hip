@@ -434,10 +434,17 @@ int engine_reserve(Engine* e, int cap) {
   e->d_sys = ns;
   if (e->d_states) MG_HIP(hipFree(e->d_states));
   MG_HIP(hipMalloc(&e->d_states, sizeof(CGState) * (size_t)ncap));
+  // readback slots of iterations still in flight (written by now: the stream
+  // was synchronized above) keep their states for the collects to come
   for (int i = 0; i < Engine::NSLOT; ++i) {
-    if (e->h_states[i]) MG_HIP(hipHostFree(e->h_states[i]));
-    MG_HIP(hipHostMalloc((void**)&e->h_states[i], sizeof(CGState) * (size_t)ncap,
+    CGState* h = nullptr;
+    MG_HIP(hipHostMalloc((void**)&h, sizeof(CGState) * (size_t)ncap,
                          hipHostMallocMapped | hipHostMallocCoherent));
+    if (e->h_states[i]) {
+      memcpy(h, e->h_states[i], sizeof(CGState) * (size_t)e->cap);
+      MG_HIP(hipHostFree(e->h_states[i]));
+    }
+    e->h_states[i] = h;
     MG_HIP(hipHostGetDevicePointer((void**)&e->hd_states[i], e->h_states[i], 0));
   }
   const size_t dbytes = e->level_sz * e->L + e->sys_sz;

@@ -317,6 +317,57 @@ def test_server_continuous_batching_equals_alone(dev, depth, streams):
         assert np.array_equal(xa, rq["x"])
 
 
+@pytest.mark.parametrize("depth", [1, 2])
+def test_server_table_growth_keeps_queued_readbacks(dev, depth):
+    """A lane whose member tables grow (a ninth member joins) while iterations
+    are still queued keeps those iterations' readbacks: growing reallocates
+    the readback slots, and the slots of queued iterations must carry their
+    states over, else the host reads whatever the new allocation held (the CPU
+    emulation fills new allocations with a nonzero pattern, which reads as a
+    finished system).  Every result must equal solving alone."""
+    from mgpcg_amd import serve
+    from mgpcg_amd import systems
+
+    set_knobs(server_depth=depth, server_streams=1)
+    try:
+        emu = _emu(dev)
+        lo, hi = (16, 24) if emu else (32, 64)
+        reqs = [scenes.c5_request(2000 + i, lo=lo, hi=hi, L=3) for i in range(10)]
+        devs = [systems.upload(r, dev) for r in reqs]
+        srv = serve.Server(hbm_budget_bytes=4 << 30, precision="mixed", n_levels=3, device=dev)
+        done = []
+
+        def step():
+            for rq in srv.step():
+                rq["x"] = _x(rq["solver"])
+                srv.retire(rq)
+                done.append(rq)
+
+        for i in range(3):
+            srv.submit(devs[i], meta=i)
+        step()
+        step()
+        assert len(srv.running) == 3 and not done  # running, iterations queued
+        for i in range(3, len(reqs)):
+            srv.submit(devs[i], meta=i)
+        steps = 0
+        while srv.busy():
+            steps += 1
+            assert steps < 2000
+            step()
+        assert srv.max_members >= 9  # the lane's tables grew while it ran
+        assert len(done) == len(reqs)
+        for rq in done:
+            sysd = reqs[rq["meta"]]
+            print(f"case {rq['meta']} {sysd['name']}", flush=True)
+            xa, ra, _ = gpu_solve(sysd, dev, "mixed")
+            assert rq["result"]["status"] == ra["status"], rq["result"]
+            assert ra["iters"] == rq["result"]["iters"]
+            assert np.array_equal(xa, rq["x"])
+    finally:
+        set_knobs(server_depth=2, server_streams=2)
+
+
 @pytest.fixture(params=[1, 2], ids=["newest_first", "round_robin"])
 def deferred_streams(dev, request):
     """On the CPU emulation: the library's streams queue their work and run it
