@@ -24,7 +24,14 @@ Also reported, on the same requests:
                                 measured on the device (first workgroup start
                                 to last workgroup end) during the timed steps;
   roofline_isolated             the same with one engine stream, where no
-                                other lane's kernels share the device;
+                                other lane's kernels share the device (when
+                                the headline runs more than one);
+  two_lanes                     the headline workload over two engine streams
+                                (when the headline runs one);
+  c3_256                        the 256^3 workload, 2 in flight, one lane,
+                                with `lanes`: the same over --c3-c5-streams;
+  c4_batch64, c5_poisson        64 mixed systems in one batch; Poisson
+                                arrivals at 4 rates on --c3-c5-streams lanes;
   roofline_dense                the same kernel on a dense 256^3 system (every
                                 block has FLUID cells): its bandwidth when it
                                 cannot skip work;
@@ -87,6 +94,8 @@ def parse():
                     help="admissions per server step (0: no limit)")
     ap.add_argument("--server-depth", type=int, default=1,
                     help="iterations the server keeps queued on the device ahead of the host")
+    ap.add_argument("--c3-c5-streams", dest="c5_streams", type=int, default=2,
+                    help="engine lanes of the C5 server and of C3's second measurement")
     ap.add_argument("--server-streams", type=int, default=1,
                     help="engine streams the server spreads the requests in flight over")
     ap.add_argument("--canonical-assembly", action="store_true",
@@ -386,6 +395,7 @@ def measure_batch64(args, dev):
 def measure_poisson(args, dev, rates=(50, 200, 1000, 2000), per_rate=48, pool=16):
     """Open-loop serving: requests arrive at Poisson times with rate lambda,
     sides log-uniform in [64, 256] per axis (pool + sphere scenes)."""
+    from mgpcg_amd import _lib
     from mgpcg_amd import scenes
     from mgpcg_amd import serve
     from mgpcg_amd import systems
@@ -394,7 +404,10 @@ def measure_poisson(args, dev, rates=(50, 200, 1000, 2000), per_rate=48, pool=16
     L = 5
     scs = [systems.upload(scenes.c5_request(1000 + i, L=L), dev) for i in range(pool)]
     torch.cuda.synchronize()
+    # the lane count is read when the server is made
+    _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_SERVER_STREAMS, args.c5_streams))
     srv = serve.Server(96 << 30, precision=args.precision, n_levels=L, device=dev)
+    _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_SERVER_STREAMS, args.server_streams))
     serve.closed_loop(srv, lambda i: scs[i % pool], pool, 8)  # resident hierarchies, warm
     out = []
     for lam in rates:
@@ -404,7 +417,8 @@ def measure_poisson(args, dev, rates=(50, 200, 1000, 2000), per_rate=48, pool=16
         st = serve.latency_stats(recs)
         out.append({"lambda": lam, "solves_per_s": round(len(recs) / secs, 3),
                     "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-                    "converged": len(recs), "max_members": srv.max_members})
+                    "converged": len(recs), "max_members": srv.max_members,
+                    "server_streams": args.c5_streams})
     del srv
     return out
 
@@ -766,6 +780,16 @@ def main():
                               "error_vs_fp64": fp64_error(args, big_scenes[0], c3["n"],
                                                           c3["levels"], dev)}
             del big_scenes
+            # one system per lane: one lane's coarse levels beside the other's level 0
+            # (its roofline launch shares the device, so the line above keeps its own)
+            if args.c5_streams != args.server_streams:
+                big2 = measure_served(args, "c3", c3["n"], c3["levels"], c3["seeds"],
+                                      c3["concurrency"], 8, 2, dev, streams=args.c5_streams)
+                big2.pop("scenes")
+                line["c3_256"]["lanes"] = {"server_streams": args.c5_streams,
+                                           "solves_per_s": round(big2["value"], 3),
+                                           "p50_ms": round(big2["lat"]["p50_ms"], 3),
+                                           "p99_ms": round(big2["lat"]["p99_ms"], 3)}
             line["roofline_dense"] = measure_dense(args, dev)
             if not args.no_mix:
                 line["c4_batch64"] = measure_batch64(args, dev)
