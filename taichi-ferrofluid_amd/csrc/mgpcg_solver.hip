@@ -106,7 +106,10 @@ struct mgpcg_solver {
   // engine, waits for this event before it resets the state.
   hipEvent_t ev_retired = nullptr;
   bool retired_pending = false;
-  const Engine* retired_on = nullptr;  // the engine that recorded ev_retired
+  // The engine that recorded ev_retired; only compared, never followed.  An
+  // engine synchronizes its stream before it is destroyed, so a stale match
+  // (a new engine at a freed one's address) skips a wait that is complete.
+  const Engine* retired_on = nullptr;
   // Recorded on the system's stream when it is submitted to a server: its
   // assembly is complete when this event is.
   hipEvent_t ev_submitted = nullptr;
