@@ -268,8 +268,9 @@ struct Engine {
   // state readbacks in flight (a ring: the server keeps several iterations queued)
   static const int NSLOT = 4;
   CGState* h_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
-  // the same pinned slots as the device addresses them: the gather kernel
-  // writes the states there itself (no copy dispatched after it)
+  // the same pinned slots as the device addresses them: the iteration's last
+  // finalize (after a graph replay, the gather kernel) writes the states there
+  // itself (no copy dispatched after it)
   CGState* hd_states[NSLOT] = {nullptr, nullptr, nullptr, nullptr};
   // Set while an eager iteration is enqueued: the readback slot FIN_BETA's
   // finalize writes (its items are the gather's, in the same order).  NULL
