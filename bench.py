@@ -94,6 +94,9 @@ def parse():
                     help="admissions per server step (0: no limit)")
     ap.add_argument("--server-depth", type=int, default=1,
                     help="iterations the server keeps queued on the device ahead of the host")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="time only the level-0 half-sweep on the dense 256^3 system "
+                         "(after the calibration copies) and print its line")
     ap.add_argument("--c3-c5-streams", dest="c5_streams", type=int, default=2,
                     help="engine lanes of the C5 server and of C3's second measurement")
     ap.add_argument("--server-streams", type=int, default=1,
@@ -614,11 +617,94 @@ def traffic_summary(fetch_db, write_db, fetch_log, write_log, command):
     return "\n".join(out)
 
 
+def dense_summary(fetch_db, write_db, trace_db, log):
+    """Text summary of three rocprofv3 runs of `bench.py --dense-only` (a kernel
+    trace, then FETCH_SIZE, then WRITE_SIZE in KiB per dispatch): the dense
+    256^3 half-sweep's HBM bytes per cell (each counter scaled by its reading
+    for the known-size calibration copies, 4-byte accesses) and its rocprofv3
+    durations, beside the line's HIP-event launch time."""
+    import sqlite3
+
+    line = None
+    with open(log) as f:
+        for ln in f:
+            ln = ln.strip()
+            if ln.startswith("{") and '"roofline_dense"' in ln:
+                line = json.loads(ln)
+    assert line is not None, "no --dense-only line in " + log
+    rd = line["roofline_dense"]
+    kname = "k_stage<float, 3, 0, 1>"
+
+    def counter(db, name):
+        con = sqlite3.connect(db)
+        return con.execute("select name, counter_value from pmc_events where counter_name = ? "
+                           "order by dispatch_id", (name,)).fetchall()
+
+    res = {}
+    kib_cal = line["copy_calibration_mib"] * 1024.0
+    for ctr, db in (("FETCH_SIZE", fetch_db), ("WRITE_SIZE", write_db)):
+        rows = counter(db, ctr)
+        cal_s = [v for n_, v in rows if "CUDAFunctorOnSelf_add" in n_][:3]
+        k = [v for n_, v in rows if kname in n_]
+        assert cal_s and k, (ctr, len(cal_s), len(k))
+        res[ctr] = {"scale": kib_cal / (sum(cal_s) / len(cal_s)), "n": len(k),
+                    "kib": sum(k) / len(k)}
+    con = sqlite3.connect(trace_db)
+    durs = [(b - a) / 1e3 for name, a, b in
+            con.execute("select name, start, end from kernels order by start").fetchall()
+            if kname in name]
+    assert durs
+    avg_us = sum(durs) / len(durs)
+    cells = rd["cells_per_launch"]
+    fetched = res["FETCH_SIZE"]["kib"] * 1024 * res["FETCH_SIZE"]["scale"]
+    written = res["WRITE_SIZE"]["kib"] * 1024 * res["WRITE_SIZE"]["scale"]
+    own = rd["bytes_per_cell"] * cells
+    out = []
+    w = out.append
+    w(f"# HBM traffic and duration of the level-0 half-sweep on the dense system ({rd['system']})")
+    w("")
+    w("three runs of the same command, from the repository root on the GPU box after")
+    w("`cd /tmp && export TMPDIR=/tmp` in the shell:")
+    w("")
+    w("    rocprofv3 --kernel-trace --stats -d gpurun_out/dense_trace -o dense -- "
+      "python3 bench.py --dense-only")
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        w(f"    timeout -s KILL 300 rocprofv3 --pmc {ctr} -d gpurun_out/dense_{ctr} -o dense -- "
+          f"python3 bench.py --dense-only")
+    w("")
+    w(f"Every dispatch of mg::{kname} in that process is the dense system's half-sweep")
+    w("smooth(0, 0) (the warm-up and the timed launches of its HIP-event timing).")
+    w("")
+    w(f"calibration copies: {line['copy_calibration_mib']} MiB, 4-byte accesses: FETCH_SIZE scale "
+      f"{res['FETCH_SIZE']['scale']:.4f}, WRITE_SIZE scale {res['WRITE_SIZE']['scale']:.4f}")
+    w("")
+    w(f"kernel mg::{kname}, {cells} level-0 cells (one colour swept per launch):")
+    w(f"  dispatches (trace / fetch run / write run)  {len(durs)} / {res['FETCH_SIZE']['n']} / "
+      f"{res['WRITE_SIZE']['n']}")
+    w(f"  rocprofv3 average duration                  {avg_us:.3f} us")
+    w(f"  HIP-event launch time of the bench line     {rd['launch_us']} us (back-to-back launches)")
+    w(f"  fetched per cell, scaled                    {fetched / cells:.3f} B")
+    w(f"  written per cell, scaled                    {written / cells:.3f} B")
+    w(f"  traffic per cell                            {(fetched + written) / cells:.3f} B "
+      f"(the kernel's own model: {rd['bytes_per_cell']} B)")
+    tb = (fetched + written) / (avg_us * 1e-6) / 1e12
+    w(f"  traffic / rocprofv3 duration                {tb:.3f} TB/s, frac {tb * 1e3 / HBM_PEAK_GBS:.4f}")
+    ob = own / (avg_us * 1e-6) / 1e12
+    w(f"  own bytes / rocprofv3 duration              {ob:.3f} TB/s, frac {ob * 1e3 / HBM_PEAK_GBS:.4f}")
+    w(f"  own bytes / HIP-event launch time           {rd['achieved'] / 1e3:.3f} TB/s, "
+      f"frac {rd['frac']}")
+    return "\n".join(out)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--traffic-summary":
         # bench.py --traffic-summary <fetch db> <write db> <fetch log> <write log> <bench args>
         print(traffic_summary(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5],
                               " ".join(sys.argv[6:])))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--dense-summary":
+        # bench.py --dense-summary <fetch db> <write db> <trace db> <dense-only log>
+        print(dense_summary(*sys.argv[2:6]))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--profile-summary":
         # bench.py --profile-summary <rocpd db> <file with the bench JSON line> <command>
@@ -656,6 +742,10 @@ def main():
     _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_PROBE_EVERY, args.probe_every))
     _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_SERVER_DEPTH, args.server_depth))
     _lib.check(_lib.load().mgpcg_set_option(None, _lib.OPT_SERVER_STREAMS, args.server_streams))
+    if args.dense_only:
+        print(json.dumps({"roofline_dense": measure_dense(args, dev),
+                          "copy_calibration_mib": args.copy_calibration_mib}), flush=True)
+        return
     wl, n, L = args.workload, args.n, args.levels
     what = WORKLOADS[wl]["what"]
 

@@ -2009,16 +2009,17 @@ int prepare_op(Engine* e, const mgpcg_solver* s, int op, int level, Engine::Stag
   return 0;
 }
 
-// Launch one stage on the prepared one-item table.
+// Launch one stage on the prepared one-item table.  level0: a half-sweep of
+// level 0 runs the level-0 instantiation, as in a solve.
 template <typename T, typename TX, int DIM>
 struct RunOp {
-  static int run(Engine* e, int op, const Engine::StageRef& r, int arg) {
+  static int run(Engine* e, int op, const Engine::StageRef& r, int arg, bool level0) {
     using Sd = Sched<T, TX, DIM>;
     const int nb = r.nblocks;
     switch (op) {
       case MGPCG_OP_SMOOTH:
-        if (arg & 1) Sd::template stage<ST_SMOOTH1>(e, r);
-        else Sd::template stage<ST_SMOOTH0>(e, r);
+        if (arg & 1) Sd::template stage<ST_SMOOTH1>(e, r, level0);
+        else Sd::template stage<ST_SMOOTH0>(e, r, level0);
         return 0;
       case MGPCG_OP_SMOOTH_PAIR_FROM_ZERO:
         Sd::template stage<ST_PAIR_FROM_ZERO>(e, r);
@@ -2071,7 +2072,7 @@ extern "C" int mgpcg_debug_op(mgpcg_solver_t* s, int op, int level, int arg) {
   } else {
     Engine::StageRef r;
     if (prepare_op(e, s, op, level, &r)) return -2;
-    int rc = dispatch<RunOp>(e->precision, e->dim, e, op, r, arg);
+    int rc = dispatch<RunOp>(e->precision, e->dim, e, op, r, arg, level == 0);
     if (rc) return rc;
   }
   MG_HIP(hipGetLastError());
@@ -2106,7 +2107,7 @@ extern "C" int mgpcg_time_op(mgpcg_solver_t* s, int op, int level, int arg, int 
     if (op == MGPCG_OP_VCYCLE) {
       dispatch<RunVcycle>(e->precision, e->dim, e);
     } else {
-      rc = dispatch<RunOp>(e->precision, e->dim, e, op, r, arg);
+      rc = dispatch<RunOp>(e->precision, e->dim, e, op, r, arg, level == 0);
       if (rc) return rc;
     }
   }
